@@ -1,0 +1,1019 @@
+/*
+ * bssn_oracle.c -- CPU restatement of the AMSS-NCKU BSSN RHS + RK4 path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and
+ * the CPU-baseline leg of the benchmark load this library; the product path
+ * never does.
+ *
+ * Every routine evaluates, point by point, the same arithmetic the reference
+ * Fortran evaluates array statement by array statement, in the same
+ * operation order (build with -ffp-contract=off), so the restatement can be
+ * compared with the reference build to the last bit.  Citations are into
+ * amss-ncku-python/AMSS_NCKU_source/.
+ */
+#include "bssn_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NF 24
+#define IX(ex, i, j, k) ((size_t)(i) + (size_t)(ex)[0] * ((size_t)(j) + (size_t)(ex)[1] * (size_t)(k)))
+
+/* Parity of each evolved field under x, y, z reflection (bssn_class.C:183-361). */
+static const double PAR[NF][3] = {
+    {1, 1, 1},   {1, 1, 1},   {1, 1, 1},   {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1},
+    {1, -1, -1}, {1, 1, 1},   {1, 1, 1},   {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1},
+    {1, -1, -1}, {1, 1, 1},   {-1, 1, 1},  {1, -1, 1},  {1, 1, -1},  {1, 1, 1},
+    {-1, 1, 1},  {1, -1, 1},  {1, 1, -1},  {-1, 1, 1},  {1, -1, 1},  {1, 1, -1}};
+
+/* ------------------------------------------------------------------------ */
+/* Halo-padded read, as produced by symmetry_bd (fmisc.f90:877-901): an index
+ * below 0 mirrors onto -i-1 and picks up the parity of that axis.  `off` is
+ * added before mirroring (lopsided on gxx = dxx + 1, bssn_rhs.f90:148,948). */
+static inline double fh(const double *f, const int *ex, int i, int j, int k,
+                        const double *S, double off)
+{
+    double s = 1.0;
+    if (i < 0) { i = -i - 1; s *= S[0]; }
+    if (j < 0) { j = -j - 1; s *= S[1]; }
+    if (k < 0) { k = -k - 1; s *= S[2]; }
+    double v = f[IX(ex, i, j, k)] + off;
+    return s == 1.0 ? v : v * s;
+}
+
+typedef struct {
+    int ex[3];
+    int lo2[3];   /* lowest readable 0-based index for fderivs/fdderivs */
+    int lo3[3];   /* ... for lopsided / kodis */
+    double dX, dY, dZ;
+} grid_t;
+
+static void grid_init(grid_t *g, const int *ex, const double *X, const double *Y,
+                      const double *Z, int symmetry, int ko_octant_only)
+{
+    for (int a = 0; a < 3; a++) g->ex[a] = ex[a];
+    g->dX = X[1] - X[0];
+    g->dY = Y[1] - Y[0];
+    g->dZ = Z[1] - Z[0];
+    /* diff_new.f90:975-980, lopsidediff.f90:230-232, kodiss.f90:156-158 */
+    int zs = symmetry > 0 && fabs(Z[0]) < g->dZ;
+    int xs = symmetry > 1 && fabs(X[0]) < g->dX;
+    int ys = symmetry > 1 && fabs(Y[0]) < g->dY;
+    (void)ko_octant_only;
+    g->lo2[0] = xs ? -2 : 0; g->lo2[1] = ys ? -2 : 0; g->lo2[2] = zs ? -2 : 0;
+    g->lo3[0] = xs ? -3 : 0; g->lo3[1] = ys ? -3 : 0; g->lo3[2] = zs ? -3 : 0;
+}
+
+static inline int room(int i, int r, int lo, int n) { return i - r >= lo && i + r <= n - 1; }
+
+/* fderivs at one point (diff_new.f90:996-1100, joint three-axis test). */
+static void d1_pt(const grid_t *g, const double *f, const double *S, int i, int j, int k,
+                  double *fx, double *fy, double *fz)
+{
+    const int *ex = g->ex;
+    *fx = *fy = *fz = 0.0;
+    if (i > ex[0] - 2 || j > ex[1] - 2 || k > ex[2] - 2) return;
+    const double d12dx = 1.0 / 12.0 / g->dX, d12dy = 1.0 / 12.0 / g->dY, d12dz = 1.0 / 12.0 / g->dZ;
+    const double d2dx = 1.0 / 2.0 / g->dX, d2dy = 1.0 / 2.0 / g->dY, d2dz = 1.0 / 2.0 / g->dZ;
+#define F(a, b, c) fh(f, ex, a, b, c, S, 0.0)
+    if (room(i, 2, g->lo2[0], ex[0]) && room(j, 2, g->lo2[1], ex[1]) && room(k, 2, g->lo2[2], ex[2])) {
+        *fx = d12dx * (F(i - 2, j, k) - 8.0 * F(i - 1, j, k) + 8.0 * F(i + 1, j, k) - F(i + 2, j, k));
+        *fy = d12dy * (F(i, j - 2, k) - 8.0 * F(i, j - 1, k) + 8.0 * F(i, j + 1, k) - F(i, j + 2, k));
+        *fz = d12dz * (F(i, j, k - 2) - 8.0 * F(i, j, k - 1) + 8.0 * F(i, j, k + 1) - F(i, j, k + 2));
+    } else if (room(i, 1, g->lo2[0], ex[0]) && room(j, 1, g->lo2[1], ex[1]) && room(k, 1, g->lo2[2], ex[2])) {
+        *fx = d2dx * (-F(i - 1, j, k) + F(i + 1, j, k));
+        *fy = d2dy * (-F(i, j - 1, k) + F(i, j + 1, k));
+        *fz = d2dz * (-F(i, j, k - 1) + F(i, j, k + 1));
+    }
+#undef F
+}
+
+/* fdderivs at one point (diff_new.f90:1397-1524). o = xx xy xz yy yz zz. */
+static void d2_pt(const grid_t *g, const double *f, const double *S, int i, int j, int k, double *o)
+{
+    const int *ex = g->ex;
+    for (int a = 0; a < 6; a++) o[a] = 0.0;
+    if (i > ex[0] - 2 || j > ex[1] - 2 || k > ex[2] - 2) return;
+    const double dX = g->dX, dY = g->dY, dZ = g->dZ;
+    const double F1o12 = 1.0 / 12.0, F1o144 = 1.0 / 144.0;
+#define F(a, b, c) fh(f, ex, a, b, c, S, 0.0)
+    if (room(i, 2, g->lo2[0], ex[0]) && room(j, 2, g->lo2[1], ex[1]) && room(k, 2, g->lo2[2], ex[2])) {
+        const double Fdxdx = F1o12 / (dX * dX), Fdydy = F1o12 / (dY * dY), Fdzdz = F1o12 / (dZ * dZ);
+        const double Fdxdy = F1o144 / (dX * dY), Fdxdz = F1o144 / (dX * dZ), Fdydz = F1o144 / (dY * dZ);
+        o[0] = Fdxdx * (-F(i - 2, j, k) + 16.0 * F(i - 1, j, k) - 30.0 * F(i, j, k) - F(i + 2, j, k) + 16.0 * F(i + 1, j, k));
+        o[3] = Fdydy * (-F(i, j - 2, k) + 16.0 * F(i, j - 1, k) - 30.0 * F(i, j, k) - F(i, j + 2, k) + 16.0 * F(i, j + 1, k));
+        o[5] = Fdzdz * (-F(i, j, k - 2) + 16.0 * F(i, j, k - 1) - 30.0 * F(i, j, k) - F(i, j, k + 2) + 16.0 * F(i, j, k + 1));
+#define ROWX(jj, kk) (F(i - 2, jj, kk) - 8.0 * F(i - 1, jj, kk) + 8.0 * F(i + 1, jj, kk) - F(i + 2, jj, kk))
+#define ROWY(ii, kk) (F(ii, j - 2, kk) - 8.0 * F(ii, j - 1, kk) + 8.0 * F(ii, j + 1, kk) - F(ii, j + 2, kk))
+        o[1] = Fdxdy * ((ROWX(j - 2, k)) - 8.0 * (ROWX(j - 1, k)) + 8.0 * (ROWX(j + 1, k)) - (ROWX(j + 2, k)));
+        o[2] = Fdxdz * ((ROWX(j, k - 2)) - 8.0 * (ROWX(j, k - 1)) + 8.0 * (ROWX(j, k + 1)) - (ROWX(j, k + 2)));
+        o[4] = Fdydz * ((ROWY(i, k - 2)) - 8.0 * (ROWY(i, k - 1)) + 8.0 * (ROWY(i, k + 1)) - (ROWY(i, k + 2)));
+#undef ROWX
+#undef ROWY
+    } else if (room(i, 1, g->lo2[0], ex[0]) && room(j, 1, g->lo2[1], ex[1]) && room(k, 1, g->lo2[2], ex[2])) {
+        const double Sdxdx = 1.0 / (dX * dX), Sdydy = 1.0 / (dY * dY), Sdzdz = 1.0 / (dZ * dZ);
+        const double Sdxdy = 0.25 / (dX * dY), Sdxdz = 0.25 / (dX * dZ), Sdydz = 0.25 / (dY * dZ);
+        o[0] = Sdxdx * (F(i - 1, j, k) - 2.0 * F(i, j, k) + F(i + 1, j, k));
+        o[3] = Sdydy * (F(i, j - 1, k) - 2.0 * F(i, j, k) + F(i, j + 1, k));
+        o[5] = Sdzdz * (F(i, j, k - 1) - 2.0 * F(i, j, k) + F(i, j, k + 1));
+        o[1] = Sdxdy * (F(i - 1, j - 1, k) - F(i + 1, j - 1, k) - F(i - 1, j + 1, k) + F(i + 1, j + 1, k));
+        o[2] = Sdxdz * (F(i - 1, j, k - 1) - F(i + 1, j, k - 1) - F(i - 1, j, k + 1) + F(i + 1, j, k + 1));
+        o[4] = Sdydz * (F(i, j - 1, k - 1) - F(i, j + 1, k - 1) - F(i, j - 1, k + 1) + F(i, j + 1, k + 1));
+    }
+#undef F
+}
+
+/* lopsided at one point, one direction at a time (lopsidediff.f90:321-481).
+ * p(m) reads the field m points along the axis; returns the updated rhs. */
+typedef struct { const double *f; const int *ex; const double *S; double off; int i, j, k, ax; } line_t;
+static inline double lp(const line_t *L, int m)
+{
+    int i = L->i, j = L->j, k = L->k;
+    if (L->ax == 0) i += m; else if (L->ax == 1) j += m; else k += m;
+    return fh(L->f, L->ex, i, j, k, L->S, L->off);
+}
+
+static double lop_dir(double r, const line_t *L, double beta, double d12, int pos, int lo, int n)
+{
+    if (beta > 0.0) {
+        if (pos + 3 <= n - 1)
+            r = r + beta * d12 * (-3.0 * lp(L, -1) - 10.0 * lp(L, 0) + 18.0 * lp(L, 1) - 6.0 * lp(L, 2) + lp(L, 3));
+        else if (pos + 2 <= n - 1)
+            r = r + beta * d12 * (lp(L, -2) - 8.0 * lp(L, -1) + 8.0 * lp(L, 1) - lp(L, 2));
+        else if (pos + 1 <= n - 1)
+            r = r - beta * d12 * (-3.0 * lp(L, 1) - 10.0 * lp(L, 0) + 18.0 * lp(L, -1) - 6.0 * lp(L, -2) + lp(L, -3));
+    } else if (beta < 0.0) {
+        if (pos - 3 >= lo)
+            r = r - beta * d12 * (-3.0 * lp(L, 1) - 10.0 * lp(L, 0) + 18.0 * lp(L, -1) - 6.0 * lp(L, -2) + lp(L, -3));
+        else if (pos - 2 >= lo)
+            r = r + beta * d12 * (lp(L, -2) - 8.0 * lp(L, -1) + 8.0 * lp(L, 1) - lp(L, 2));
+        else if (pos - 1 >= lo)
+            r = r + beta * d12 * (-3.0 * lp(L, -1) - 10.0 * lp(L, 0) + 18.0 * lp(L, 1) - 6.0 * lp(L, 2) + lp(L, 3));
+    }
+    return r;
+}
+
+static double lop_pt(const grid_t *g, const double *f, double off, const double *S, int i, int j, int k,
+                     double bx, double by, double bz, double r)
+{
+    const int *ex = g->ex;
+    if (i > ex[0] - 2 || j > ex[1] - 2 || k > ex[2] - 2) return r;
+    line_t L = {f, ex, S, off, i, j, k, 0};
+    r = lop_dir(r, &L, bx, 1.0 / 12.0 / g->dX, i, g->lo3[0], ex[0]);
+    L.ax = 1;
+    r = lop_dir(r, &L, by, 1.0 / 12.0 / g->dY, j, g->lo3[1], ex[1]);
+    L.ax = 2;
+    r = lop_dir(r, &L, bz, 1.0 / 12.0 / g->dZ, k, g->lo3[2], ex[2]);
+    return r;
+}
+
+/* kodis at one point (kodiss.f90:162-212). */
+static double ko_pt(const grid_t *g, const double *f, const double *S, int i, int j, int k, double eps,
+                    double r)
+{
+    const int *ex = g->ex;
+    if (!(room(i, 3, g->lo3[0], ex[0]) && room(j, 3, g->lo3[1], ex[1]) && room(k, 3, g->lo3[2], ex[2])))
+        return r;
+#define F(a, b, c) fh(f, ex, a, b, c, S, 0.0)
+    const double c0 = F(i, j, k);
+    const double ax = (F(i - 3, j, k) + F(i + 3, j, k)) - 6.0 * (F(i - 2, j, k) + F(i + 2, j, k)) +
+                      15.0 * (F(i - 1, j, k) + F(i + 1, j, k)) - 20.0 * c0;
+    const double ay = (F(i, j - 3, k) + F(i, j + 3, k)) - 6.0 * (F(i, j - 2, k) + F(i, j + 2, k)) +
+                      15.0 * (F(i, j - 1, k) + F(i, j + 1, k)) - 20.0 * c0;
+    const double az = (F(i, j, k - 3) + F(i, j, k + 3)) - 6.0 * (F(i, j, k - 2) + F(i, j, k + 2)) +
+                      15.0 * (F(i, j, k - 1) + F(i, j, k + 1)) - 20.0 * c0;
+#undef F
+    return r + eps / 64.0 * (ax / g->dX + ay / g->dY + az / g->dZ);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Array forms of the stencil routines, for direct comparison with the
+ * reference symbols fderivs_, fdderivs_, lopsided_, kodis_, symmetry_bd_. */
+void orc_fderivs(const int *ex, const double *f, double *fx, double *fy, double *fz, const double *X,
+                 const double *Y, const double *Z, double S1, double S2, double S3, int symmetry)
+{
+    grid_t g;
+    grid_init(&g, ex, X, Y, Z, symmetry, 0);
+    const double S[3] = {S1, S2, S3};
+    for (int k = 0; k < ex[2]; k++)
+        for (int j = 0; j < ex[1]; j++)
+            for (int i = 0; i < ex[0]; i++) {
+                size_t p = IX(ex, i, j, k);
+                d1_pt(&g, f, S, i, j, k, &fx[p], &fy[p], &fz[p]);
+            }
+}
+
+void orc_fdderivs(const int *ex, const double *f, double *fxx, double *fxy, double *fxz, double *fyy,
+                  double *fyz, double *fzz, const double *X, const double *Y, const double *Z, double S1,
+                  double S2, double S3, int symmetry)
+{
+    grid_t g;
+    grid_init(&g, ex, X, Y, Z, symmetry, 0);
+    const double S[3] = {S1, S2, S3};
+    double o[6];
+    for (int k = 0; k < ex[2]; k++)
+        for (int j = 0; j < ex[1]; j++)
+            for (int i = 0; i < ex[0]; i++) {
+                size_t p = IX(ex, i, j, k);
+                d2_pt(&g, f, S, i, j, k, o);
+                fxx[p] = o[0]; fxy[p] = o[1]; fxz[p] = o[2];
+                fyy[p] = o[3]; fyz[p] = o[4]; fzz[p] = o[5];
+            }
+}
+
+void orc_lopsided(const int *ex, const double *X, const double *Y, const double *Z, const double *f,
+                  double *f_rhs, const double *Sfx, const double *Sfy, const double *Sfz, int symmetry,
+                  const double *SoA)
+{
+    grid_t g;
+    grid_init(&g, ex, X, Y, Z, symmetry, 0);
+    for (int k = 0; k < ex[2]; k++)
+        for (int j = 0; j < ex[1]; j++)
+            for (int i = 0; i < ex[0]; i++) {
+                size_t p = IX(ex, i, j, k);
+                f_rhs[p] = lop_pt(&g, f, 0.0, SoA, i, j, k, Sfx[p], Sfy[p], Sfz[p], f_rhs[p]);
+            }
+}
+
+void orc_kodis(const int *ex, const double *X, const double *Y, const double *Z, const double *f,
+               double *f_rhs, const double *SoA, int symmetry, double eps)
+{
+    grid_t g;
+    grid_init(&g, ex, X, Y, Z, symmetry, 1);
+    for (int k = 0; k < ex[2]; k++)
+        for (int j = 0; j < ex[1]; j++)
+            for (int i = 0; i < ex[0]; i++) {
+                size_t p = IX(ex, i, j, k);
+                f_rhs[p] = ko_pt(&g, f, SoA, i, j, k, eps, f_rhs[p]);
+            }
+}
+
+void orc_symmetry_bd(int ord, const int *ex, const double *f, double *fhout, const double *SoA)
+{
+    const int n0 = ex[0] + ord, n1 = ex[1] + ord, n2 = ex[2] + ord;
+    for (int k = 0; k < n2; k++)
+        for (int j = 0; j < n1; j++)
+            for (int i = 0; i < n0; i++)
+                fhout[(size_t)i + (size_t)n0 * ((size_t)j + (size_t)n1 * k)] =
+                    fh(f, ex, i - ord, j - ord, k - ord, SoA, 0.0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* The RHS (bssn_rhs.f90:5-1186), GAUGE 0 (:792-799) or GAUGE 2 (:809-820).  */
+enum { CHI, TRK, DXX, GXY, GXZ, DYY, GYZ, DZZ, AXX, AXY, AXZ, AYY, AYZ, AZZ,
+       GAMX, GAMY, GAMZ, LAP, BETAX, BETAY, BETAZ, DTSFX, DTSFY, DTSFZ };
+
+int orc_compute_rhs_bssn(const int *ex, double T, const double *X, const double *Y, const double *Z,
+                         const double *const *st, double *const *rhs, const double *const *mat,
+                         double *const *gamo, double *const *rico, double *const *res, int symmetry,
+                         int lev, double eps, int co, int gauge)
+{
+    (void)T; (void)lev;
+    const size_t N = (size_t)ex[0] * ex[1] * ex[2];
+    /* NaN sanity check over 21 fields (bssn_rhs.f90:109-138). */
+    {
+        double s = 0.0;
+        for (int v = 0; v < 21; v++)
+            for (size_t p = 0; p < N; p++) s += st[v][p];
+        if (s != s) return 1;
+    }
+    grid_t g;
+    grid_init(&g, ex, X, Y, Z, symmetry, 0);
+    const double ZEO = 0.0, ONE = 1.0, TWO = 2.0, FOUR = 4.0, EIGHT = 8.0, HALF = 0.5;
+    const double FF = 0.75, eta = 2.0;
+    const double F1o3 = 1.0 / 3.0, F2o3 = 2.0 / 3.0, F3o2 = 1.5, F16 = 16.0, F8 = 8.0;
+    const double PI = acos(-ONE);
+    (void)ZEO;
+
+    for (int k = 0; k < ex[2]; k++)
+    for (int j = 0; j < ex[1]; j++)
+    for (int i = 0; i < ex[0]; i++) {
+        const size_t p = IX(ex, i, j, k);
+        const double chi = st[CHI][p], trK = st[TRK][p];
+        const double dxx = st[DXX][p], gxy = st[GXY][p], gxz = st[GXZ][p];
+        const double dyy = st[DYY][p], gyz = st[GYZ][p], dzz = st[DZZ][p];
+        const double Axx = st[AXX][p], Axy = st[AXY][p], Axz = st[AXZ][p];
+        const double Ayy = st[AYY][p], Ayz = st[AYZ][p], Azz = st[AZZ][p];
+        const double Gamx = st[GAMX][p], Gamy = st[GAMY][p], Gamz = st[GAMZ][p];
+        const double Lap = st[LAP][p], betax = st[BETAX][p], betay = st[BETAY][p], betaz = st[BETAZ][p];
+        const double dtSfx = st[DTSFX][p], dtSfy = st[DTSFY][p], dtSfz = st[DTSFZ][p];
+        double rho = 0, Sx = 0, Sy = 0, Sz = 0, Sxx = 0, Sxy = 0, Sxz = 0, Syy = 0, Syz = 0, Szz = 0;
+        if (mat) {
+            rho = mat[0][p]; Sx = mat[1][p]; Sy = mat[2][p]; Sz = mat[3][p];
+            Sxx = mat[4][p]; Sxy = mat[5][p]; Sxz = mat[6][p]; Syy = mat[7][p]; Syz = mat[8][p]; Szz = mat[9][p];
+        }
+        double dd[6];
+#define D1(v, a, b, c) d1_pt(&g, st[v], PAR[v], i, j, k, &(a), &(b), &(c))
+#define D2(v, a, b, c, d, e, f_) do { d2_pt(&g, st[v], PAR[v], i, j, k, dd); \
+        a = dd[0]; b = dd[1]; c = dd[2]; d = dd[3]; e = dd[4]; f_ = dd[5]; } while (0)
+
+        /* :146-150 */
+        const double alpn1 = Lap + ONE, chin1 = chi + ONE;
+        const double gxx = dxx + ONE, gyy = dyy + ONE, gzz = dzz + ONE;
+        /* :152-160 */
+        double betaxx, betaxy, betaxz, betayx, betayy, betayz, betazx, betazy, betazz;
+        D1(BETAX, betaxx, betaxy, betaxz);
+        D1(BETAY, betayx, betayy, betayz);
+        D1(BETAZ, betazx, betazy, betazz);
+        const double div_beta = betaxx + betayy + betazz;
+        double chix, chiy, chiz;
+        D1(CHI, chix, chiy, chiz);
+        double chi_rhs = F2o3 * chin1 * (alpn1 * trK - div_beta);
+        /* :162-191 */
+        double gxxx, gxxy, gxxz, gxyx, gxyy, gxyz, gxzx, gxzy, gxzz;
+        double gyyx, gyyy, gyyz, gyzx, gyzy, gyzz, gzzx, gzzy, gzzz;
+        D1(DXX, gxxx, gxxy, gxxz);
+        D1(GXY, gxyx, gxyy, gxyz);
+        D1(GXZ, gxzx, gxzy, gxzz);
+        D1(DYY, gyyx, gyyy, gyyz);
+        D1(GYZ, gyzx, gyzy, gyzz);
+        D1(DZZ, gzzx, gzzy, gzzz);
+        double gxx_rhs = -TWO * alpn1 * Axx - F2o3 * gxx * div_beta +
+                         TWO * (gxx * betaxx + gxy * betayx + gxz * betazx);
+        double gyy_rhs = -TWO * alpn1 * Ayy - F2o3 * gyy * div_beta +
+                         TWO * (gxy * betaxy + gyy * betayy + gyz * betazy);
+        double gzz_rhs = -TWO * alpn1 * Azz - F2o3 * gzz * div_beta +
+                         TWO * (gxz * betaxz + gyz * betayz + gzz * betazz);
+        double gxy_rhs = -TWO * alpn1 * Axy + F1o3 * gxy * div_beta + gxx * betaxy + gxz * betazy +
+                         gyy * betayx + gyz * betazx - gxy * betazz;
+        double gyz_rhs = -TWO * alpn1 * Ayz + F1o3 * gyz * div_beta + gxy * betaxz + gyy * betayz +
+                         gxz * betaxy + gzz * betazy - gyz * betaxx;
+        double gxz_rhs = -TWO * alpn1 * Axz + F1o3 * gxz * div_beta + gxx * betaxz + gxy * betayz +
+                         gyz * betayx + gzz * betazx - gxz * betayy;
+        /* :194-201 */
+        double gupzz = gxx * gyy * gzz + gxy * gyz * gxz + gxz * gxy * gyz - gxz * gyy * gxz -
+                       gxy * gxy * gzz - gxx * gyz * gyz;
+        const double gupxx = (gyy * gzz - gyz * gyz) / gupzz;
+        const double gupxy = -(gxy * gzz - gyz * gxz) / gupzz;
+        const double gupxz = (gxy * gyz - gyy * gxz) / gupzz;
+        const double gupyy = (gxx * gzz - gxz * gxz) / gupzz;
+        const double gupyz = -(gxx * gyz - gxy * gxz) / gupzz;
+        gupzz = (gxx * gyy - gxy * gxy) / gupzz;
+        /* :203-232 */
+        if (co == 0) {
+            res[4][p] = Gamx - (gupxx * (gupxx * gxxx + gupxy * gxyx + gupxz * gxzx) +
+                                gupxy * (gupxx * gxyx + gupxy * gyyx + gupxz * gyzx) +
+                                gupxz * (gupxx * gxzx + gupxy * gyzx + gupxz * gzzx) +
+                                gupxx * (gupxy * gxxy + gupyy * gxyy + gupyz * gxzy) +
+                                gupxy * (gupxy * gxyy + gupyy * gyyy + gupyz * gyzy) +
+                                gupxz * (gupxy * gxzy + gupyy * gyzy + gupyz * gzzy) +
+                                gupxx * (gupxz * gxxz + gupyz * gxyz + gupzz * gxzz) +
+                                gupxy * (gupxz * gxyz + gupyz * gyyz + gupzz * gyzz) +
+                                gupxz * (gupxz * gxzz + gupyz * gyzz + gupzz * gzzz));
+            res[5][p] = Gamy - (gupxx * (gupxy * gxxx + gupyy * gxyx + gupyz * gxzx) +
+                                gupxy * (gupxy * gxyx + gupyy * gyyx + gupyz * gyzx) +
+                                gupxz * (gupxy * gxzx + gupyy * gyzx + gupyz * gzzx) +
+                                gupxy * (gupxy * gxxy + gupyy * gxyy + gupyz * gxzy) +
+                                gupyy * (gupxy * gxyy + gupyy * gyyy + gupyz * gyzy) +
+                                gupyz * (gupxy * gxzy + gupyy * gyzy + gupyz * gzzy) +
+                                gupxy * (gupxz * gxxz + gupyz * gxyz + gupzz * gxzz) +
+                                gupyy * (gupxz * gxyz + gupyz * gyyz + gupzz * gyzz) +
+                                gupyz * (gupxz * gxzz + gupyz * gyzz + gupzz * gzzz));
+            res[6][p] = Gamz - (gupxx * (gupxz * gxxx + gupyz * gxyx + gupzz * gxzx) +
+                                gupxy * (gupxz * gxyx + gupyz * gyyx + gupzz * gyzx) +
+                                gupxz * (gupxz * gxzx + gupyz * gyzx + gupzz * gzzx) +
+                                gupxy * (gupxz * gxxy + gupyz * gxyy + gupzz * gxzy) +
+                                gupyy * (gupxz * gxyy + gupyz * gyyy + gupzz * gyzy) +
+                                gupyz * (gupxz * gxzy + gupyz * gyzy + gupzz * gzzy) +
+                                gupxz * (gupxz * gxxz + gupyz * gxyz + gupzz * gxzz) +
+                                gupyz * (gupxz * gxyz + gupyz * gyyz + gupzz * gyzz) +
+                                gupzz * (gupxz * gxzz + gupyz * gyzz + gupzz * gzzz));
+        }
+        /* :235-257 conformal Christoffels of the second kind */
+        double Gamxxx = HALF * (gupxx * gxxx + gupxy * (TWO * gxyx - gxxy) + gupxz * (TWO * gxzx - gxxz));
+        double Gamyxx = HALF * (gupxy * gxxx + gupyy * (TWO * gxyx - gxxy) + gupyz * (TWO * gxzx - gxxz));
+        double Gamzxx = HALF * (gupxz * gxxx + gupyz * (TWO * gxyx - gxxy) + gupzz * (TWO * gxzx - gxxz));
+        double Gamxyy = HALF * (gupxx * (TWO * gxyy - gyyx) + gupxy * gyyy + gupxz * (TWO * gyzy - gyyz));
+        double Gamyyy = HALF * (gupxy * (TWO * gxyy - gyyx) + gupyy * gyyy + gupyz * (TWO * gyzy - gyyz));
+        double Gamzyy = HALF * (gupxz * (TWO * gxyy - gyyx) + gupyz * gyyy + gupzz * (TWO * gyzy - gyyz));
+        double Gamxzz = HALF * (gupxx * (TWO * gxzz - gzzx) + gupxy * (TWO * gyzz - gzzy) + gupxz * gzzz);
+        double Gamyzz = HALF * (gupxy * (TWO * gxzz - gzzx) + gupyy * (TWO * gyzz - gzzy) + gupyz * gzzz);
+        double Gamzzz = HALF * (gupxz * (TWO * gxzz - gzzx) + gupyz * (TWO * gyzz - gzzy) + gupzz * gzzz);
+        double Gamxxy = HALF * (gupxx * gxxy + gupxy * gyyx + gupxz * (gxzy + gyzx - gxyz));
+        double Gamyxy = HALF * (gupxy * gxxy + gupyy * gyyx + gupyz * (gxzy + gyzx - gxyz));
+        double Gamzxy = HALF * (gupxz * gxxy + gupyz * gyyx + gupzz * (gxzy + gyzx - gxyz));
+        double Gamxxz = HALF * (gupxx * gxxz + gupxy * (gxyz + gyzx - gxzy) + gupxz * gzzx);
+        double Gamyxz = HALF * (gupxy * gxxz + gupyy * (gxyz + gyzx - gxzy) + gupyz * gzzx);
+        double Gamzxz = HALF * (gupxz * gxxz + gupyz * (gxyz + gyzx - gxzy) + gupzz * gzzx);
+        double Gamxyz = HALF * (gupxx * (gxyz + gxzy - gyzx) + gupxy * gyyz + gupxz * gzzy);
+        double Gamyyz = HALF * (gupxy * (gxyz + gxzy - gyzx) + gupyy * gyyz + gupyz * gzzy);
+        double Gamzyz = HALF * (gupxz * (gxyz + gxzy - gyzx) + gupyz * gyyz + gupzz * gzzy);
+        /* :260-282 A^ij kept in the R slots */
+        double Rxx = gupxx * gupxx * Axx + gupxy * gupxy * Ayy + gupxz * gupxz * Azz +
+                     TWO * (gupxx * gupxy * Axy + gupxx * gupxz * Axz + gupxy * gupxz * Ayz);
+        double Ryy = gupxy * gupxy * Axx + gupyy * gupyy * Ayy + gupyz * gupyz * Azz +
+                     TWO * (gupxy * gupyy * Axy + gupxy * gupyz * Axz + gupyy * gupyz * Ayz);
+        double Rzz = gupxz * gupxz * Axx + gupyz * gupyz * Ayy + gupzz * gupzz * Azz +
+                     TWO * (gupxz * gupyz * Axy + gupxz * gupzz * Axz + gupyz * gupzz * Ayz);
+        double Rxy = gupxx * gupxy * Axx + gupxy * gupyy * Ayy + gupxz * gupyz * Azz +
+                     (gupxx * gupyy + gupxy * gupxy) * Axy + (gupxx * gupyz + gupxz * gupxy) * Axz +
+                     (gupxy * gupyz + gupxz * gupyy) * Ayz;
+        double Rxz = gupxx * gupxz * Axx + gupxy * gupyz * Ayy + gupxz * gupzz * Azz +
+                     (gupxx * gupyz + gupxy * gupxz) * Axy + (gupxx * gupzz + gupxz * gupxz) * Axz +
+                     (gupxy * gupzz + gupxz * gupyz) * Ayz;
+        double Ryz = gupxy * gupxz * Axx + gupyy * gupyz * Ayy + gupyz * gupzz * Azz +
+                     (gupxy * gupyz + gupyy * gupxz) * Axy + (gupxy * gupzz + gupyz * gupxz) * Axz +
+                     (gupyy * gupzz + gupyz * gupyz) * Ayz;
+        /* :285-313 */
+        double Lapx, Lapy, Lapz, Kx, Ky, Kz;
+        D1(LAP, Lapx, Lapy, Lapz);
+        D1(TRK, Kx, Ky, Kz);
+        double Gamx_rhs = -TWO * (Lapx * Rxx + Lapy * Rxy + Lapz * Rxz) +
+            TWO * alpn1 * (-F3o2 / chin1 * (chix * Rxx + chiy * Rxy + chiz * Rxz) -
+                           gupxx * (F2o3 * Kx + EIGHT * PI * Sx) - gupxy * (F2o3 * Ky + EIGHT * PI * Sy) -
+                           gupxz * (F2o3 * Kz + EIGHT * PI * Sz) + Gamxxx * Rxx + Gamxyy * Ryy +
+                           Gamxzz * Rzz + TWO * (Gamxxy * Rxy + Gamxxz * Rxz + Gamxyz * Ryz));
+        double Gamy_rhs = -TWO * (Lapx * Rxy + Lapy * Ryy + Lapz * Ryz) +
+            TWO * alpn1 * (-F3o2 / chin1 * (chix * Rxy + chiy * Ryy + chiz * Ryz) -
+                           gupxy * (F2o3 * Kx + EIGHT * PI * Sx) - gupyy * (F2o3 * Ky + EIGHT * PI * Sy) -
+                           gupyz * (F2o3 * Kz + EIGHT * PI * Sz) + Gamyxx * Rxx + Gamyyy * Ryy +
+                           Gamyzz * Rzz + TWO * (Gamyxy * Rxy + Gamyxz * Rxz + Gamyyz * Ryz));
+        double Gamz_rhs = -TWO * (Lapx * Rxz + Lapy * Ryz + Lapz * Rzz) +
+            TWO * alpn1 * (-F3o2 / chin1 * (chix * Rxz + chiy * Ryz + chiz * Rzz) -
+                           gupxz * (F2o3 * Kx + EIGHT * PI * Sx) - gupyz * (F2o3 * Ky + EIGHT * PI * Sy) -
+                           gupzz * (F2o3 * Kz + EIGHT * PI * Sz) + Gamzxx * Rxx + Gamzyy * Ryy +
+                           Gamzzz * Rzz + TWO * (Gamzxy * Rxy + Gamzxz * Rxz + Gamzyz * Ryz));
+        /* :315-353 second derivatives of the shift land in the g_ij,k slots */
+        D2(BETAX, gxxx, gxyx, gxzx, gyyx, gyzx, gzzx);
+        D2(BETAY, gxxy, gxyy, gxzy, gyyy, gyzy, gzzy);
+        D2(BETAZ, gxxz, gxyz, gxzz, gyyz, gyzz, gzzz);
+        double fxx = gxxx + gxyy + gxzz;
+        double fxy = gxyx + gyyy + gyzz;
+        double fxz = gxzx + gyzy + gzzz;
+        const double Gamxa = gupxx * Gamxxx + gupyy * Gamxyy + gupzz * Gamxzz +
+                             TWO * (gupxy * Gamxxy + gupxz * Gamxxz + gupyz * Gamxyz);
+        const double Gamya = gupxx * Gamyxx + gupyy * Gamyyy + gupzz * Gamyzz +
+                             TWO * (gupxy * Gamyxy + gupxz * Gamyxz + gupyz * Gamyyz);
+        const double Gamza = gupxx * Gamzxx + gupyy * Gamzyy + gupzz * Gamzzz +
+                             TWO * (gupxy * Gamzxy + gupxz * Gamzxz + gupyz * Gamzyz);
+        double Gamxx, Gamxy, Gamxz, Gamyx, Gamyy, Gamyz, Gamzx, Gamzy, Gamzz;
+        D1(GAMX, Gamxx, Gamxy, Gamxz);
+        D1(GAMY, Gamyx, Gamyy, Gamyz);
+        D1(GAMZ, Gamzx, Gamzy, Gamzz);
+        Gamx_rhs = Gamx_rhs + F2o3 * Gamxa * div_beta - Gamxa * betaxx - Gamya * betaxy - Gamza * betaxz +
+                   F1o3 * (gupxx * fxx + gupxy * fxy + gupxz * fxz) + gupxx * gxxx + gupyy * gyyx +
+                   gupzz * gzzx + TWO * (gupxy * gxyx + gupxz * gxzx + gupyz * gyzx);
+        Gamy_rhs = Gamy_rhs + F2o3 * Gamya * div_beta - Gamxa * betayx - Gamya * betayy - Gamza * betayz +
+                   F1o3 * (gupxy * fxx + gupyy * fxy + gupyz * fxz) + gupxx * gxxy + gupyy * gyyy +
+                   gupzz * gzzy + TWO * (gupxy * gxyy + gupxz * gxzy + gupyz * gyzy);
+        Gamz_rhs = Gamz_rhs + F2o3 * Gamza * div_beta - Gamxa * betazx - Gamya * betazy - Gamza * betazz +
+                   F1o3 * (gupxz * fxx + gupyz * fxy + gupzz * fxz) + gupxx * gxxz + gupyy * gyyz +
+                   gupzz * gzzz + TWO * (gupxy * gxyz + gupxz * gxzz + gupyz * gyzz);
+        /* :356-375 first-kind Christoffels stored in the g_ij,k slots */
+        gxxx = gxx * Gamxxx + gxy * Gamyxx + gxz * Gamzxx;
+        gxyx = gxx * Gamxxy + gxy * Gamyxy + gxz * Gamzxy;
+        gxzx = gxx * Gamxxz + gxy * Gamyxz + gxz * Gamzxz;
+        gyyx = gxx * Gamxyy + gxy * Gamyyy + gxz * Gamzyy;
+        gyzx = gxx * Gamxyz + gxy * Gamyyz + gxz * Gamzyz;
+        gzzx = gxx * Gamxzz + gxy * Gamyzz + gxz * Gamzzz;
+        gxxy = gxy * Gamxxx + gyy * Gamyxx + gyz * Gamzxx;
+        gxyy = gxy * Gamxxy + gyy * Gamyxy + gyz * Gamzxy;
+        gxzy = gxy * Gamxxz + gyy * Gamyxz + gyz * Gamzxz;
+        gyyy = gxy * Gamxyy + gyy * Gamyyy + gyz * Gamzyy;
+        gyzy = gxy * Gamxyz + gyy * Gamyyz + gyz * Gamzyz;
+        gzzy = gxy * Gamxzz + gyy * Gamyzz + gyz * Gamzzz;
+        gxxz = gxz * Gamxxx + gyz * Gamyxx + gzz * Gamzxx;
+        gxyz = gxz * Gamxxy + gyz * Gamyxy + gzz * Gamzxy;
+        gxzz = gxz * Gamxxz + gyz * Gamyxz + gzz * Gamzxz;
+        gyyz = gxz * Gamxyy + gyz * Gamyyy + gzz * Gamzyy;
+        gyzz = gxz * Gamxyz + gyz * Gamyyz + gzz * Gamzyz;
+        gzzz = gxz * Gamxzz + gyz * Gamyzz + gzz * Gamzzz;
+        /* :378-400 */
+        double fyy, fyz, fzz;
+#define LAPG(v, R) do { D2(v, fxx, fxy, fxz, fyy, fyz, fzz); \
+        R = gupxx * fxx + gupyy * fyy + gupzz * fzz + (gupxy * fxy + gupxz * fxz + gupyz * fyz) * TWO; } while (0)
+        LAPG(DXX, Rxx);
+        LAPG(DYY, Ryy);
+        LAPG(DZZ, Rzz);
+        LAPG(GXY, Rxy);
+        LAPG(GXZ, Rxz);
+        LAPG(GYZ, Ryz);
+#undef LAPG
+        /* :402-601 conformal Ricci */
+        Rxx = -HALF * Rxx + gxx * Gamxx + gxy * Gamyx + gxz * Gamzx + Gamxa * gxxx + Gamya * gxyx +
+              Gamza * gxzx +
+              gupxx * (TWO * (Gamxxx * gxxx + Gamyxx * gxyx + Gamzxx * gxzx) + Gamxxx * gxxx +
+                       Gamyxx * gxxy + Gamzxx * gxxz) +
+              gupxy * (TWO * (Gamxxx * gxyx + Gamyxx * gyyx + Gamzxx * gyzx + Gamxxy * gxxx +
+                              Gamyxy * gxyx + Gamzxy * gxzx) +
+                       Gamxxy * gxxx + Gamyxy * gxxy + Gamzxy * gxxz + Gamxxx * gxyx + Gamyxx * gxyy +
+                       Gamzxx * gxyz) +
+              gupxz * (TWO * (Gamxxx * gxzx + Gamyxx * gyzx + Gamzxx * gzzx + Gamxxz * gxxx +
+                              Gamyxz * gxyx + Gamzxz * gxzx) +
+                       Gamxxz * gxxx + Gamyxz * gxxy + Gamzxz * gxxz + Gamxxx * gxzx + Gamyxx * gxzy +
+                       Gamzxx * gxzz) +
+              gupyy * (TWO * (Gamxxy * gxyx + Gamyxy * gyyx + Gamzxy * gyzx) + Gamxxy * gxyx +
+                       Gamyxy * gxyy + Gamzxy * gxyz) +
+              gupyz * (TWO * (Gamxxy * gxzx + Gamyxy * gyzx + Gamzxy * gzzx + Gamxxz * gxyx +
+                              Gamyxz * gyyx + Gamzxz * gyzx) +
+                       Gamxxz * gxyx + Gamyxz * gxyy + Gamzxz * gxyz + Gamxxy * gxzx + Gamyxy * gxzy +
+                       Gamzxy * gxzz) +
+              gupzz * (TWO * (Gamxxz * gxzx + Gamyxz * gyzx + Gamzxz * gzzx) + Gamxxz * gxzx +
+                       Gamyxz * gxzy + Gamzxz * gxzz);
+        Ryy = -HALF * Ryy + gxy * Gamxy + gyy * Gamyy + gyz * Gamzy + Gamxa * gxyy + Gamya * gyyy +
+              Gamza * gyzy +
+              gupxx * (TWO * (Gamxxy * gxxy + Gamyxy * gxyy + Gamzxy * gxzy) + Gamxxy * gxyx +
+                       Gamyxy * gxyy + Gamzxy * gxyz) +
+              gupxy * (TWO * (Gamxxy * gxyy + Gamyxy * gyyy + Gamzxy * gyzy + Gamxyy * gxxy +
+                              Gamyyy * gxyy + Gamzyy * gxzy) +
+                       Gamxyy * gxyx + Gamyyy * gxyy + Gamzyy * gxyz + Gamxxy * gyyx + Gamyxy * gyyy +
+                       Gamzxy * gyyz) +
+              gupxz * (TWO * (Gamxxy * gxzy + Gamyxy * gyzy + Gamzxy * gzzy + Gamxyz * gxxy +
+                              Gamyyz * gxyy + Gamzyz * gxzy) +
+                       Gamxyz * gxyx + Gamyyz * gxyy + Gamzyz * gxyz + Gamxxy * gyzx + Gamyxy * gyzy +
+                       Gamzxy * gyzz) +
+              gupyy * (TWO * (Gamxyy * gxyy + Gamyyy * gyyy + Gamzyy * gyzy) + Gamxyy * gyyx +
+                       Gamyyy * gyyy + Gamzyy * gyyz) +
+              gupyz * (TWO * (Gamxyy * gxzy + Gamyyy * gyzy + Gamzyy * gzzy + Gamxyz * gxyy +
+                              Gamyyz * gyyy + Gamzyz * gyzy) +
+                       Gamxyz * gyyx + Gamyyz * gyyy + Gamzyz * gyyz + Gamxyy * gyzx + Gamyyy * gyzy +
+                       Gamzyy * gyzz) +
+              gupzz * (TWO * (Gamxyz * gxzy + Gamyyz * gyzy + Gamzyz * gzzy) + Gamxyz * gyzx +
+                       Gamyyz * gyzy + Gamzyz * gyzz);
+        Rzz = -HALF * Rzz + gxz * Gamxz + gyz * Gamyz + gzz * Gamzz + Gamxa * gxzz + Gamya * gyzz +
+              Gamza * gzzz +
+              gupxx * (TWO * (Gamxxz * gxxz + Gamyxz * gxyz + Gamzxz * gxzz) + Gamxxz * gxzx +
+                       Gamyxz * gxzy + Gamzxz * gxzz) +
+              gupxy * (TWO * (Gamxxz * gxyz + Gamyxz * gyyz + Gamzxz * gyzz + Gamxyz * gxxz +
+                              Gamyyz * gxyz + Gamzyz * gxzz) +
+                       Gamxyz * gxzx + Gamyyz * gxzy + Gamzyz * gxzz + Gamxxz * gyzx + Gamyxz * gyzy +
+                       Gamzxz * gyzz) +
+              gupxz * (TWO * (Gamxxz * gxzz + Gamyxz * gyzz + Gamzxz * gzzz + Gamxzz * gxxz +
+                              Gamyzz * gxyz + Gamzzz * gxzz) +
+                       Gamxzz * gxzx + Gamyzz * gxzy + Gamzzz * gxzz + Gamxxz * gzzx + Gamyxz * gzzy +
+                       Gamzxz * gzzz) +
+              gupyy * (TWO * (Gamxyz * gxyz + Gamyyz * gyyz + Gamzyz * gyzz) + Gamxyz * gyzx +
+                       Gamyyz * gyzy + Gamzyz * gyzz) +
+              gupyz * (TWO * (Gamxyz * gxzz + Gamyyz * gyzz + Gamzyz * gzzz + Gamxzz * gxyz +
+                              Gamyzz * gyyz + Gamzzz * gyzz) +
+                       Gamxzz * gyzx + Gamyzz * gyzy + Gamzzz * gyzz + Gamxyz * gzzx + Gamyyz * gzzy +
+                       Gamzyz * gzzz) +
+              gupzz * (TWO * (Gamxzz * gxzz + Gamyzz * gyzz + Gamzzz * gzzz) + Gamxzz * gzzx +
+                       Gamyzz * gzzy + Gamzzz * gzzz);
+        Rxy = HALF * (-Rxy + gxx * Gamxy + gxy * Gamyy + gxz * Gamzy + gxy * Gamxx + gyy * Gamyx +
+                      gyz * Gamzx + Gamxa * gxyx + Gamya * gyyx + Gamza * gyzx + Gamxa * gxxy +
+                      Gamya * gxyy + Gamza * gxzy) +
+              gupxx * (Gamxxx * gxxy + Gamyxx * gxyy + Gamzxx * gxzy + Gamxxy * gxxx + Gamyxy * gxyx +
+                       Gamzxy * gxzx + Gamxxx * gxyx + Gamyxx * gxyy + Gamzxx * gxyz) +
+              gupxy * (Gamxxx * gxyy + Gamyxx * gyyy + Gamzxx * gyzy + Gamxxy * gxyx + Gamyxy * gyyx +
+                       Gamzxy * gyzx + Gamxxy * gxyx + Gamyxy * gxyy + Gamzxy * gxyz + Gamxxy * gxxy +
+                       Gamyxy * gxyy + Gamzxy * gxzy + Gamxyy * gxxx + Gamyyy * gxyx + Gamzyy * gxzx +
+                       Gamxxx * gyyx + Gamyxx * gyyy + Gamzxx * gyyz) +
+              gupxz * (Gamxxx * gxzy + Gamyxx * gyzy + Gamzxx * gzzy + Gamxxy * gxzx + Gamyxy * gyzx +
+                       Gamzxy * gzzx + Gamxxz * gxyx + Gamyxz * gxyy + Gamzxz * gxyz + Gamxxz * gxxy +
+                       Gamyxz * gxyy + Gamzxz * gxzy + Gamxyz * gxxx + Gamyyz * gxyx + Gamzyz * gxzx +
+                       Gamxxx * gyzx + Gamyxx * gyzy + Gamzxx * gyzz) +
+              gupyy * (Gamxxy * gxyy + Gamyxy * gyyy + Gamzxy * gyzy + Gamxyy * gxyx + Gamyyy * gyyx +
+                       Gamzyy * gyzx + Gamxxy * gyyx + Gamyxy * gyyy + Gamzxy * gyyz) +
+              gupyz * (Gamxxy * gxzy + Gamyxy * gyzy + Gamzxy * gzzy + Gamxyy * gxzx + Gamyyy * gyzx +
+                       Gamzyy * gzzx + Gamxxz * gyyx + Gamyxz * gyyy + Gamzxz * gyyz + Gamxxz * gxyy +
+                       Gamyxz * gyyy + Gamzxz * gyzy + Gamxyz * gxyx + Gamyyz * gyyx + Gamzyz * gyzx +
+                       Gamxxy * gyzx + Gamyxy * gyzy + Gamzxy * gyzz) +
+              gupzz * (Gamxxz * gxzy + Gamyxz * gyzy + Gamzxz * gzzy + Gamxyz * gxzx + Gamyyz * gyzx +
+                       Gamzyz * gzzx + Gamxxz * gyzx + Gamyxz * gyzy + Gamzxz * gyzz);
+        Rxz = HALF * (-Rxz + gxx * Gamxz + gxy * Gamyz + gxz * Gamzz + gxz * Gamxx + gyz * Gamyx +
+                      gzz * Gamzx + Gamxa * gxzx + Gamya * gyzx + Gamza * gzzx + Gamxa * gxxz +
+                      Gamya * gxyz + Gamza * gxzz) +
+              gupxx * (Gamxxx * gxxz + Gamyxx * gxyz + Gamzxx * gxzz + Gamxxz * gxxx + Gamyxz * gxyx +
+                       Gamzxz * gxzx + Gamxxx * gxzx + Gamyxx * gxzy + Gamzxx * gxzz) +
+              gupxy * (Gamxxx * gxyz + Gamyxx * gyyz + Gamzxx * gyzz + Gamxxz * gxyx + Gamyxz * gyyx +
+                       Gamzxz * gyzx + Gamxxy * gxzx + Gamyxy * gxzy + Gamzxy * gxzz + Gamxxy * gxxz +
+                       Gamyxy * gxyz + Gamzxy * gxzz + Gamxyz * gxxx + Gamyyz * gxyx + Gamzyz * gxzx +
+                       Gamxxx * gyzx + Gamyxx * gyzy + Gamzxx * gyzz) +
+              gupxz * (Gamxxx * gxzz + Gamyxx * gyzz + Gamzxx * gzzz + Gamxxz * gxzx + Gamyxz * gyzx +
+                       Gamzxz * gzzx + Gamxxz * gxzx + Gamyxz * gxzy + Gamzxz * gxzz + Gamxxz * gxxz +
+                       Gamyxz * gxyz + Gamzxz * gxzz + Gamxzz * gxxx + Gamyzz * gxyx + Gamzzz * gxzx +
+                       Gamxxx * gzzx + Gamyxx * gzzy + Gamzxx * gzzz) +
+              gupyy * (Gamxxy * gxyz + Gamyxy * gyyz + Gamzxy * gyzz + Gamxyz * gxyx + Gamyyz * gyyx +
+                       Gamzyz * gyzx + Gamxxy * gyzx + Gamyxy * gyzy + Gamzxy * gyzz) +
+              gupyz * (Gamxxy * gxzz + Gamyxy * gyzz + Gamzxy * gzzz + Gamxyz * gxzx + Gamyyz * gyzx +
+                       Gamzyz * gzzx + Gamxxz * gyzx + Gamyxz * gyzy + Gamzxz * gyzz + Gamxxz * gxyz +
+                       Gamyxz * gyyz + Gamzxz * gyzz + Gamxzz * gxyx + Gamyzz * gyyx + Gamzzz * gyzx +
+                       Gamxxy * gzzx + Gamyxy * gzzy + Gamzxy * gzzz) +
+              gupzz * (Gamxxz * gxzz + Gamyxz * gyzz + Gamzxz * gzzz + Gamxzz * gxzx + Gamyzz * gyzx +
+                       Gamzzz * gzzx + Gamxxz * gzzx + Gamyxz * gzzy + Gamzxz * gzzz);
+        Ryz = HALF * (-Ryz + gxy * Gamxz + gyy * Gamyz + gyz * Gamzz + gxz * Gamxy + gyz * Gamyy +
+                      gzz * Gamzy + Gamxa * gxzy + Gamya * gyzy + Gamza * gzzy + Gamxa * gxyz +
+                      Gamya * gyyz + Gamza * gyzz) +
+              gupxx * (Gamxxy * gxxz + Gamyxy * gxyz + Gamzxy * gxzz + Gamxxz * gxxy + Gamyxz * gxyy +
+                       Gamzxz * gxzy + Gamxxy * gxzx + Gamyxy * gxzy + Gamzxy * gxzz) +
+              gupxy * (Gamxxy * gxyz + Gamyxy * gyyz + Gamzxy * gyzz + Gamxxz * gxyy + Gamyxz * gyyy +
+                       Gamzxz * gyzy + Gamxyy * gxzx + Gamyyy * gxzy + Gamzyy * gxzz + Gamxyy * gxxz +
+                       Gamyyy * gxyz + Gamzyy * gxzz + Gamxyz * gxxy + Gamyyz * gxyy + Gamzyz * gxzy +
+                       Gamxxy * gyzx + Gamyxy * gyzy + Gamzxy * gyzz) +
+              gupxz * (Gamxxy * gxzz + Gamyxy * gyzz + Gamzxy * gzzz + Gamxxz * gxzy + Gamyxz * gyzy +
+                       Gamzxz * gzzy + Gamxyz * gxzx + Gamyyz * gxzy + Gamzyz * gxzz + Gamxyz * gxxz +
+                       Gamyyz * gxyz + Gamzyz * gxzz + Gamxzz * gxxy + Gamyzz * gxyy + Gamzzz * gxzy +
+                       Gamxxy * gzzx + Gamyxy * gzzy + Gamzxy * gzzz) +
+              gupyy * (Gamxyy * gxyz + Gamyyy * gyyz + Gamzyy * gyzz + Gamxyz * gxyy + Gamyyz * gyyy +
+                       Gamzyz * gyzy + Gamxyy * gyzx + Gamyyy * gyzy + Gamzyy * gyzz) +
+              gupyz * (Gamxyy * gxzz + Gamyyy * gyzz + Gamzyy * gzzz + Gamxyz * gxzy + Gamyyz * gyzy +
+                       Gamzyz * gzzy + Gamxyz * gyzx + Gamyyz * gyzy + Gamzyz * gyzz + Gamxyz * gxyz +
+                       Gamyyz * gyyz + Gamzyz * gyzz + Gamxzz * gxyy + Gamyzz * gyyy + Gamzzz * gyzy +
+                       Gamxyy * gzzx + Gamyyy * gzzy + Gamzyy * gzzz) +
+              gupzz * (Gamxyz * gxzz + Gamyyz * gyzz + Gamzyz * gzzz + Gamxzz * gxzy + Gamyzz * gyzy +
+                       Gamzzz * gzzy + Gamxyz * gzzx + Gamyyz * gzzy + Gamzyz * gzzz);
+        /* :603-626 chi part of the Ricci tensor */
+        D2(CHI, fxx, fxy, fxz, fyy, fyz, fzz);
+        fxx = fxx - Gamxxx * chix - Gamyxx * chiy - Gamzxx * chiz;
+        fxy = fxy - Gamxxy * chix - Gamyxy * chiy - Gamzxy * chiz;
+        fxz = fxz - Gamxxz * chix - Gamyxz * chiy - Gamzxz * chiz;
+        fyy = fyy - Gamxyy * chix - Gamyyy * chiy - Gamzyy * chiz;
+        fyz = fyz - Gamxyz * chix - Gamyyz * chiy - Gamzyz * chiz;
+        fzz = fzz - Gamxzz * chix - Gamyzz * chiy - Gamzzz * chiz;
+        double f = gupxx * (fxx - F3o2 / chin1 * chix * chix) + gupyy * (fyy - F3o2 / chin1 * chiy * chiy) +
+                   gupzz * (fzz - F3o2 / chin1 * chiz * chiz) + TWO * gupxy * (fxy - F3o2 / chin1 * chix * chiy) +
+                   TWO * gupxz * (fxz - F3o2 / chin1 * chix * chiz) + TWO * gupyz * (fyz - F3o2 / chin1 * chiy * chiz);
+        Rxx = Rxx + (fxx - chix * chix / chin1 / TWO + gxx * f) / chin1 / TWO;
+        Ryy = Ryy + (fyy - chiy * chiy / chin1 / TWO + gyy * f) / chin1 / TWO;
+        Rzz = Rzz + (fzz - chiz * chiz / chin1 / TWO + gzz * f) / chin1 / TWO;
+        Rxy = Rxy + (fxy - chix * chiy / chin1 / TWO + gxy * f) / chin1 / TWO;
+        Rxz = Rxz + (fxz - chix * chiz / chin1 / TWO + gxz * f) / chin1 / TWO;
+        Ryz = Ryz + (fyz - chiy * chiz / chin1 / TWO + gyz * f) / chin1 / TWO;
+        /* :629-664 lapse: physical Christoffels, D_i D_j alpha */
+        D2(LAP, fxx, fxy, fxz, fyy, fyz, fzz);
+        gxxx = (gupxx * chix + gupxy * chiy + gupxz * chiz) / chin1;
+        gxxy = (gupxy * chix + gupyy * chiy + gupyz * chiz) / chin1;
+        gxxz = (gupxz * chix + gupyz * chiy + gupzz * chiz) / chin1;
+        Gamxxx = Gamxxx - ((chix + chix) / chin1 - gxx * gxxx) * HALF;
+        Gamyxx = Gamyxx - (-gxx * gxxy) * HALF;
+        Gamzxx = Gamzxx - (-gxx * gxxz) * HALF;
+        Gamxyy = Gamxyy - (-gyy * gxxx) * HALF;
+        Gamyyy = Gamyyy - ((chiy + chiy) / chin1 - gyy * gxxy) * HALF;
+        Gamzyy = Gamzyy - (-gyy * gxxz) * HALF;
+        Gamxzz = Gamxzz - (-gzz * gxxx) * HALF;
+        Gamyzz = Gamyzz - (-gzz * gxxy) * HALF;
+        Gamzzz = Gamzzz - ((chiz + chiz) / chin1 - gzz * gxxz) * HALF;
+        Gamxxy = Gamxxy - (chiy / chin1 - gxy * gxxx) * HALF;
+        Gamyxy = Gamyxy - (chix / chin1 - gxy * gxxy) * HALF;
+        Gamzxy = Gamzxy - (-gxy * gxxz) * HALF;
+        Gamxxz = Gamxxz - (chiz / chin1 - gxz * gxxx) * HALF;
+        Gamyxz = Gamyxz - (-gxz * gxxy) * HALF;
+        Gamzxz = Gamzxz - (chix / chin1 - gxz * gxxz) * HALF;
+        Gamxyz = Gamxyz - (-gyz * gxxx) * HALF;
+        Gamyyz = Gamyyz - (chiz / chin1 - gyz * gxxy) * HALF;
+        Gamzyz = Gamzyz - (chiy / chin1 - gyz * gxxz) * HALF;
+        fxx = fxx - Gamxxx * Lapx - Gamyxx * Lapy - Gamzxx * Lapz;
+        fyy = fyy - Gamxyy * Lapx - Gamyyy * Lapy - Gamzyy * Lapz;
+        fzz = fzz - Gamxzz * Lapx - Gamyzz * Lapy - Gamzzz * Lapz;
+        fxy = fxy - Gamxxy * Lapx - Gamyxy * Lapy - Gamzxy * Lapz;
+        fxz = fxz - Gamxxz * Lapx - Gamyxz * Lapy - Gamzxz * Lapz;
+        fyz = fyz - Gamxyz * Lapx - Gamyyz * Lapy - Gamzyz * Lapz;
+        double trK_rhs = gupxx * fxx + gupyy * fyy + gupzz * fzz + TWO * (gupxy * fxy + gupxz * fxz + gupyz * fyz);
+        /* :666-703 matter trace and the trace-free source (BAM form) */
+        double S = chin1 * (gupxx * Sxx + gupyy * Syy + gupzz * Szz + TWO * (gupxy * Sxy + gupxz * Sxz + gupyz * Syz));
+        f = F2o3 * trK * trK -
+            (gupxx * (gupxx * Axx * Axx + gupyy * Axy * Axy + gupzz * Axz * Axz +
+                      TWO * (gupxy * Axx * Axy + gupxz * Axx * Axz + gupyz * Axy * Axz)) +
+             gupyy * (gupxx * Axy * Axy + gupyy * Ayy * Ayy + gupzz * Ayz * Ayz +
+                      TWO * (gupxy * Axy * Ayy + gupxz * Axy * Ayz + gupyz * Ayy * Ayz)) +
+             gupzz * (gupxx * Axz * Axz + gupyy * Ayz * Ayz + gupzz * Azz * Azz +
+                      TWO * (gupxy * Axz * Ayz + gupxz * Axz * Azz + gupyz * Ayz * Azz)) +
+             TWO * (gupxy * (gupxx * Axx * Axy + gupyy * Axy * Ayy + gupzz * Axz * Ayz +
+                             gupxy * (Axx * Ayy + Axy * Axy) + gupxz * (Axx * Ayz + Axz * Axy) +
+                             gupyz * (Axy * Ayz + Axz * Ayy)) +
+                    gupxz * (gupxx * Axx * Axz + gupyy * Axy * Ayz + gupzz * Axz * Azz +
+                             gupxy * (Axx * Ayz + Axy * Axz) + gupxz * (Axx * Azz + Axz * Axz) +
+                             gupyz * (Axy * Azz + Axz * Ayz)) +
+                    gupyz * (gupxx * Axy * Axz + gupyy * Ayy * Ayz + gupzz * Ayz * Azz +
+                             gupxy * (Axy * Ayz + Ayy * Axz) + gupxz * (Axy * Azz + Ayz * Axz) +
+                             gupyz * (Ayy * Azz + Ayz * Ayz)))) -
+            1.6e1 * PI * rho + EIGHT * PI * S;
+        f = -F1o3 * (gupxx * fxx + gupyy * fyy + gupzz * fzz + TWO * (gupxy * fxy + gupxz * fxz + gupyz * fyz) +
+                     alpn1 / chin1 * f);
+        fxx = alpn1 * (Rxx - EIGHT * PI * Sxx) - fxx;
+        fxy = alpn1 * (Rxy - EIGHT * PI * Sxy) - fxy;
+        fxz = alpn1 * (Rxz - EIGHT * PI * Sxz) - fxz;
+        fyy = alpn1 * (Ryy - EIGHT * PI * Syy) - fyy;
+        fyz = alpn1 * (Ryz - EIGHT * PI * Syz) - fyz;
+        fzz = alpn1 * (Rzz - EIGHT * PI * Szz) - fzz;
+        /* :720-777 */
+        double Axx_rhs = fxx - gxx * f;
+        double Ayy_rhs = fyy - gyy * f;
+        double Azz_rhs = fzz - gzz * f;
+        double Axy_rhs = fxy - gxy * f;
+        double Axz_rhs = fxz - gxz * f;
+        double Ayz_rhs = fyz - gyz * f;
+        fxx = gupxx * Axx * Axx + gupyy * Axy * Axy + gupzz * Axz * Axz +
+              TWO * (gupxy * Axx * Axy + gupxz * Axx * Axz + gupyz * Axy * Axz);
+        fyy = gupxx * Axy * Axy + gupyy * Ayy * Ayy + gupzz * Ayz * Ayz +
+              TWO * (gupxy * Axy * Ayy + gupxz * Axy * Ayz + gupyz * Ayy * Ayz);
+        fzz = gupxx * Axz * Axz + gupyy * Ayz * Ayz + gupzz * Azz * Azz +
+              TWO * (gupxy * Axz * Ayz + gupxz * Axz * Azz + gupyz * Ayz * Azz);
+        fxy = gupxx * Axx * Axy + gupyy * Axy * Ayy + gupzz * Axz * Ayz + gupxy * (Axx * Ayy + Axy * Axy) +
+              gupxz * (Axx * Ayz + Axz * Axy) + gupyz * (Axy * Ayz + Axz * Ayy);
+        fxz = gupxx * Axx * Axz + gupyy * Axy * Ayz + gupzz * Axz * Azz + gupxy * (Axx * Ayz + Axy * Axz) +
+              gupxz * (Axx * Azz + Axz * Axz) + gupyz * (Axy * Azz + Axz * Ayz);
+        fyz = gupxx * Axy * Axz + gupyy * Ayy * Ayz + gupzz * Ayz * Azz + gupxy * (Axy * Ayz + Ayy * Axz) +
+              gupxz * (Axy * Azz + Ayz * Axz) + gupyz * (Ayy * Azz + Ayz * Ayz);
+        f = chin1;
+        trK_rhs = f * trK_rhs;
+        Axx_rhs = f * Axx_rhs + alpn1 * (trK * Axx - TWO * fxx) +
+                  TWO * (Axx * betaxx + Axy * betayx + Axz * betazx) - F2o3 * Axx * div_beta;
+        Ayy_rhs = f * Ayy_rhs + alpn1 * (trK * Ayy - TWO * fyy) +
+                  TWO * (Axy * betaxy + Ayy * betayy + Ayz * betazy) - F2o3 * Ayy * div_beta;
+        Azz_rhs = f * Azz_rhs + alpn1 * (trK * Azz - TWO * fzz) +
+                  TWO * (Axz * betaxz + Ayz * betayz + Azz * betazz) - F2o3 * Azz * div_beta;
+        Axy_rhs = f * Axy_rhs + alpn1 * (trK * Axy - TWO * fxy) + Axx * betaxy + Axz * betazy + Ayy * betayx +
+                  Ayz * betazx + F1o3 * Axy * div_beta - Axy * betazz;
+        Ayz_rhs = f * Ayz_rhs + alpn1 * (trK * Ayz - TWO * fyz) + Axy * betaxz + Ayy * betayz + Axz * betaxy +
+                  Azz * betazy + F1o3 * Ayz * div_beta - Ayz * betaxx;
+        Axz_rhs = f * Axz_rhs + alpn1 * (trK * Axz - TWO * fxz) + Axx * betaxz + Axy * betayz + Ayz * betayx +
+                  Azz * betazx + F1o3 * Axz * div_beta - Axz * betayy;
+        /* :780-787 */
+        S = f * (gupxx * Sxx + gupyy * Syy + gupzz * Szz + TWO * (gupxy * Sxy + gupxz * Sxz + gupyz * Syz));
+        trK_rhs = -trK_rhs + alpn1 * (F1o3 * trK * trK + gupxx * fxx + gupyy * fyy + gupzz * fzz +
+                                      TWO * (gupxy * fxy + gupxz * fxz + gupyz * fyz) + FOUR * PI * (rho + S));
+        /* :791-820 gauge */
+        double Lap_rhs = -TWO * alpn1 * trK;
+        double betax_rhs = FF * dtSfx, betay_rhs = FF * dtSfy, betaz_rhs = FF * dtSfz;
+        double dtSfx_rhs, dtSfy_rhs, dtSfz_rhs;
+        if (gauge == 2) {
+            /* fderivs of chi again into the dtSf_rhs slots (same values as chix..) */
+            double cx, cy, cz;
+            D1(CHI, cx, cy, cz);
+            double reta = gupxx * cx * cx + gupyy * cy * cy + gupzz * cz * cz +
+                          TWO * (gupxy * cx * cy + gupxz * cx * cz + gupyz * cy * cz);
+            reta = 1.31 / 2 * sqrt(reta / chin1) / ((1 - sqrt(chin1)) * (1 - sqrt(chin1)));
+            dtSfx_rhs = Gamx_rhs - reta * dtSfx;
+            dtSfy_rhs = Gamy_rhs - reta * dtSfy;
+            dtSfz_rhs = Gamz_rhs - reta * dtSfz;
+        } else {
+            dtSfx_rhs = Gamx_rhs - eta * dtSfx;
+            dtSfy_rhs = Gamy_rhs - eta * dtSfy;
+            dtSfz_rhs = Gamz_rhs - eta * dtSfz;
+        }
+        /* :948-980 advection, then :983-1042 KO dissipation */
+        double r[NF] = {chi_rhs, trK_rhs, gxx_rhs, gxy_rhs, gxz_rhs, gyy_rhs, gyz_rhs, gzz_rhs,
+                        Axx_rhs, Axy_rhs, Axz_rhs, Ayy_rhs, Ayz_rhs, Azz_rhs,
+                        Gamx_rhs, Gamy_rhs, Gamz_rhs, Lap_rhs, betax_rhs, betay_rhs, betaz_rhs,
+                        dtSfx_rhs, dtSfy_rhs, dtSfz_rhs};
+        for (int v = 0; v < NF; v++) {
+            const double off = (v == DXX || v == DYY || v == DZZ) ? ONE : 0.0;
+            r[v] = lop_pt(&g, st[v], off, PAR[v], i, j, k, betax, betay, betaz, r[v]);
+        }
+        if (eps > 0)
+            for (int v = 0; v < NF; v++) r[v] = ko_pt(&g, st[v], PAR[v], i, j, k, eps, r[v]);
+        for (int v = 0; v < NF; v++) rhs[v][p] = r[v];
+
+        const double G2[18] = {Gamxxx, Gamxxy, Gamxxz, Gamxyy, Gamxyz, Gamxzz,
+                               Gamyxx, Gamyxy, Gamyxz, Gamyyy, Gamyyz, Gamyzz,
+                               Gamzxx, Gamzxy, Gamzxz, Gamzyy, Gamzyz, Gamzzz};
+        for (int a = 0; a < 18; a++) gamo[a][p] = G2[a];
+        rico[0][p] = Rxx; rico[1][p] = Rxy; rico[2][p] = Rxz;
+        rico[3][p] = Ryy; rico[4][p] = Ryz; rico[5][p] = Rzz;
+
+        /* :1044-1134 constraint residuals */
+        if (co == 0) {
+            double ham = gupxx * Rxx + gupyy * Ryy + gupzz * Rzz + TWO * (gupxy * Rxy + gupxz * Rxz + gupyz * Ryz);
+            ham = chin1 * ham + F2o3 * trK * trK -
+                  (gupxx * (gupxx * Axx * Axx + gupyy * Axy * Axy + gupzz * Axz * Axz +
+                            TWO * (gupxy * Axx * Axy + gupxz * Axx * Axz + gupyz * Axy * Axz)) +
+                   gupyy * (gupxx * Axy * Axy + gupyy * Ayy * Ayy + gupzz * Ayz * Ayz +
+                            TWO * (gupxy * Axy * Ayy + gupxz * Axy * Ayz + gupyz * Ayy * Ayz)) +
+                   gupzz * (gupxx * Axz * Axz + gupyy * Ayz * Ayz + gupzz * Azz * Azz +
+                            TWO * (gupxy * Axz * Ayz + gupxz * Axz * Azz + gupyz * Ayz * Azz)) +
+                   TWO * (gupxy * (gupxx * Axx * Axy + gupyy * Axy * Ayy + gupzz * Axz * Ayz +
+                                   gupxy * (Axx * Ayy + Axy * Axy) + gupxz * (Axx * Ayz + Axz * Axy) +
+                                   gupyz * (Axy * Ayz + Axz * Ayy)) +
+                          gupxz * (gupxx * Axx * Axz + gupyy * Axy * Ayz + gupzz * Axz * Azz +
+                                   gupxy * (Axx * Ayz + Axy * Axz) + gupxz * (Axx * Azz + Axz * Axz) +
+                                   gupyz * (Axy * Azz + Axz * Ayz)) +
+                          gupyz * (gupxx * Axy * Axz + gupyy * Ayy * Ayz + gupzz * Ayz * Azz +
+                                   gupxy * (Axy * Ayz + Ayy * Axz) + gupxz * (Axy * Azz + Ayz * Axz) +
+                                   gupyz * (Ayy * Azz + Ayz * Ayz)))) -
+                  F16 * PI * rho;
+            res[0][p] = ham;
+            D1(AXX, gxxx, gxxy, gxxz);
+            D1(AXY, gxyx, gxyy, gxyz);
+            D1(AXZ, gxzx, gxzy, gxzz);
+            D1(AYY, gyyx, gyyy, gyyz);
+            D1(AYZ, gyzx, gyzy, gyzz);
+            D1(AZZ, gzzx, gzzy, gzzz);
+            gxxx = gxxx - (Gamxxx * Axx + Gamyxx * Axy + Gamzxx * Axz + Gamxxx * Axx + Gamyxx * Axy + Gamzxx * Axz) - chix * Axx / chin1;
+            gxyx = gxyx - (Gamxxy * Axx + Gamyxy * Axy + Gamzxy * Axz + Gamxxx * Axy + Gamyxx * Ayy + Gamzxx * Ayz) - chix * Axy / chin1;
+            gxzx = gxzx - (Gamxxz * Axx + Gamyxz * Axy + Gamzxz * Axz + Gamxxx * Axz + Gamyxx * Ayz + Gamzxx * Azz) - chix * Axz / chin1;
+            gyyx = gyyx - (Gamxxy * Axy + Gamyxy * Ayy + Gamzxy * Ayz + Gamxxy * Axy + Gamyxy * Ayy + Gamzxy * Ayz) - chix * Ayy / chin1;
+            gyzx = gyzx - (Gamxxz * Axy + Gamyxz * Ayy + Gamzxz * Ayz + Gamxxy * Axz + Gamyxy * Ayz + Gamzxy * Azz) - chix * Ayz / chin1;
+            gzzx = gzzx - (Gamxxz * Axz + Gamyxz * Ayz + Gamzxz * Azz + Gamxxz * Axz + Gamyxz * Ayz + Gamzxz * Azz) - chix * Azz / chin1;
+            gxxy = gxxy - (Gamxxy * Axx + Gamyxy * Axy + Gamzxy * Axz + Gamxxy * Axx + Gamyxy * Axy + Gamzxy * Axz) - chiy * Axx / chin1;
+            gxyy = gxyy - (Gamxyy * Axx + Gamyyy * Axy + Gamzyy * Axz + Gamxxy * Axy + Gamyxy * Ayy + Gamzxy * Ayz) - chiy * Axy / chin1;
+            gxzy = gxzy - (Gamxyz * Axx + Gamyyz * Axy + Gamzyz * Axz + Gamxxy * Axz + Gamyxy * Ayz + Gamzxy * Azz) - chiy * Axz / chin1;
+            gyyy = gyyy - (Gamxyy * Axy + Gamyyy * Ayy + Gamzyy * Ayz + Gamxyy * Axy + Gamyyy * Ayy + Gamzyy * Ayz) - chiy * Ayy / chin1;
+            gyzy = gyzy - (Gamxyz * Axy + Gamyyz * Ayy + Gamzyz * Ayz + Gamxyy * Axz + Gamyyy * Ayz + Gamzyy * Azz) - chiy * Ayz / chin1;
+            gzzy = gzzy - (Gamxyz * Axz + Gamyyz * Ayz + Gamzyz * Azz + Gamxyz * Axz + Gamyyz * Ayz + Gamzyz * Azz) - chiy * Azz / chin1;
+            gxxz = gxxz - (Gamxxz * Axx + Gamyxz * Axy + Gamzxz * Axz + Gamxxz * Axx + Gamyxz * Axy + Gamzxz * Axz) - chiz * Axx / chin1;
+            gxyz = gxyz - (Gamxyz * Axx + Gamyyz * Axy + Gamzyz * Axz + Gamxxz * Axy + Gamyxz * Ayy + Gamzxz * Ayz) - chiz * Axy / chin1;
+            gxzz = gxzz - (Gamxzz * Axx + Gamyzz * Axy + Gamzzz * Axz + Gamxxz * Axz + Gamyxz * Ayz + Gamzxz * Azz) - chiz * Axz / chin1;
+            gyyz = gyyz - (Gamxyz * Axy + Gamyyz * Ayy + Gamzyz * Ayz + Gamxyz * Axy + Gamyyz * Ayy + Gamzyz * Ayz) - chiz * Ayy / chin1;
+            gyzz = gyzz - (Gamxzz * Axy + Gamyzz * Ayy + Gamzzz * Ayz + Gamxyz * Axz + Gamyyz * Ayz + Gamzyz * Azz) - chiz * Ayz / chin1;
+            gzzz = gzzz - (Gamxzz * Axz + Gamyzz * Ayz + Gamzzz * Azz + Gamxzz * Axz + Gamyzz * Ayz + Gamzzz * Azz) - chiz * Azz / chin1;
+            double movx = gupxx * gxxx + gupyy * gxyy + gupzz * gxzz + gupxy * gxyx + gupxz * gxzx + gupyz * gxzy +
+                          gupxy * gxxy + gupxz * gxxz + gupyz * gxyz;
+            double movy = gupxx * gxyx + gupyy * gyyy + gupzz * gyzz + gupxy * gyyx + gupxz * gyzx + gupyz * gyzy +
+                          gupxy * gxyy + gupxz * gxyz + gupyz * gyyz;
+            double movz = gupxx * gxzx + gupyy * gyzy + gupzz * gzzz + gupxy * gyzx + gupxz * gzzx + gupyz * gzzy +
+                          gupxy * gxzy + gupxz * gxzz + gupyz * gyzz;
+            res[1][p] = movx - F2o3 * Kx - F8 * PI * Sx;
+            res[2][p] = movy - F2o3 * Ky - F8 * PI * Sy;
+            res[3][p] = movz - F2o3 * Kz - F8 * PI * Sz;
+        }
+#undef D1
+#undef D2
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+void orc_enforce_ga(const int *ex, double *const *g, double *const *a)
+{
+    const size_t N = (size_t)ex[0] * ex[1] * ex[2];
+    const double F1o3 = 1.0 / 3.0, ONE = 1.0, TWO = 2.0;
+    for (size_t p = 0; p < N; p++) {
+        double gxx = g[0][p] + ONE, gxy = g[1][p], gxz = g[2][p];
+        double gyy = g[3][p] + ONE, gyz = g[4][p], gzz = g[5][p] + ONE;
+        double gupzz = gxx * gyy * gzz + gxy * gyz * gxz + gxz * gxy * gyz - gxz * gyy * gxz -
+                       gxy * gxy * gzz - gxx * gyz * gyz;
+        gupzz = ONE / pow(gupzz, F1o3);
+        gxx = gxx * gupzz; gxy = gxy * gupzz; gxz = gxz * gupzz;
+        gyy = gyy * gupzz; gyz = gyz * gupzz; gzz = gzz * gupzz;
+        g[0][p] = gxx - ONE; g[1][p] = gxy; g[2][p] = gxz;
+        g[3][p] = gyy - ONE; g[4][p] = gyz; g[5][p] = gzz - ONE;
+        const double gupxx = (gyy * gzz - gyz * gyz);
+        const double gupxy = -(gxy * gzz - gyz * gxz);
+        const double gupxz = (gxy * gyz - gyy * gxz);
+        const double gupyy = (gxx * gzz - gxz * gxz);
+        const double gupyz = -(gxx * gyz - gxy * gxz);
+        gupzz = (gxx * gyy - gxy * gxy);
+        const double trA = gupxx * a[0][p] + gupyy * a[3][p] + gupzz * a[5][p] +
+                           TWO * (gupxy * a[1][p] + gupxz * a[2][p] + gupyz * a[4][p]);
+        a[0][p] = a[0][p] - F1o3 * gxx * trA;
+        a[1][p] = a[1][p] - F1o3 * gxy * trA;
+        a[2][p] = a[2][p] - F1o3 * gxz * trA;
+        a[3][p] = a[3][p] - F1o3 * gyy * trA;
+        a[4][p] = a[4][p] - F1o3 * gyz * trA;
+        a[5][p] = a[5][p] - F1o3 * gzz * trA;
+    }
+}
+
+int orc_rungekutta4_rout(const int *ex, double dT, const double *f0, double *f1, double *f_rhs, int RK4)
+{
+    const size_t N = (size_t)ex[0] * ex[1] * ex[2];
+    const double F1o6 = 1.0 / 6.0, HLF = 0.5, TWO = 2.0;
+    if (RK4 < 0 || RK4 > 3) return 1;
+    for (size_t p = 0; p < N; p++) {
+        if (RK4 == 0) {
+            f1[p] = f0[p] + HLF * dT * f_rhs[p];
+        } else if (RK4 == 1) {
+            f_rhs[p] = f_rhs[p] + TWO * f1[p];
+            f1[p] = f0[p] + HLF * dT * f1[p];
+        } else if (RK4 == 2) {
+            f_rhs[p] = f_rhs[p] + TWO * f1[p];
+            f1[p] = f0[p] + dT * f1[p];
+        } else {
+            f1[p] = f0[p] + F1o6 * dT * (f1[p] + f_rhs[p]);
+        }
+    }
+    return 0;
+}
+
+void orc_lowerboundset(const int *ex, double *chi, double tinny)
+{
+    const size_t N = (size_t)ex[0] * ex[1] * ex[2];
+    for (size_t p = 0; p < N; p++)
+        if (chi[p] < tinny) chi[p] = tinny;
+}
+
+double orc_l2normhelper(const int *ex, const double *X, const double *Y, const double *Z, double xmin,
+                        double ymin, double zmin, double xmax, double ymax, double zmax, const double *f,
+                        int gw)
+{
+    const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+    int imin = gw, jmin = gw, kmin = gw;
+    int imax = ex[0] - gw - 1, jmax = ex[1] - gw - 1, kmax = ex[2] - gw - 1;
+    if (fabs(X[ex[0] - 1] - xmax) < dX) imax = ex[0] - 1;
+    if (fabs(Y[ex[1] - 1] - ymax) < dY) jmax = ex[1] - 1;
+    if (fabs(Z[ex[2] - 1] - zmax) < dZ) kmax = ex[2] - 1;
+    if (fabs(X[0] - xmin) < dX) imin = 0;
+    if (fabs(Y[0] - ymin) < dY) jmin = 0;
+    if (fabs(Z[0] - zmin) < dZ) kmin = 0;
+    double s = 0.0;
+    for (int k = kmin; k <= kmax; k++)
+        for (int j = jmin; j <= jmax; j++)
+            for (int i = imin; i <= imax; i++) {
+                const double v = f[IX(ex, i, j, k)];
+                s += v * v;
+            }
+    return s * dX * dY * dZ;
+}
+
+void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const double *Z, const double *bbox,
+                         const double *f0, double *f, int symmetry)
+{
+    const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+    /* layer boxes (0-based, inclusive) for the six faces: sommerfeld_rout.f90:61-119 */
+    int lay[6][6];
+    int on[6] = {0, 0, 0, 0, 0, 0};
+    const int n0 = ex[0] - 1, n1 = ex[1] - 1, n2 = ex[2] - 1;
+    if (fabs(X[n0] - bbox[3]) < dX) { on[0] = 1; int t[6] = {n0, 0, 0, n0, n1, n2}; memcpy(lay[0], t, sizeof t); }
+    if (fabs(Y[n1] - bbox[4]) < dY) { on[1] = 1; int t[6] = {0, n1, 0, n0, n1, n2}; memcpy(lay[1], t, sizeof t); }
+    if (fabs(Z[n2] - bbox[5]) < dZ) { on[2] = 1; int t[6] = {0, 0, n2, n0, n1, n2}; memcpy(lay[2], t, sizeof t); }
+    if (fabs(X[0] - bbox[0]) < dX && !(symmetry == 2 && fabs(bbox[0]) < dX / 2)) {
+        on[3] = 1; int t[6] = {0, 0, 0, 0, n1, n2}; memcpy(lay[3], t, sizeof t); }
+    if (fabs(Y[0] - bbox[1]) < dY && !(symmetry == 2 && fabs(bbox[1]) < dY / 2)) {
+        on[4] = 1; int t[6] = {0, 0, 0, n0, 0, n2}; memcpy(lay[4], t, sizeof t); }
+    if (fabs(Z[0] - bbox[2]) < dZ && !(symmetry > 0 && fabs(bbox[2]) < dZ / 2)) {
+        on[5] = 1; int t[6] = {0, 0, 0, n0, n1, 0}; memcpy(lay[5], t, sizeof t); }
+    for (int gp = 0; gp < 6; gp++) {
+        if (!on[gp]) continue;
+        for (int k = lay[gp][2]; k <= lay[gp][5]; k++)
+            for (int j = lay[gp][1]; j <= lay[gp][4]; j++)
+                for (int i = lay[gp][0]; i <= lay[gp][3]; i++) f[IX(ex, i, j, k)] = f0[IX(ex, i, j, k)];
+    }
+}
+
+/* Frozen-ghost boundary: every point within gw of an array face takes f0. */
+static void ghost_copy(const int *ex, int gw, const double *f0, double *f)
+{
+    for (int k = 0; k < ex[2]; k++)
+        for (int j = 0; j < ex[1]; j++)
+            for (int i = 0; i < ex[0]; i++) {
+                if (i < gw || j < gw || k < gw || i >= ex[0] - gw || j >= ex[1] - gw || k >= ex[2] - gw) {
+                    const size_t p = IX(ex, i, j, k);
+                    f[p] = f0[p];
+                }
+            }
+}
+
+int orc_bssn_step(const int *ex, const double *X, const double *Y, const double *Z, const double *bbox,
+                  double *const *U, double dT, double eps, double chitiny, int symmetry, int lev, int gauge,
+                  int bc, int gw, double *ham)
+{
+    const size_t N = (size_t)ex[0] * ex[1] * ex[2];
+    double *mem = (double *)calloc(N * (3 * NF + 18 + 6 + 7), sizeof(double));
+    if (!mem) return 2;
+    double *pre[NF], *cor[NF], *acc[NF], *gam[18], *ric[6], *res[7];
+    double *q = mem;
+    for (int v = 0; v < NF; v++) { pre[v] = q; q += N; }
+    for (int v = 0; v < NF; v++) { cor[v] = q; q += N; }
+    for (int v = 0; v < NF; v++) { acc[v] = q; q += N; }
+    for (int a = 0; a < 18; a++) { gam[a] = q; q += N; }
+    for (int a = 0; a < 6; a++) { ric[a] = q; q += N; }
+    for (int a = 0; a < 7; a++) { res[a] = q; q += N; }
+    const double TINNY = chitiny - 1.0; /* bssn_class.C:179 */
+    int err = 0;
+    /* predictor: bssn_class.C:3042-3163 */
+    orc_enforce_ga(ex, U + DXX, U + AXX);
+    err |= orc_compute_rhs_bssn(ex, 0.0, X, Y, Z, (const double *const *)U, acc, NULL, gam, ric, res, symmetry,
+                                lev, eps, 0, gauge);
+    for (int v = 0; v < NF; v++) {
+        orc_rungekutta4_rout(ex, dT, U[v], pre[v], acc[v], 0);
+        if (bc == 0) ghost_copy(ex, gw, U[v], pre[v]);
+        else if (bc == 1) orc_sommerfeld_copy(ex, X, Y, Z, bbox, U[v], pre[v], symmetry);
+    }
+    orc_lowerboundset(ex, pre[CHI], TINNY);
+    if (ham) memcpy(ham, res[0], N * sizeof(double));
+    /* correctors: bssn_class.C:3401-3781 */
+    for (int it = 1; it < 4; it++) {
+        orc_enforce_ga(ex, pre + DXX, pre + AXX);
+        err |= orc_compute_rhs_bssn(ex, 0.0, X, Y, Z, (const double *const *)pre, cor, NULL, gam, ric, res,
+                                    symmetry, lev, eps, 1, gauge);
+        for (int v = 0; v < NF; v++) {
+            orc_rungekutta4_rout(ex, dT, U[v], cor[v], acc[v], it);
+            if (bc == 0) ghost_copy(ex, gw, U[v], cor[v]);
+            else if (bc == 1) orc_sommerfeld_copy(ex, X, Y, Z, bbox, U[v], cor[v], symmetry);
+        }
+        orc_lowerboundset(ex, cor[CHI], TINNY);
+        if (it < 3)
+            for (int v = 0; v < NF; v++) { double *t = pre[v]; pre[v] = cor[v]; cor[v] = t; }
+    }
+    /* rotation: the corrector output becomes the state (bssn_class.C:3845-3859) */
+    for (int v = 0; v < NF; v++) memcpy(U[v], cor[v], N * sizeof(double));
+    free(mem);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11). */
+static inline void mulhilo32(unsigned a, unsigned b, unsigned *hi, unsigned *lo)
+{
+    unsigned long long p = (unsigned long long)a * b;
+    *hi = (unsigned)(p >> 32);
+    *lo = (unsigned)p;
+}
+
+void orc_philox4x32_10(const unsigned *ctr, const unsigned *key, unsigned *out)
+{
+    unsigned c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        unsigned hi0, lo0, hi1, lo1;
+        mulhilo32(0xD2511F53u, c0, &hi0, &lo0);
+        mulhilo32(0xCD9E8D57u, c2, &hi1, &lo1);
+        unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}

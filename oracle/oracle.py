@@ -1,0 +1,303 @@
+"""ctypes front end to the CPU oracle of the BSSN RHS + RK4 path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+benchmark's CPU-baseline leg, never by the product path.
+
+Two back ends with the same Python signatures:
+
+* ``Orc``  -- the C restatement in ``oracle/liborc.so`` (``bssn_oracle.c``);
+* ``Ref``  -- the reference Fortran itself, built by ``build_ref.sh`` into
+  ``oracle/_ref/libamssref.so`` (only where the reference sources were
+  present at build time).  Its configured gauge is GAUGE 2.
+
+Arrays are numpy float64 of shape (nz, ny, nx), C-contiguous, which is the
+reference's Fortran (nx, ny, nz) layout with i fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NF = 24
+FIELDS = ["chi", "trK", "dxx", "gxy", "gxz", "dyy", "gyz", "dzz",
+          "Axx", "Axy", "Axz", "Ayy", "Ayz", "Azz", "Gamx", "Gamy", "Gamz",
+          "Lap", "betax", "betay", "betaz", "dtSfx", "dtSfy", "dtSfz"]
+# Reflection parity of each field (bssn_class.C:183-361).
+PARITY = [(1, 1, 1), (1, 1, 1), (1, 1, 1), (-1, -1, 1), (-1, 1, -1), (1, 1, 1),
+          (1, -1, -1), (1, 1, 1), (1, 1, 1), (-1, -1, 1), (-1, 1, -1), (1, 1, 1),
+          (1, -1, -1), (1, 1, 1), (-1, 1, 1), (1, -1, 1), (1, 1, -1), (1, 1, 1),
+          (-1, 1, 1), (1, -1, 1), (1, 1, -1), (-1, 1, 1), (1, -1, 1), (1, 1, -1)]
+GAM_NAMES = ["Gamxxx", "Gamxxy", "Gamxxz", "Gamxyy", "Gamxyz", "Gamxzz",
+             "Gamyxx", "Gamyxy", "Gamyxz", "Gamyyy", "Gamyyz", "Gamyzz",
+             "Gamzxx", "Gamzxy", "Gamzxz", "Gamzyy", "Gamzyz", "Gamzzz"]
+RIC_NAMES = ["Rxx", "Rxy", "Rxz", "Ryy", "Ryz", "Rzz"]
+RES_NAMES = ["ham_Res", "movx_Res", "movy_Res", "movz_Res", "Gmx_Res", "Gmy_Res", "Gmz_Res"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _ex(shape):
+    nz, ny, nx = shape
+    return (C.c_int * 3)(nx, ny, nz)
+
+
+def _arr_ptrs(arrs):
+    return (_dp * len(arrs))(*[_ptr(a) for a in arrs])
+
+
+def have_ref() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libamssref.so"))
+
+
+class Orc:
+    """C restatement (liborc.so)."""
+
+    def __init__(self, path: str | None = None):
+        path = path or os.path.join(HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run `make -C oracle`)")
+        self.lib = L = C.CDLL(path)
+        L.orc_compute_rhs_bssn.restype = C.c_int
+        L.orc_rungekutta4_rout.restype = C.c_int
+        L.orc_l2normhelper.restype = C.c_double
+        L.orc_bssn_step.restype = C.c_int
+        self.gauge_default = 0
+
+    # stencils ------------------------------------------------------------
+    def fderivs(self, f, X, Y, Z, sym, symmetry=0):
+        fx, fy, fz = (np.zeros_like(f) for _ in range(3))
+        self.lib.orc_fderivs(_ex(f.shape), _ptr(f), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(X), _ptr(Y),
+                             _ptr(Z), C.c_double(sym[0]), C.c_double(sym[1]), C.c_double(sym[2]),
+                             C.c_int(symmetry))
+        return fx, fy, fz
+
+    def fdderivs(self, f, X, Y, Z, sym, symmetry=0):
+        o = [np.zeros_like(f) for _ in range(6)]
+        self.lib.orc_fdderivs(_ex(f.shape), _ptr(f), *[_ptr(a) for a in o], _ptr(X), _ptr(Y), _ptr(Z),
+                              C.c_double(sym[0]), C.c_double(sym[1]), C.c_double(sym[2]),
+                              C.c_int(symmetry))
+        return o
+
+    def lopsided(self, f, f_rhs, bx, by, bz, X, Y, Z, sym, symmetry=0):
+        r = f_rhs.copy()
+        soa = (C.c_double * 3)(*sym)
+        self.lib.orc_lopsided(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(f), _ptr(r), _ptr(bx),
+                              _ptr(by), _ptr(bz), C.c_int(symmetry), soa)
+        return r
+
+    def kodis(self, f, f_rhs, X, Y, Z, sym, eps, symmetry=0):
+        r = f_rhs.copy()
+        soa = (C.c_double * 3)(*sym)
+        self.lib.orc_kodis(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(f), _ptr(r), soa,
+                           C.c_int(symmetry), C.c_double(eps))
+        return r
+
+    # RHS -----------------------------------------------------------------
+    def compute_rhs(self, st, X, Y, Z, eps, co=1, symmetry=0, matter=None, gauge=None, res_init=None):
+        gauge = self.gauge_default if gauge is None else gauge
+        shape = st[0].shape
+        rhs = [np.zeros(shape) for _ in range(NF)]
+        gam = [np.zeros(shape) for _ in range(18)]
+        ric = [np.zeros(shape) for _ in range(6)]
+        res = [np.zeros(shape) if res_init is None else res_init[a].copy() for a in range(7)]
+        mat = _arr_ptrs(matter) if matter is not None else None
+        gont = self.lib.orc_compute_rhs_bssn(
+            _ex(shape), C.c_double(0.0), _ptr(X), _ptr(Y), _ptr(Z), _arr_ptrs(st), _arr_ptrs(rhs), mat,
+            _arr_ptrs(gam), _arr_ptrs(ric), _arr_ptrs(res), C.c_int(symmetry), C.c_int(0),
+            C.c_double(eps), C.c_int(co), C.c_int(gauge))
+        return dict(gont=gont, rhs=rhs, gam=gam, ric=ric, res=res)
+
+    # pointwise / RK4 -------------------------------------------------------
+    def enforce_ga(self, st):
+        """In place on st[2:8] (metric) and st[8:14] (A)."""
+        self.lib.orc_enforce_ga(_ex(st[0].shape), _arr_ptrs(st[2:8]), _arr_ptrs(st[8:14]))
+
+    def rk4(self, f0, f1, frhs, dT, stage):
+        return self.lib.orc_rungekutta4_rout(_ex(f0.shape), C.c_double(dT), _ptr(f0), _ptr(f1), _ptr(frhs),
+                                             C.c_int(stage))
+
+    def lowerboundset(self, chi, tinny):
+        self.lib.orc_lowerboundset(_ex(chi.shape), _ptr(chi), C.c_double(tinny))
+
+    def l2norm(self, f, X, Y, Z, bbox, gw):
+        return self.lib.orc_l2normhelper(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), *[C.c_double(b) for b in bbox],
+                                         _ptr(f), C.c_int(gw))
+
+    def sommerfeld_copy(self, f0, f, X, Y, Z, bbox, symmetry=0):
+        bb = (C.c_double * 6)(*bbox)
+        self.lib.orc_sommerfeld_copy(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, _ptr(f0), _ptr(f),
+                                     C.c_int(symmetry))
+
+    def step(self, U, X, Y, Z, bbox, dT, eps, chitiny=1e-5, symmetry=0, gauge=None, bc=0, gw=3, want_ham=False):
+        """One RK4 step in place on U (list of 24 arrays). Returns (gont, ham)."""
+        gauge = self.gauge_default if gauge is None else gauge
+        ham = np.zeros(U[0].shape) if want_ham else None
+        bb = (C.c_double * 6)(*bbox)
+        gont = self.lib.orc_bssn_step(_ex(U[0].shape), _ptr(X), _ptr(Y), _ptr(Z), bb, _arr_ptrs(U),
+                                      C.c_double(dT), C.c_double(eps), C.c_double(chitiny), C.c_int(symmetry),
+                                      C.c_int(0), C.c_int(gauge), C.c_int(bc), C.c_int(gw),
+                                      _ptr(ham) if ham is not None else None)
+        return gont, ham
+
+    def philox(self, ctr, key):
+        c = (C.c_uint * 4)(*ctr)
+        k = (C.c_uint * 2)(*key)
+        o = (C.c_uint * 4)()
+        self.lib.orc_philox4x32_10(c, k, o)
+        return list(o)
+
+
+class Ref:
+    """The reference Fortran (Fortran calling convention: every argument by reference)."""
+
+    gauge_default = 2
+
+    def __init__(self, opt: str = ""):
+        path = os.path.join(HERE, "_ref", f"libamssref{opt}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference build absent: {path}")
+        self.lib = C.CDLL(path)
+        self.lib.compute_rhs_bssn_.restype = C.c_int
+
+    @staticmethod
+    def _d(x):
+        return C.byref(C.c_double(x))
+
+    @staticmethod
+    def _i(x):
+        return C.byref(C.c_int(x))
+
+    def fderivs(self, f, X, Y, Z, sym, symmetry=0):
+        fx, fy, fz = (np.zeros_like(f) for _ in range(3))
+        self.lib.fderivs_(_ex(f.shape), _ptr(f), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(X), _ptr(Y), _ptr(Z),
+                          self._d(sym[0]), self._d(sym[1]), self._d(sym[2]), self._i(symmetry), self._i(0))
+        return fx, fy, fz
+
+    def fdderivs(self, f, X, Y, Z, sym, symmetry=0):
+        o = [np.zeros_like(f) for _ in range(6)]
+        self.lib.fdderivs_(_ex(f.shape), _ptr(f), *[_ptr(a) for a in o], _ptr(X), _ptr(Y), _ptr(Z),
+                           self._d(sym[0]), self._d(sym[1]), self._d(sym[2]), self._i(symmetry), self._i(0))
+        return o
+
+    def lopsided(self, f, f_rhs, bx, by, bz, X, Y, Z, sym, symmetry=0):
+        r = f_rhs.copy()
+        soa = (C.c_double * 3)(*sym)
+        self.lib.lopsided_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(f), _ptr(r), _ptr(bx), _ptr(by),
+                           _ptr(bz), self._i(symmetry), soa)
+        return r
+
+    def kodis(self, f, f_rhs, X, Y, Z, sym, eps, symmetry=0):
+        r = f_rhs.copy()
+        soa = (C.c_double * 3)(*sym)
+        self.lib.kodis_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(f), _ptr(r), soa, self._i(symmetry),
+                        self._d(eps))
+        return r
+
+    def compute_rhs(self, st, X, Y, Z, eps, co=1, symmetry=0, matter=None, gauge=None, res_init=None):
+        if gauge not in (None, 2):
+            raise ValueError("the reference build is configured with GAUGE 2")
+        shape = st[0].shape
+        rhs = [np.zeros(shape) for _ in range(NF)]
+        gam = [np.zeros(shape) for _ in range(18)]
+        ric = [np.zeros(shape) for _ in range(6)]
+        res = [np.zeros(shape) if res_init is None else res_init[a].copy() for a in range(7)]
+        mat = matter if matter is not None else [np.zeros(shape) for _ in range(10)]
+        args = [_ex(shape), self._d(0.0), _ptr(X), _ptr(Y), _ptr(Z)]
+        args += [_ptr(a) for a in st] + [_ptr(a) for a in rhs] + [_ptr(a) for a in mat]
+        args += [_ptr(a) for a in gam] + [_ptr(a) for a in ric] + [_ptr(a) for a in res]
+        args += [self._i(symmetry), self._i(0), self._d(eps), self._i(co)]
+        gont = self.lib.compute_rhs_bssn_(*args)
+        return dict(gont=gont, rhs=rhs, gam=gam, ric=ric, res=res)
+
+    def enforce_ga(self, st):
+        self.lib.enforce_ga_(_ex(st[0].shape), *[_ptr(a) for a in st[2:14]])
+
+    def rk4(self, f0, f1, frhs, dT, stage):
+        self.lib.rungekutta4_rout_(_ex(f0.shape), self._d(dT), _ptr(f0), _ptr(f1), _ptr(frhs), self._i(stage))
+        return 0
+
+    def lowerboundset(self, chi, tinny):
+        self.lib.lowerboundset_(_ex(chi.shape), _ptr(chi), self._d(tinny))
+
+    def l2norm(self, f, X, Y, Z, bbox, gw):
+        out = C.c_double(0.0)
+        self.lib.l2normhelper_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), *[self._d(b) for b in bbox],
+                               _ptr(f), C.byref(out), self._i(gw))
+        return out.value
+
+    def sommerfeld_copy(self, f0, f, X, Y, Z, bbox, symmetry=0):
+        """sommerfeld_rout with precor == 1 (the copy branch)."""
+        soa = (C.c_double * 3)(1.0, 1.0, 1.0)
+        dummy = np.zeros_like(f0)
+        self.lib.sommerfeld_rout_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), *[self._d(b) for b in bbox],
+                                  self._d(0.0), _ptr(dummy), _ptr(dummy), _ptr(f0), _ptr(f), soa,
+                                  self._i(symmetry), self._i(1))
+
+    def symmetry_bd(self, ord_, f, sym):
+        nz, ny, nx = f.shape
+        out = np.zeros((nz + ord_, ny + ord_, nx + ord_))
+        soa = (C.c_double * 3)(*sym)
+        self.lib.symmetry_bd_(self._i(ord_), _ex(f.shape), _ptr(f), _ptr(out), soa)
+        return out
+
+    def step(self, U, X, Y, Z, bbox, dT, eps, chitiny=1e-5, symmetry=0, gauge=None, bc=0, gw=3, want_ham=False):
+        """One RK4 step driven in the order of bssn_class::Step, calling the
+        reference routines (bssn_class.C:2981-3901); boundary as in Orc.step."""
+        if gauge not in (None, 2):
+            raise ValueError("the reference build is configured with GAUGE 2")
+        return step_driver(self, U, X, Y, Z, bbox, dT, eps, chitiny, symmetry, bc, gw, want_ham)
+
+
+def _ghost_copy(f0, f, gw):
+    nz, ny, nx = f.shape
+    m = np.ones(f.shape, bool)
+    m[gw:nz - gw, gw:ny - gw, gw:nx - gw] = False
+    f[m] = f0[m]
+
+
+def step_driver(be, U, X, Y, Z, bbox, dT, eps, chitiny, symmetry, bc, gw, want_ham):
+    """Python statement of bssn_class::Step for one block, over a back end's
+    array routines (used with the reference build)."""
+    tinny = chitiny - 1.0
+    shape = U[0].shape
+    pre = [np.zeros(shape) for _ in range(NF)]
+    cor = [np.zeros(shape) for _ in range(NF)]
+
+    def bcf(f0, f):
+        if bc == 0:
+            _ghost_copy(f0, f, gw)
+        elif bc == 1:
+            be.sommerfeld_copy(f0, f, X, Y, Z, bbox, symmetry)
+
+    be.enforce_ga(U)
+    o = be.compute_rhs(U, X, Y, Z, eps, co=0, symmetry=symmetry)
+    gont = o["gont"]
+    acc = o["rhs"]
+    ham = o["res"][0].copy() if want_ham else None
+    for v in range(NF):
+        be.rk4(U[v], pre[v], acc[v], dT, 0)
+        bcf(U[v], pre[v])
+    be.lowerboundset(pre[0], tinny)
+    for it in (1, 2, 3):
+        be.enforce_ga(pre)
+        o = be.compute_rhs(pre, X, Y, Z, eps, co=1, symmetry=symmetry)
+        gont |= o["gont"]
+        for v in range(NF):
+            np.copyto(cor[v], o["rhs"][v])
+            be.rk4(U[v], cor[v], acc[v], dT, it)
+            bcf(U[v], cor[v])
+        be.lowerboundset(cor[0], tinny)
+        if it < 3:
+            pre, cor = cor, pre
+    for v in range(NF):
+        np.copyto(U[v], cor[v])
+    return gont, ham
