@@ -1,0 +1,163 @@
+/*
+ * amss_bssn.h -- C ABI of the MI355X (gfx950) BSSN RHS + RK4 library
+ * (libamssbssn.so).
+ *
+ * Two levels of entry points:
+ *
+ * 1. Link-compatible replacements of the reference's Fortran entry points
+ *    (trailing-underscore mangling of -Dfortran3, every argument by
+ *    reference, host arrays in Fortran order, i fastest).  They stage the
+ *    host arrays through the GPU and return the same results.  Each cites
+ *    the reference declaration it replaces (paths relative to
+ *    amss-ncku-python/AMSS_NCKU_source/).
+ *
+ * 2. A device-resident API: device pointers, stream-ordered, no hidden
+ *    host synchronisation except where documented.  Blocks, patches and the
+ *    Step driver live behind an opaque level handle.
+ *
+ * Return codes: 0 ok, 1 non-finite input (the reference's gont = 1),
+ * 2 invalid argument, 3 HIP runtime error, 4 unsupported configuration.
+ * amss_last_error() describes the last non-zero code of the calling thread.
+ * No entry point aborts the process or throws.
+ */
+#ifndef AMSS_BSSN_H
+#define AMSS_BSSN_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMSS_OK 0
+#define AMSS_ERR_NONFINITE 1
+#define AMSS_ERR_ARG 2
+#define AMSS_ERR_HIP 3
+#define AMSS_ERR_UNSUPPORTED 4
+
+/* Library version (major*10000 + minor*100 + patch). */
+int amss_version(void);
+/* Text of the calling thread's last error (empty when none). */
+const char *amss_last_error(void);
+/* Gauge used by the Fortran-ABI replacements: 0 (B^i driver, eta = 2;
+ * bssn_rhs.f90:792-799, the default) or 2 (bssn_rhs.f90:809-820).  The
+ * reference fixes it at compile time (macrodef.fh GAUGE). */
+int amss_set_gauge(int gauge);
+
+/* ---------------------------------------------------------------------------
+ * 1. Fortran-ABI replacements (host pointers)
+ * ------------------------------------------------------------------------- */
+
+/* Replaces f_compute_rhs_bssn (bssn_rhs.h:37-54; bssn_rhs.f90:5-1186).
+ * 98 arguments: ex, T, X, Y, Z, the 24 state arrays, the 24 RHS arrays,
+ * rho, Sx..Sz, Sxx..Szz, the 18 physical Christoffels, the 6 physical Ricci
+ * components, ham/mom x3/Gam x3 residuals (written only when co == 0),
+ * Symmetry, Lev, eps, co.  Returns 1 (nothing written) when the sum of the
+ * first 21 state arrays is NaN.  Symmetry != 0 returns 4. */
+int compute_rhs_bssn_(int *ex, double *T, double *X, double *Y, double *Z,
+                      double *chi, double *trK, double *dxx, double *gxy, double *gxz, double *dyy,
+                      double *gyz, double *dzz, double *Axx, double *Axy, double *Axz, double *Ayy,
+                      double *Ayz, double *Azz, double *Gamx, double *Gamy, double *Gamz, double *Lap,
+                      double *betax, double *betay, double *betaz, double *dtSfx, double *dtSfy,
+                      double *dtSfz, double *chi_rhs, double *trK_rhs, double *gxx_rhs, double *gxy_rhs,
+                      double *gxz_rhs, double *gyy_rhs, double *gyz_rhs, double *gzz_rhs, double *Axx_rhs,
+                      double *Axy_rhs, double *Axz_rhs, double *Ayy_rhs, double *Ayz_rhs, double *Azz_rhs,
+                      double *Gamx_rhs, double *Gamy_rhs, double *Gamz_rhs, double *Lap_rhs,
+                      double *betax_rhs, double *betay_rhs, double *betaz_rhs, double *dtSfx_rhs,
+                      double *dtSfy_rhs, double *dtSfz_rhs, double *rho, double *Sx, double *Sy, double *Sz,
+                      double *Sxx, double *Sxy, double *Sxz, double *Syy, double *Syz, double *Szz,
+                      double *Gamxxx, double *Gamxxy, double *Gamxxz, double *Gamxyy, double *Gamxyz,
+                      double *Gamxzz, double *Gamyxx, double *Gamyxy, double *Gamyxz, double *Gamyyy,
+                      double *Gamyyz, double *Gamyzz, double *Gamzxx, double *Gamzxy, double *Gamzxz,
+                      double *Gamzyy, double *Gamzyz, double *Gamzzz, double *Rxx, double *Rxy,
+                      double *Rxz, double *Ryy, double *Ryz, double *Rzz, double *ham_Res,
+                      double *movx_Res, double *movy_Res, double *movz_Res, double *Gmx_Res,
+                      double *Gmy_Res, double *Gmz_Res, int *Symmetry, int *Lev, double *eps, int *co);
+
+/* Replaces f_rungekutta4_rout (rungekutta4_rout.h:34-36; rungekutta4_rout.f90:94-139).
+ * A stage outside 0..3 returns 2 instead of stopping the program. */
+int rungekutta4_rout_(int *ex, double *dT, double *f0, double *f1, double *f_rhs, int *RK4);
+
+/* Replaces f_enforce_ga (enforce_algebra.h:26-28; enforce_algebra.f90:72-127). */
+void enforce_ga_(int *ex, double *dxx, double *gxy, double *gxz, double *dyy, double *gyz, double *dzz,
+                 double *Axx, double *Axy, double *Axz, double *Ayy, double *Ayz, double *Azz);
+
+/* Replaces f_lowerboundset (fmisc.h:210; fmisc.f90:1644-1657). */
+void lowerboundset_(int *ex, double *chi, double *tinny);
+
+/* Replaces f_l2normhelper (fmisc.h:161-164; fmisc.f90:1263-1309). */
+void l2normhelper_(int *ex, double *X, double *Y, double *Z, double *xmin, double *ymin, double *zmin,
+                   double *xmax, double *ymax, double *zmax, double *f, double *f_out, int *gw);
+
+/* Replaces f_sommerfeld_rout (sommerfeld_rout.h:26-29; sommerfeld_rout.f90:9-221)
+ * for precor == 1 (copy f0 onto the outermost layer of every physical face).
+ * precor == 0 (the interpolating Sommerfeld predictor) is not provided: the
+ * call leaves f untouched and amss_last_error() says so. */
+void sommerfeld_rout_(int *ex, double *X, double *Y, double *Z, double *xmin, double *ymin, double *zmin,
+                      double *xmax, double *ymax, double *zmax, double *dT, double *chi0, double *Lap0,
+                      double *f0, double *f, double *SoA, int *Symmetry, int *precor);
+
+/* ---------------------------------------------------------------------------
+ * 2. Device-resident API
+ * ------------------------------------------------------------------------- */
+
+/* RHS of one block on device arrays (same outputs as compute_rhs_bssn_).
+ * ex: shape with ghosts; h: dX, dY, dZ (= X[1]-X[0] ...); state/rhs: 24
+ * device pointers in the reference field order; matter: 10 pointers or NULL
+ * (vacuum); gam (18), ric (6): required; res (7): required when co == 0.
+ * stream: a hipStream_t or NULL.  flag (device int, may be NULL) is set to 1
+ * when an input is NaN; the call itself does not synchronise. */
+int amss_rhs_bssn_dev(const int *ex, const double *h, const double *const *state, double *const *rhs,
+                      const double *const *matter, double *const *gam, double *const *ric, double *const *res,
+                      int symmetry, double eps, int co, int gauge, int *flag, void *stream);
+
+/* enforce_ga on device arrays (12 pointers: dxx gxy gxz dyy gyz dzz Axx..Azz). */
+int amss_enforce_ga_dev(const int *ex, double *const *ga, void *stream);
+
+/* ---- levels: Blocks + the Step driver ------------------------------------ */
+typedef struct amss_level amss_level;
+
+/* Boundary handling after each RK4 sub-step:
+ *  AMSS_BC_FROZEN_GHOSTS: every point within gw of a patch face that no
+ *    other patch of the level owns keeps the step's base state (the copy
+ *    outer boundary of the benchmark configurations);
+ *  AMSS_BC_COPY_LAYER: the reference's sommerfeld_rout(precor = 1) one-layer
+ *    copy on the patch faces, with the reference's stencils up to the edge;
+ *  AMSS_BC_NONE: no boundary step. */
+#define AMSS_BC_FROZEN_GHOSTS 0
+#define AMSS_BC_COPY_LAYER 1
+#define AMSS_BC_NONE 2
+
+/* Grid-function lists of a Block (bssn_class.C:412-535). */
+#define AMSS_LIST_STATE 0 /* StateList     (t^n)            */
+#define AMSS_LIST_OLD 1   /* OldStateList  (t^{n-1})        */
+#define AMSS_LIST_PRE 2   /* SynchList_pre (stage input)    */
+#define AMSS_LIST_COR 3   /* SynchList_cor (stage output/k) */
+#define AMSS_LIST_RHS 4   /* RHSList       (RK4 accumulator) */
+#define AMSS_LIST_RES 5   /* Cons_Ham, Cons_Px..Pz, Cons_Gx..Gz (7 functions) */
+
+/* One refinement level of npatch patches (one Block each).  shapes: 3 ints
+ * per patch (points, ghosts included); bboxes: 6 doubles per patch (xmin
+ * ymin zmin xmax ymax zmax of the cell-centred array, X = xmin + (i+1/2) h).
+ * All patches share one grid spacing.  Returns NULL on error. */
+amss_level *amss_level_create(int npatch, const int *shapes, const double *bboxes, int gauge, double eps,
+                              double chitiny, int bc, int gw);
+void amss_level_destroy(amss_level *L);
+/* Device pointer of grid function `field` of `list` on `patch`. */
+double *amss_level_field(amss_level *L, int patch, int list, int field);
+/* Copy a whole grid function in (dir 0) or out (dir 1); host_or_dev is any
+ * pointer hipMemcpy accepts.  Synchronous. */
+int amss_level_copy(amss_level *L, int patch, int list, int field, double *host_or_dev, int dir);
+/* Advance nsteps RK4 steps of size dT (bssn_class::Step order).
+ * mode 0: one launch per patch and sub-step; mode 1: batched work queue over
+ * all patches; mode 2: batched, replayed from a captured hipGraph.
+ * want_ham: compute the Hamiltonian residual in the last step's predictor.
+ * Synchronises at the end; returns 1 if any RHS saw a NaN input. */
+int amss_level_step(amss_level *L, double dT, int nsteps, int mode, int want_ham);
+/* sqrt(sum H^2 dV) over the patches' interiors (ghost zones excluded), from
+ * the last predictor that computed residuals (Parallel.C:4214-4243). */
+int amss_level_ham_l2(amss_level *L, double *out);
+/* The stream the level's work is ordered on (a hipStream_t). */
+void *amss_level_stream(amss_level *L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMSS_BSSN_H */

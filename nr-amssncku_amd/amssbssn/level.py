@@ -1,0 +1,88 @@
+"""Python driver of one refinement level on the GPU (the C++ BssnLevel behind
+the C ABI).  Mirrors how bssn_class drives Step for a level: the state lives
+in HBM in each Block's grid functions; Python only moves initial data in and
+results out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib, AmssError
+from .grid import NF, Config
+
+BC_FROZEN_GHOSTS, BC_COPY_LAYER, BC_NONE = 0, 1, 2
+LIST_STATE, LIST_OLD, LIST_PRE, LIST_COR, LIST_RHS, LIST_RES = range(6)
+MODE_PLAIN, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
+
+
+class Level:
+    def __init__(self, patches, gauge: int = 0, eps: float = 0.02, chitiny: float = 1e-5,
+                 bc: int = BC_FROZEN_GHOSTS, gw: int = 3):
+        L = lib()
+        self.patches = list(patches)
+        shapes = (C.c_int * (3 * len(patches)))(*[s for p in patches for s in p.shape])
+        bbs = (C.c_double * (6 * len(patches)))(*[b for p in patches for b in p.bbox])
+        self.h = L.amss_level_create(len(patches), shapes, bbs, gauge, eps, chitiny, bc, gw)
+        if not self.h:
+            raise AmssError("amss_level_create: " + (L.amss_last_error() or b"").decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().amss_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @classmethod
+    def from_config(cls, cfg: Config, gauge: int = 0, bc: int = BC_FROZEN_GHOSTS):
+        lv = cls(cfg.patches, gauge=gauge, eps=cfg.eps, chitiny=cfg.chitiny, bc=bc, gw=cfg.patches[0].gw)
+        for i in range(len(cfg.patches)):
+            lv.set_state(i, cfg.initial_state(i))
+        return lv
+
+    # -- data movement -------------------------------------------------------
+    def set_field(self, patch: int, list_: int, v: int, arr) -> None:
+        if hasattr(arr, "data_ptr"):  # a torch tensor (host or device)
+            ptr = C.c_void_p(arr.data_ptr())
+        else:
+            arr = np.ascontiguousarray(arr, dtype=np.float64)
+            ptr = arr.ctypes.data_as(C.c_void_p)
+        check(lib().amss_level_copy(self.h, patch, list_, v, ptr, 0), "amss_level_copy(in)")
+
+    def get_field(self, patch: int, list_: int, v: int) -> np.ndarray:
+        shp = self.patches[patch].shape
+        out = np.empty((shp[2], shp[1], shp[0]))
+        check(lib().amss_level_copy(self.h, patch, list_, v, out.ctypes.data_as(C.c_void_p), 1),
+              "amss_level_copy(out)")
+        return out
+
+    def set_state(self, patch: int, st) -> None:
+        for v in range(NF):
+            self.set_field(patch, LIST_STATE, v, st[v])
+
+    def get_state(self, patch: int):
+        return [self.get_field(patch, LIST_STATE, v) for v in range(NF)]
+
+    def field_ptr(self, patch: int, list_: int, v: int) -> int:
+        return lib().amss_level_field(self.h, patch, list_, v)
+
+    # -- evolution -------------------------------------------------------------
+    def step(self, dt: float, nsteps: int = 1, mode: int = MODE_PLAIN, want_ham: bool = False) -> int:
+        rc = lib().amss_level_step(self.h, dt, nsteps, mode, 1 if want_ham else 0)
+        if rc not in (0, 1):
+            check(rc, "amss_level_step")
+        return rc
+
+    def ham_l2(self) -> float:
+        out = C.c_double(0.0)
+        check(lib().amss_level_ham_l2(self.h, C.byref(out)), "amss_level_ham_l2")
+        return out.value
+
+    def stream(self) -> int:
+        return lib().amss_level_stream(self.h)

@@ -1,0 +1,454 @@
+// amss_capi.cpp -- C ABI of the gfx950 BSSN library: Fortran-ABI
+// replacements (host arrays staged through HBM), the device-resident API and
+// the level / Step driver.  No entry point throws or aborts; errors are
+// return codes plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/amss_bssn.h"
+#include "amss_host.h"
+
+namespace amss {
+
+static thread_local std::string g_err;
+static int g_gauge = 0;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(AMSS_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- finite-difference coefficients from the coordinate spacings ----------
+static Coef make_coef(const double h[3], double eps) {
+  Coef C;
+  for (int a = 0; a < 3; ++a) {
+    C.h[a] = h[a];
+    C.d1[a] = 1.0 / 12.0 / h[a];    // diff_new.f90:987-989
+    C.d1lo[a] = 1.0 / 2.0 / h[a];   // :991-993
+    C.d2[a] = (1.0 / 12.0) / (h[a] * h[a]);  // :1382-1384
+    C.d2lo[a] = 1.0 / (h[a] * h[a]);         // :1378-1380
+    C.ko[a] = eps / 64.0 / h[a];             // kodiss.f90:139, 192-207
+  }
+  const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+  for (int q = 0; q < 3; ++q) {
+    C.dm[q] = (1.0 / 144.0) / (h[pa[q]] * h[pb[q]]);  // :1390-1392
+    C.dmlo[q] = 0.25 / (h[pa[q]] * h[pb[q]]);         // :1386-1388
+  }
+  return C;
+}
+
+static Box make_box(const int* ex, int lo, int hi_off) {
+  Box b;
+  for (int a = 0; a < 3; ++a) {
+    b.n[a] = ex[a];
+    b.lo[a] = lo;
+    b.hi[a] = ex[a] - hi_off;
+  }
+  b.sy = ex[0];
+  b.sz = (long)ex[0] * ex[1];
+  return b;
+}
+
+// ---- a grow-only device workspace + stream for the host-pointer entry points
+struct HostShim {
+  std::mutex mu;
+  double* buf = nullptr;
+  size_t cap = 0;  // doubles
+  int* flag = nullptr;
+  double* part = nullptr;
+  hipStream_t s = nullptr;
+  int ensure(size_t n) {
+    if (!s) {
+      HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      HIPCHK(hipMalloc(&flag, sizeof(int)));
+      HIPCHK(hipMalloc(&part, 4096 * sizeof(double)));
+    }
+    if (n > cap) {
+      if (buf) HIPCHK(hipFree(buf));
+      buf = nullptr;
+      cap = 0;
+      HIPCHK(hipMalloc(&buf, n * sizeof(double)));
+      cap = n;
+    }
+    return 0;
+  }
+};
+static HostShim g_shim;
+
+// Host-side NaN test with the reference's meaning: the sum of the first 21
+// state arrays is NaN (bssn_rhs.f90:109-138).
+static bool sum_is_nan(double* const* st, long n) {
+  double s = 0.0;
+  for (int v = 0; v < 21; ++v)
+    for (long p = 0; p < n; ++p) s += st[v][p];
+  return s != s;
+}
+
+static int rhs_host(const int* ex, double* X, double* Y, double* Z, double* const* st, double* const* rhs,
+                    double* const* mat, double* const* gam, double* const* ric, double* const* res, int symmetry,
+                    double eps, int co) {
+  if (!ex || ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "compute_rhs_bssn_: shape below 2");
+  if (symmetry != 0) return fail(AMSS_ERR_UNSUPPORTED, "compute_rhs_bssn_: only Symmetry = 0 is provided");
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  if (sum_is_nan(st, n)) return AMSS_ERR_NONFINITE;
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  const int nin = NFIELD + NMATTER, nout = NFIELD + 18 + 6 + 7;
+  if (int r = g_shim.ensure((size_t)(nin + nout) * n)) return r;
+  double* d = g_shim.buf;
+  FieldsIn in, mt;
+  FieldsOut out;
+  AuxOut aux;
+  const size_t bytes = n * sizeof(double);
+  for (int v = 0; v < NFIELD; ++v) {
+    in.p[v] = d + v * n;
+    HIPCHK(hipMemcpyAsync((void*)in.p[v], st[v], bytes, hipMemcpyHostToDevice, g_shim.s));
+  }
+  for (int m = 0; m < NMATTER; ++m) {
+    mt.p[m] = d + (NFIELD + m) * n;
+    HIPCHK(hipMemcpyAsync((void*)mt.p[m], mat[m], bytes, hipMemcpyHostToDevice, g_shim.s));
+  }
+  for (int m = NMATTER; m < NFIELD; ++m) mt.p[m] = mt.p[0];
+  double* o = d + nin * n;
+  for (int v = 0; v < NFIELD; ++v) out.p[v] = o + v * n;
+  for (int a = 0; a < 18; ++a) aux.gam[a] = o + (NFIELD + a) * n;
+  for (int a = 0; a < 6; ++a) aux.ric[a] = o + (NFIELD + 18 + a) * n;
+  for (int a = 0; a < 7; ++a) aux.res[a] = o + (NFIELD + 24 + a) * n;
+  const double h[3] = {X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0]};
+  const Coef C = make_coef(h, eps > 0 ? eps : 0.0);
+  HIPCHK(hipMemsetAsync(g_shim.flag, 0, sizeof(int), g_shim.s));
+  const Box box = make_box(ex, 3, 3);
+  launch_rhs(in, mt, out, aux, box, C, g_gauge, co, g_shim.flag, true, g_shim.s);
+  HIPCHK(hipGetLastError());
+  for (int v = 0; v < NFIELD; ++v) HIPCHK(hipMemcpyAsync(rhs[v], out.p[v], bytes, hipMemcpyDeviceToHost, g_shim.s));
+  for (int a = 0; a < 18; ++a) HIPCHK(hipMemcpyAsync(gam[a], aux.gam[a], bytes, hipMemcpyDeviceToHost, g_shim.s));
+  for (int a = 0; a < 6; ++a) HIPCHK(hipMemcpyAsync(ric[a], aux.ric[a], bytes, hipMemcpyDeviceToHost, g_shim.s));
+  if (co == 0)
+    for (int a = 0; a < 7; ++a) HIPCHK(hipMemcpyAsync(res[a], aux.res[a], bytes, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+}  // namespace amss
+
+using namespace amss;
+
+extern "C" {
+
+int amss_version(void) { return 100; }
+const char* amss_last_error(void) { return g_err.c_str(); }
+int amss_set_gauge(int gauge) {
+  if (gauge != 0 && gauge != 2) return fail(AMSS_ERR_UNSUPPORTED, "amss_set_gauge: gauge 0 or 2");
+  g_gauge = gauge;
+  return 0;
+}
+
+int compute_rhs_bssn_(int* ex, double* T, double* X, double* Y, double* Z, double* chi, double* trK, double* dxx,
+                      double* gxy, double* gxz, double* dyy, double* gyz, double* dzz, double* Axx, double* Axy,
+                      double* Axz, double* Ayy, double* Ayz, double* Azz, double* Gamx, double* Gamy, double* Gamz,
+                      double* Lap, double* betax, double* betay, double* betaz, double* dtSfx, double* dtSfy,
+                      double* dtSfz, double* chi_rhs, double* trK_rhs, double* gxx_rhs, double* gxy_rhs,
+                      double* gxz_rhs, double* gyy_rhs, double* gyz_rhs, double* gzz_rhs, double* Axx_rhs,
+                      double* Axy_rhs, double* Axz_rhs, double* Ayy_rhs, double* Ayz_rhs, double* Azz_rhs,
+                      double* Gamx_rhs, double* Gamy_rhs, double* Gamz_rhs, double* Lap_rhs, double* betax_rhs,
+                      double* betay_rhs, double* betaz_rhs, double* dtSfx_rhs, double* dtSfy_rhs,
+                      double* dtSfz_rhs, double* rho, double* Sx, double* Sy, double* Sz, double* Sxx,
+                      double* Sxy, double* Sxz, double* Syy, double* Syz, double* Szz, double* Gamxxx,
+                      double* Gamxxy, double* Gamxxz, double* Gamxyy, double* Gamxyz, double* Gamxzz,
+                      double* Gamyxx, double* Gamyxy, double* Gamyxz, double* Gamyyy, double* Gamyyz,
+                      double* Gamyzz, double* Gamzxx, double* Gamzxy, double* Gamzxz, double* Gamzyy,
+                      double* Gamzyz, double* Gamzzz, double* Rxx, double* Rxy, double* Rxz, double* Ryy,
+                      double* Ryz, double* Rzz, double* ham_Res, double* movx_Res, double* movy_Res,
+                      double* movz_Res, double* Gmx_Res, double* Gmy_Res, double* Gmz_Res, int* Symmetry, int* Lev,
+                      double* eps, int* co) {
+  (void)T;
+  (void)Lev;
+  double* st[NFIELD] = {chi, trK, dxx, gxy, gxz, dyy, gyz, dzz, Axx, Axy, Axz, Ayy,
+                        Ayz, Azz, Gamx, Gamy, Gamz, Lap, betax, betay, betaz, dtSfx, dtSfy, dtSfz};
+  double* rhs[NFIELD] = {chi_rhs, trK_rhs, gxx_rhs, gxy_rhs, gxz_rhs, gyy_rhs, gyz_rhs, gzz_rhs,
+                         Axx_rhs, Axy_rhs, Axz_rhs, Ayy_rhs, Ayz_rhs, Azz_rhs, Gamx_rhs, Gamy_rhs,
+                         Gamz_rhs, Lap_rhs, betax_rhs, betay_rhs, betaz_rhs, dtSfx_rhs, dtSfy_rhs, dtSfz_rhs};
+  double* mat[NMATTER] = {rho, Sx, Sy, Sz, Sxx, Sxy, Sxz, Syy, Syz, Szz};
+  double* gam[18] = {Gamxxx, Gamxxy, Gamxxz, Gamxyy, Gamxyz, Gamxzz, Gamyxx, Gamyxy, Gamyxz,
+                     Gamyyy, Gamyyz, Gamyzz, Gamzxx, Gamzxy, Gamzxz, Gamzyy, Gamzyz, Gamzzz};
+  double* ric[6] = {Rxx, Rxy, Rxz, Ryy, Ryz, Rzz};
+  double* res[7] = {ham_Res, movx_Res, movy_Res, movz_Res, Gmx_Res, Gmy_Res, Gmz_Res};
+  return rhs_host(ex, X, Y, Z, st, rhs, mat, gam, ric, res, *Symmetry, *eps, *co);
+}
+
+int rungekutta4_rout_(int* ex, double* dT, double* f0, double* f1, double* f_rhs, int* RK4) {
+  if (*RK4 < 0 || *RK4 > 3) return fail(AMSS_ERR_ARG, "rungekutta4_rout_: something is wrong in RK4 counting");
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure(3 * (size_t)n)) return r;
+  double *d0 = g_shim.buf, *d1 = d0 + n, *dr = d1 + n;
+  const size_t b = n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(d0, f0, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(d1, f1, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dr, f_rhs, b, hipMemcpyHostToDevice, g_shim.s));
+  launch_rk4_update(d0, d1, dr, n, *dT, *RK4, g_shim.s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(f1, d1, b, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipMemcpyAsync(f_rhs, dr, b, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+static int enforce_host(int* ex, double* const* ga) {
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure(12 * (size_t)n)) return r;
+  FieldsOut f;
+  for (int v = 0; v < NFIELD; ++v) f.p[v] = nullptr;
+  const size_t b = n * sizeof(double);
+  for (int s = 0; s < 12; ++s) {
+    f.p[DXX + s] = g_shim.buf + s * n;
+    HIPCHK(hipMemcpyAsync(f.p[DXX + s], ga[s], b, hipMemcpyHostToDevice, g_shim.s));
+  }
+  launch_enforce(f, n, g_shim.s);
+  HIPCHK(hipGetLastError());
+  for (int s = 0; s < 12; ++s) HIPCHK(hipMemcpyAsync(ga[s], f.p[DXX + s], b, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+void enforce_ga_(int* ex, double* dxx, double* gxy, double* gxz, double* dyy, double* gyz, double* dzz, double* Axx,
+                 double* Axy, double* Axz, double* Ayy, double* Ayz, double* Azz) {
+  double* ga[12] = {dxx, gxy, gxz, dyy, gyz, dzz, Axx, Axy, Axz, Ayy, Ayz, Azz};
+  enforce_host(ex, ga);
+}
+
+static int lowerbound_host(int* ex, double* chi, double tinny) {
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure((size_t)n)) return r;
+  HIPCHK(hipMemcpyAsync(g_shim.buf, chi, n * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  launch_lowerbound(g_shim.buf, n, tinny, g_shim.s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(chi, g_shim.buf, n * sizeof(double), hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+void lowerboundset_(int* ex, double* chi, double* tinny) { lowerbound_host(ex, chi, *tinny); }
+
+static int l2_host(int* ex, double* X, double* Y, double* Z, const double* bb, double* f, double* out, int gw) {
+  const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+  // owned range: ghosts excluded unless the side lies on the patch boundary (fmisc.f90:1283-1301)
+  int lo[3] = {gw, gw, gw}, hi[3] = {ex[0] - gw, ex[1] - gw, ex[2] - gw};
+  const double* C[3] = {X, Y, Z};
+  const double d[3] = {dX, dY, dZ};
+  for (int a = 0; a < 3; ++a) {
+    if (std::fabs(C[a][ex[a] - 1] - bb[3 + a]) < d[a]) hi[a] = ex[a];
+    if (std::fabs(C[a][0] - bb[a]) < d[a]) lo[a] = 0;
+  }
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure((size_t)n)) return r;
+  HIPCHK(hipMemcpyAsync(g_shim.buf, f, n * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  Box b = make_box(ex, 0, 0);
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = lo[a];
+    b.hi[a] = hi[a] > lo[a] ? hi[a] : lo[a];
+  }
+  const int nb = launch_sumsq(g_shim.buf, b, g_shim.part, 4096, g_shim.s);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part(nb);
+  HIPCHK(hipMemcpyAsync(part.data(), g_shim.part, nb * sizeof(double), hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  double s = 0.0;
+  for (double x : part) s += x;
+  *out = s * dX * dY * dZ;
+  return 0;
+}
+void l2normhelper_(int* ex, double* X, double* Y, double* Z, double* xmin, double* ymin, double* zmin, double* xmax,
+                   double* ymax, double* zmax, double* f, double* f_out, int* gw) {
+  const double bb[6] = {*xmin, *ymin, *zmin, *xmax, *ymax, *zmax};
+  l2_host(ex, X, Y, Z, bb, f, f_out, *gw);
+}
+
+// One-layer faces of sommerfeld_rout's copy branch (sommerfeld_rout.f90:61-137).
+static void copy_faces(const int* ex, const double* X, const double* Y, const double* Z, const double* bb,
+                       int symmetry, bool on[6]) {
+  const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+  on[0] = std::fabs(X[ex[0] - 1] - bb[3]) < dX;
+  on[1] = std::fabs(Y[ex[1] - 1] - bb[4]) < dY;
+  on[2] = std::fabs(Z[ex[2] - 1] - bb[5]) < dZ;
+  on[3] = std::fabs(X[0] - bb[0]) < dX && !(symmetry == 2 && std::fabs(bb[0]) < dX / 2);
+  on[4] = std::fabs(Y[0] - bb[1]) < dY && !(symmetry == 2 && std::fabs(bb[1]) < dY / 2);
+  on[5] = std::fabs(Z[0] - bb[2]) < dZ && !(symmetry > 0 && std::fabs(bb[2]) < dZ / 2);
+}
+
+static int sommerfeld_host(int* ex, double* X, double* Y, double* Z, const double* bb, double* f0, double* f,
+                           int symmetry) {
+  bool on[6];
+  copy_faces(ex, X, Y, Z, bb, symmetry, on);
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure(2 * (size_t)n)) return r;
+  double *d0 = g_shim.buf, *d1 = d0 + n;
+  const size_t b = n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(d0, f0, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(d1, f, b, hipMemcpyHostToDevice, g_shim.s));
+  const int n0 = ex[0] - 1, n1 = ex[1] - 1, n2 = ex[2] - 1;
+  const int lay[6][6] = {{n0, 0, 0, n0, n1, n2}, {0, n1, 0, n0, n1, n2}, {0, 0, n2, n0, n1, n2},
+                         {0, 0, 0, 0, n1, n2},   {0, 0, 0, n0, 0, n2},   {0, 0, 0, n0, n1, 0}};
+  for (int g = 0; g < 6; ++g) {
+    if (!on[g]) continue;
+    Box bx = make_box(ex, 0, 0);
+    for (int a = 0; a < 3; ++a) {
+      bx.lo[a] = lay[g][a];
+      bx.hi[a] = lay[g][3 + a];
+    }
+    launch_copy_box(d0, d1, bx, g_shim.s);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(f, d1, b, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+void sommerfeld_rout_(int* ex, double* X, double* Y, double* Z, double* xmin, double* ymin, double* zmin,
+                      double* xmax, double* ymax, double* zmax, double* dT, double* chi0, double* Lap0, double* f0,
+                      double* f, double* SoA, int* Symmetry, int* precor) {
+  (void)dT;
+  (void)chi0;
+  (void)Lap0;
+  (void)SoA;
+  if (*precor != 1) {
+    fail(AMSS_ERR_UNSUPPORTED, "sommerfeld_rout_: only the precor == 1 copy branch is provided");
+    return;
+  }
+  const double bb[6] = {*xmin, *ymin, *zmin, *xmax, *ymax, *zmax};
+  sommerfeld_host(ex, X, Y, Z, bb, f0, f, *Symmetry);
+}
+
+// ---- device-resident API ----------------------------------------------------
+static double* g_zero = nullptr;  // zero matter arrays for vacuum calls
+static size_t g_zero_cap = 0;
+static std::mutex g_zero_mu;
+
+int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state, double* const* rhs,
+                      const double* const* matter, double* const* gam, double* const* ric, double* const* res,
+                      int symmetry, double eps, int co, int gauge, int* flag, void* stream) {
+  if (!ex || !h || !state || !rhs || !gam || !ric || (co == 0 && !res))
+    return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: null argument");
+  if (symmetry != 0) return fail(AMSS_ERR_UNSUPPORTED, "amss_rhs_bssn_dev: only Symmetry = 0 is provided");
+  if (gauge != 0 && gauge != 2) return fail(AMSS_ERR_UNSUPPORTED, "amss_rhs_bssn_dev: gauge 0 or 2");
+  if (ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: shape below 2");
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  if (n >= (1L << 29)) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: more than 2^29 points per block");
+  hipStream_t s = (hipStream_t)stream;
+  FieldsIn in, mt;
+  FieldsOut out;
+  AuxOut aux;
+  for (int v = 0; v < NFIELD; ++v) {
+    in.p[v] = state[v];
+    out.p[v] = rhs[v];
+  }
+  if (matter) {
+    for (int m = 0; m < NMATTER; ++m) mt.p[m] = matter[m];
+  } else {
+    std::lock_guard<std::mutex> lk(g_zero_mu);
+    if ((size_t)n > g_zero_cap) {
+      if (g_zero) HIPCHK(hipFree(g_zero));
+      g_zero = nullptr;
+      g_zero_cap = 0;
+      HIPCHK(hipMalloc(&g_zero, n * sizeof(double)));
+      HIPCHK(hipMemset(g_zero, 0, n * sizeof(double)));
+      g_zero_cap = n;
+    }
+    for (int m = 0; m < NMATTER; ++m) mt.p[m] = g_zero;
+  }
+  for (int m = NMATTER; m < NFIELD; ++m) mt.p[m] = mt.p[0];
+  for (int a = 0; a < 18; ++a) aux.gam[a] = gam[a];
+  for (int a = 0; a < 6; ++a) aux.ric[a] = ric[a];
+  for (int a = 0; a < 7; ++a) aux.res[a] = res ? res[a] : nullptr;
+  int* fl = flag;
+  if (!fl) {
+    std::lock_guard<std::mutex> lk(g_shim.mu);
+    if (int r = g_shim.ensure(1)) return r;
+    fl = g_shim.flag;
+  }
+  const Coef C = make_coef(h, eps > 0 ? eps : 0.0);
+  launch_rhs(in, mt, out, aux, make_box(ex, 3, 3), C, gauge, co, fl, true, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int amss_enforce_ga_dev(const int* ex, double* const* ga, void* stream) {
+  if (!ex || !ga) return fail(AMSS_ERR_ARG, "amss_enforce_ga_dev: null argument");
+  FieldsOut f;
+  for (int v = 0; v < NFIELD; ++v) f.p[v] = nullptr;
+  for (int s = 0; s < 12; ++s) f.p[DXX + s] = ga[s];
+  launch_enforce(f, (long)ex[0] * ex[1] * ex[2], (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- levels ----------------------------------------------------------------
+struct amss_level {
+  BssnLevel* L;
+};
+
+amss_level* amss_level_create(int npatch, const int* shapes, const double* bboxes, int gauge, double eps,
+                              double chitiny, int bc, int gw) {
+  if (npatch < 1 || !shapes || !bboxes) {
+    fail(AMSS_ERR_ARG, "amss_level_create: bad arguments");
+    return nullptr;
+  }
+  auto* L = new BssnLevel(npatch, shapes, bboxes, gauge, eps, chitiny, bc, gw);
+  if (!L->ok()) {
+    fail(AMSS_ERR_ARG, L->err());
+    delete L;
+    return nullptr;
+  }
+  return new amss_level{L};
+}
+
+void amss_level_destroy(amss_level* L) {
+  if (!L) return;
+  delete L->L;
+  delete L;
+}
+
+double* amss_level_field(amss_level* L, int patch, int list, int field) {
+  if (!L) return nullptr;
+  return L->L->field(patch, list, field);
+}
+
+int amss_level_copy(amss_level* L, int patch, int list, int field, double* ptr, int dir) {
+  if (!L || !ptr) return fail(AMSS_ERR_ARG, "amss_level_copy: null argument");
+  double* f = L->L->field(patch, list, field);
+  if (!f) return fail(AMSS_ERR_ARG, "amss_level_copy: no such grid function");
+  const size_t n = (size_t)L->L->npts(patch);
+  HIPCHK(hipStreamSynchronize(L->L->stream()));
+  if (dir == 0)
+    HIPCHK(hipMemcpy(f, ptr, n * sizeof(double), hipMemcpyDefault));
+  else
+    HIPCHK(hipMemcpy(ptr, f, n * sizeof(double), hipMemcpyDefault));
+  return 0;
+}
+
+int amss_level_step(amss_level* L, double dT, int nsteps, int mode, int want_ham) {
+  if (!L) return fail(AMSS_ERR_ARG, "amss_level_step: null level");
+  return L->L->step(dT, nsteps, mode, want_ham != 0);
+}
+
+int amss_level_ham_l2(amss_level* L, double* out) {
+  if (!L || !out) return fail(AMSS_ERR_ARG, "amss_level_ham_l2: null argument");
+  return L->L->ham_l2(out);
+}
+
+void* amss_level_stream(amss_level* L) { return L ? (void*)L->L->stream() : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// amss_host.h -- host-side storage surface of the BSSN path, shaped like the
+// reference's (Block.h:9-32, var.h:5-25, MyList.h:8-108, MPatch.h:11-48), with
+// grid functions resident in HBM: a Block owns one device slab and fgfs[i]
+// are device pointers into it.  BssnLevel::Step follows bssn_class::Step
+// (bssn_class.C:2981-3901) for one refinement level on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "bssn_kernels.h"
+
+namespace amss {
+
+// A grid-function descriptor (var.h:8-12): name, slot number, parities.
+struct var {
+  char name[20];
+  int sgfn;
+  double SoA[3];
+  double propspeed;
+};
+
+// Singly linked list walked in lockstep by Step (MyList.h:8-108).
+template <class T>
+struct MyList {
+  T* data;
+  MyList* next;
+  explicit MyList(T* d) : data(d), next(nullptr) {}
+  void insert(T* d) {
+    MyList* p = this;
+    while (p->next) p = p->next;
+    p->next = new MyList(d);
+  }
+  void clearList() {
+    MyList* p = next;
+    while (p) {
+      MyList* q = p->next;
+      delete p;
+      p = q;
+    }
+    next = nullptr;
+  }
+};
+
+// Block (Block.h:13-20): shape, bbox, cell-centred coordinates, rank and
+// level, and fgfs[] -- here device pointers into one HBM slab per Block.
+struct Block {
+  int shape[3];
+  double bbox[6];
+  std::vector<double> X[3];  // host copies of the coordinates
+  int rank = 0, lev = 0, cgpu = 1;
+  int fngfs = 0;
+  double* slab = nullptr;        // fngfs * N doubles, one allocation
+  std::vector<double*> fgfs;     // fgfs[i] = slab + i * N
+  long npts() const { return (long)shape[0] * shape[1] * shape[2]; }
+  double getdX(int dir) const { return X[dir][1] - X[dir][0]; }
+  // Block::swapList (Block.C:182-199): swap the slot pointers of two lists.
+  void swapList(MyList<var>* a, MyList<var>* b);
+};
+
+// Patch (MPatch.h:13-19): one Block per patch on a single GPU.
+struct Patch {
+  int lev = 0;
+  int shape[3];
+  double bbox[6];
+  MyList<Block>* blb = nullptr;
+  MyList<Block>* ble = nullptr;
+};
+
+// A contiguous copy of a box of points from one patch to another
+// (the single-rank form of Parallel::Sync's transfer segments).
+struct GhostSeg {
+  int src, dst;       // patch indices
+  int slo[3], dlo[3]; // box origins in source / destination index space
+  int ext[3];         // box extent
+};
+
+class BssnLevel {
+ public:
+  BssnLevel(int npatch, const int* shapes, const double* bboxes, int gauge, double eps, double chitiny, int bc,
+            int gw);
+  ~BssnLevel();
+  bool ok() const { return err_.empty(); }
+  const std::string& err() const { return err_; }
+
+  double* field(int patch, int list, int v);
+  int step(double dT, int nsteps, int mode, bool want_ham);
+  int ham_l2(double* out);
+  hipStream_t stream() const { return stream_; }
+  int npatch() const { return (int)patches_.size(); }
+  long npts(int patch) const;
+
+  // the reference's lists (bssn_class.C:412-535)
+  MyList<var>* OldStateList = nullptr;
+  MyList<var>* StateList = nullptr;
+  MyList<var>* SynchList_pre = nullptr;
+  MyList<var>* SynchList_cor = nullptr;
+  MyList<var>* RHSList = nullptr;
+  MyList<var>* ConsList = nullptr;
+
+ private:
+  int Step(double dT, bool want_res);  // one RK4 step, one launch per patch and sub-step
+  int StepBatched(double dT, bool want_res);
+  void build_ghost_segments();
+  void sync(MyList<var>* list, int stage);
+  FieldsIn list_in(const Block& b, MyList<var>* l) const;
+  FieldsOut list_out(const Block& b, MyList<var>* l) const;
+
+  std::vector<Patch> patches_;
+  std::vector<Block*> blocks_;
+  std::vector<var*> vars_;
+  std::vector<GhostSeg> segs_;
+  std::vector<Coef> coef_;
+  int gauge_, bc_, gw_;
+  double eps_, chitiny_;
+  hipStream_t stream_ = nullptr;
+  int* dflag_ = nullptr;
+  double* dpart_ = nullptr;
+  int nparts_ = 4096;
+  bool ham_valid_ = false;
+  std::string err_;
+};
+
+}  // namespace amss

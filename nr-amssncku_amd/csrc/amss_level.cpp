@@ -1,0 +1,347 @@
+// amss_level.cpp -- Blocks, lists and the RK4 Step of one refinement level
+// on one GPU, in the order of bssn_class::Step (bssn_class.C:2981-3901).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/amss_bssn.h"
+#include "amss_host.h"
+
+namespace amss {
+
+static const char* kNames[NFIELD] = {"phi",  "trK",  "gxx", "gxy",  "gxz", "gyy", "gyz", "gzz",
+                                     "Axx",  "Axy",  "Axz", "Ayy",  "Ayz", "Azz", "Gmx", "Gmy",
+                                     "Gmz",  "Lap",  "Sfx", "Sfy",  "Sfz", "dtSfx", "dtSfy", "dtSfz"};
+static const char* kListSfx[5] = {"0", "o", "", "1", "_rhs"};
+static const char* kConsNames[7] = {"Cons_Ham", "Cons_Px", "Cons_Py", "Cons_Pz", "Cons_Gx", "Cons_Gy", "Cons_Gz"};
+
+void Block::swapList(MyList<var>* a, MyList<var>* b) {
+  while (a && b) {
+    std::swap(fgfs[a->data->sgfn], fgfs[b->data->sgfn]);
+    a = a->next;
+    b = b->next;
+  }
+}
+
+#define LCHK(call)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      err_ = std::string(#call ": ") + hipGetErrorString(e_);                        \
+      return;                                                                        \
+    }                                                                                \
+  } while (0)
+#define RCHK(call)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      err_ = std::string(#call ": ") + hipGetErrorString(e_);                        \
+      return AMSS_ERR_HIP;                                                           \
+    }                                                                                \
+  } while (0)
+
+BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int gauge, double eps, double chitiny,
+                     int bc, int gw)
+    : gauge_(gauge), bc_(bc), gw_(gw), eps_(eps), chitiny_(chitiny) {
+  if (gauge != 0 && gauge != 2) { err_ = "gauge must be 0 or 2"; return; }
+  if (bc < 0 || bc > 2) { err_ = "unknown boundary mode"; return; }
+  if (gw < 0 || gw > 8) { err_ = "ghost width out of range"; return; }
+  // grid-function registry: 5 time levels x 24 fields, then 7 residuals
+  // (bssn_class.C:183-361), parity vectors as in the reference.
+  int sg = 0;
+  for (int l = 0; l < 5; ++l)
+    for (int v = 0; v < NFIELD; ++v) {
+      var* x = new var;
+      std::snprintf(x->name, sizeof x->name, "%s%s", kNames[v], kListSfx[l]);
+      x->sgfn = sg++;
+      for (int a = 0; a < 3; ++a) x->SoA[a] = parity(v, a);
+      x->propspeed = (v == CHI || v == TRK || v == LAP) ? std::sqrt(2.0) : 1.0;
+      vars_.push_back(x);
+    }
+  for (int a = 0; a < 7; ++a) {
+    var* x = new var;
+    std::snprintf(x->name, sizeof x->name, "%s", kConsNames[a]);
+    x->sgfn = sg++;
+    x->SoA[0] = x->SoA[1] = x->SoA[2] = 1.0;
+    x->propspeed = 1.0;
+    vars_.push_back(x);
+  }
+  MyList<var>** lists[5] = {&StateList, &OldStateList, &SynchList_pre, &SynchList_cor, &RHSList};
+  // registry order of time levels: 0 = state, 1 = old, 2 = pre, 3 = cor, 4 = rhs
+  for (int l = 0; l < 5; ++l) {
+    *lists[l] = new MyList<var>(vars_[l * NFIELD]);
+    for (int v = 1; v < NFIELD; ++v) (*lists[l])->insert(vars_[l * NFIELD + v]);
+  }
+  ConsList = new MyList<var>(vars_[5 * NFIELD]);
+  for (int a = 1; a < 7; ++a) ConsList->insert(vars_[5 * NFIELD + a]);
+
+  double h0 = 0.0;
+  for (int pi = 0; pi < npatch; ++pi) {
+    Block* b = new Block;
+    const int* sh = shapes + 3 * pi;
+    const double* bb = bboxes + 6 * pi;
+    for (int a = 0; a < 3; ++a) {
+      b->shape[a] = sh[a];
+      b->bbox[a] = bb[a];
+      b->bbox[3 + a] = bb[3 + a];
+      if (sh[a] < 7) { err_ = "every axis needs at least 7 points"; delete b; return; }
+      // cell-centred coordinates (Block.C:61-64)
+      const double h = (bb[3 + a] - bb[a]) / sh[a];
+      b->X[a].resize(sh[a]);
+      for (int j = 0; j < sh[a]; ++j) b->X[a][j] = bb[a] + (j + 0.5) * h;
+      if (pi == 0 && a == 0) h0 = h;
+      if (std::fabs(h - h0) > 1e-12 * h0) { err_ = "all patches of a level must share one grid spacing"; delete b; return; }
+    }
+    if (b->npts() >= (1L << 29)) { err_ = "more than 2^29 points in one block"; delete b; return; }
+    b->fngfs = (int)vars_.size();
+    const size_t n = b->npts();
+    LCHK(hipMalloc(&b->slab, n * b->fngfs * sizeof(double)));
+    LCHK(hipMemset(b->slab, 0, n * b->fngfs * sizeof(double)));
+    b->fgfs.resize(b->fngfs);
+    for (int i = 0; i < b->fngfs; ++i) b->fgfs[i] = b->slab + i * n;
+    blocks_.push_back(b);
+    Patch P;
+    P.lev = 0;
+    for (int a = 0; a < 3; ++a) {
+      P.shape[a] = sh[a];
+      P.bbox[a] = bb[a];
+      P.bbox[3 + a] = bb[3 + a];
+    }
+    P.blb = P.ble = new MyList<Block>(b);
+    patches_.push_back(P);
+    const double hh[3] = {b->getdX(0), b->getdX(1), b->getdX(2)};
+    coef_.push_back(Coef{});
+    Coef& C = coef_.back();
+    for (int a = 0; a < 3; ++a) {
+      C.h[a] = hh[a];
+      C.d1[a] = 1.0 / 12.0 / hh[a];
+      C.d1lo[a] = 1.0 / 2.0 / hh[a];
+      C.d2[a] = (1.0 / 12.0) / (hh[a] * hh[a]);
+      C.d2lo[a] = 1.0 / (hh[a] * hh[a]);
+      C.ko[a] = (eps > 0 ? eps : 0.0) / 64.0 / hh[a];
+    }
+    const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    for (int q = 0; q < 3; ++q) {
+      C.dm[q] = (1.0 / 144.0) / (hh[pa[q]] * hh[pb[q]]);
+      C.dmlo[q] = 0.25 / (hh[pa[q]] * hh[pb[q]]);
+    }
+  }
+  LCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  LCHK(hipMalloc(&dflag_, sizeof(int)));
+  LCHK(hipMalloc(&dpart_, nparts_ * sizeof(double)));
+  build_ghost_segments();
+}
+
+BssnLevel::~BssnLevel() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (Block* b : blocks_) {
+    if (b->slab) (void)hipFree(b->slab);
+    delete b;
+  }
+  for (Patch& P : patches_) delete P.blb;
+  for (MyList<var>* l : {StateList, OldStateList, SynchList_pre, SynchList_cor, RHSList, ConsList})
+    if (l) {
+      l->clearList();
+      delete l;
+    }
+  for (var* v : vars_) delete v;
+  if (dflag_) (void)hipFree(dflag_);
+  if (dpart_) (void)hipFree(dpart_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+double* BssnLevel::field(int patch, int list, int v) {
+  if (patch < 0 || patch >= (int)blocks_.size()) return nullptr;
+  MyList<var>* l = nullptr;
+  switch (list) {
+    case AMSS_LIST_STATE: l = StateList; break;
+    case AMSS_LIST_OLD: l = OldStateList; break;
+    case AMSS_LIST_PRE: l = SynchList_pre; break;
+    case AMSS_LIST_COR: l = SynchList_cor; break;
+    case AMSS_LIST_RHS: l = RHSList; break;
+    case AMSS_LIST_RES: l = ConsList; break;
+    default: return nullptr;
+  }
+  for (int i = 0; i < v && l; ++i) l = l->next;
+  if (!l || v < 0) return nullptr;
+  return blocks_[patch]->fgfs[l->data->sgfn];
+}
+
+long BssnLevel::npts(int patch) const {
+  if (patch < 0 || patch >= (int)blocks_.size()) return 0;
+  return blocks_[patch]->npts();
+}
+
+FieldsIn BssnLevel::list_in(const Block& b, MyList<var>* l) const {
+  FieldsIn f;
+  for (int v = 0; v < NFIELD; ++v, l = l->next) f.p[v] = b.fgfs[l->data->sgfn];
+  return f;
+}
+FieldsOut BssnLevel::list_out(const Block& b, MyList<var>* l) const {
+  FieldsOut f;
+  for (int v = 0; v < NFIELD; ++v, l = l->next) f.p[v] = b.fgfs[l->data->sgfn];
+  return f;
+}
+
+// Ghost points of patch A that lie in the interior of patch B receive B's
+// values after every sub-step (the intra-level part of Parallel::Sync).
+void BssnLevel::build_ghost_segments() {
+  segs_.clear();
+  if (blocks_.size() < 2) return;
+  double gmin[3] = {1e300, 1e300, 1e300};
+  for (Block* b : blocks_)
+    for (int a = 0; a < 3; ++a) gmin[a] = std::min(gmin[a], b->bbox[a]);
+  const double h = blocks_[0]->getdX(0);
+  std::vector<std::array<int, 3>> org(blocks_.size());
+  for (size_t i = 0; i < blocks_.size(); ++i)
+    for (int a = 0; a < 3; ++a) org[i][a] = (int)std::lround((blocks_[i]->bbox[a] - gmin[a]) / h);
+  const int gw = gw_;
+  for (size_t A = 0; A < blocks_.size(); ++A)
+    for (size_t B = 0; B < blocks_.size(); ++B) {
+      if (A == B) continue;
+      GhostSeg s;
+      bool empty = false;
+      for (int a = 0; a < 3; ++a) {
+        const int alo = org[A][a], ahi = org[A][a] + blocks_[A]->shape[a];            // A's full array
+        const int blo = org[B][a] + gw, bhi = org[B][a] + blocks_[B]->shape[a] - gw;  // B's interior
+        const int lo = std::max(alo, blo), hi = std::min(ahi, bhi);
+        if (hi <= lo) { empty = true; break; }
+        s.dlo[a] = lo - org[A][a];
+        s.slo[a] = lo - org[B][a];
+        s.ext[a] = hi - lo;
+      }
+      if (empty) continue;
+      s.src = (int)B;
+      s.dst = (int)A;
+      segs_.push_back(s);
+    }
+}
+
+void BssnLevel::sync(MyList<var>* list, int /*stage*/) {
+  for (const GhostSeg& g : segs_) {
+    SegDesc d;
+    for (int a = 0; a < 3; ++a) {
+      d.slo[a] = g.slo[a];
+      d.dlo[a] = g.dlo[a];
+      d.ext[a] = g.ext[a];
+    }
+    const Block& S = *blocks_[g.src];
+    const Block& D = *blocks_[g.dst];
+    d.ssy = S.shape[0];
+    d.ssz = (long)S.shape[0] * S.shape[1];
+    d.dsy = D.shape[0];
+    d.dsz = (long)D.shape[0] * D.shape[1];
+    launch_seg_copy(list_in(S, list), list_out(D, list), d, stream_);
+  }
+}
+
+// One RK4 step, one launch per patch and sub-step.
+int BssnLevel::Step(double dT, bool want_res) {
+  const double tinny = chitiny_ - 1.0;  // bssn_class.C:179
+  const double cst[4] = {0.5 * dT, 0.5 * dT, dT, (1.0 / 6.0) * dT};
+  // predictor's enforce_ga on the state, in place (bssn_class.C:3051-3057)
+  for (Block* b : blocks_) launch_enforce(list_out(*b, StateList), b->npts(), stream_);
+  for (int it = 0; it < 4; ++it) {
+    MyList<var>* in = it == 0 ? StateList : SynchList_pre;
+    MyList<var>* out = it == 0 ? SynchList_pre : SynchList_cor;
+    for (size_t pi = 0; pi < blocks_.size(); ++pi) {
+      Block& b = *blocks_[pi];
+      Box box;
+      for (int a = 0; a < 3; ++a) {
+        box.n[a] = b.shape[a];
+        box.lo[a] = 3;
+        box.hi[a] = b.shape[a] - 3;
+      }
+      box.sy = b.shape[0];
+      box.sz = (long)b.shape[0] * b.shape[1];
+      AuxOut aux;
+      for (int a = 0; a < 18; ++a) aux.gam[a] = nullptr;
+      for (int a = 0; a < 6; ++a) aux.ric[a] = nullptr;
+      MyList<var>* c = ConsList;
+      for (int a = 0; a < 7; ++a, c = c->next) aux.res[a] = b.fgfs[c->data->sgfn];
+      Rk4Param R{it, cst[it], tinny};
+      const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
+      launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, box,
+                 coef_[pi], R, gauge_, want_res && it == 0, dflag_, edge, stream_);
+      if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
+        Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
+        launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
+      } else if (bc_ == AMSS_BC_COPY_LAYER) {
+        // sommerfeld_rout(precor = 1) faces (sommerfeld_rout.f90:61-137), no symmetry
+        Faces w{{0, 0, 0}, {0, 0, 0}};
+        const Patch& P = patches_[pi];
+        for (int a = 0; a < 3; ++a) {
+          const double d = b.getdX(a);
+          if (std::fabs(b.X[a][b.shape[a] - 1] - P.bbox[3 + a]) < d) w.hi[a] = 1;
+          if (std::fabs(b.X[a][0] - P.bbox[a]) < d) w.lo[a] = 1;
+        }
+        launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
+      }
+    }
+    sync(out, it);  // Parallel::Sync (bssn_class.C:3340, 3710)
+    if (it > 0 && it < 3)
+      for (Block* b : blocks_) b->swapList(SynchList_pre, SynchList_cor);  // :3765-3781
+  }
+  // rotate: the corrector output becomes the state (:3845-3859)
+  for (Block* b : blocks_) {
+    b->swapList(StateList, SynchList_cor);
+    b->swapList(OldStateList, SynchList_cor);
+  }
+  if (hipGetLastError() != hipSuccess) {
+    err_ = "kernel launch failed";
+    return AMSS_ERR_HIP;
+  }
+  return 0;
+}
+
+int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
+  if (mode != 0) {
+    err_ = "batched modes are not built into this library";
+    return AMSS_ERR_UNSUPPORTED;
+  }
+  RCHK(hipMemsetAsync(dflag_, 0, sizeof(int), stream_));
+  for (int n = 0; n < nsteps; ++n)
+    if (int r = Step(dT, want_ham && n == nsteps - 1)) return r;
+  int flag = 0;
+  RCHK(hipMemcpyAsync(&flag, dflag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  RCHK(hipStreamSynchronize(stream_));
+  if (want_ham) ham_valid_ = true;
+  if (flag) {
+    err_ = "NaN found in a right-hand-side input";
+    return AMSS_ERR_NONFINITE;
+  }
+  return 0;
+}
+
+int BssnLevel::ham_l2(double* out) {
+  if (!ham_valid_) {
+    err_ = "no Hamiltonian residual computed yet (step with want_ham)";
+    return AMSS_ERR_ARG;
+  }
+  double s = 0.0;
+  for (Block* b : blocks_) {
+    Box box;
+    for (int a = 0; a < 3; ++a) {
+      box.n[a] = b->shape[a];
+      box.lo[a] = gw_;
+      box.hi[a] = b->shape[a] - gw_;
+    }
+    box.sy = b->shape[0];
+    box.sz = (long)b->shape[0] * b->shape[1];
+    const int nb = launch_sumsq(b->fgfs[ConsList->data->sgfn], box, dpart_, nparts_, stream_);
+    std::vector<double> part(nb);
+    RCHK(hipMemcpyAsync(part.data(), dpart_, nb * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    RCHK(hipStreamSynchronize(stream_));
+    double t = 0.0;
+    for (double x : part) t += x;
+    s += t * b->getdX(0) * b->getdX(1) * b->getdX(2);
+  }
+  *out = std::sqrt(s);
+  return 0;
+}
+
+}  // namespace amss

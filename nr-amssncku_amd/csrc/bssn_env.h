@@ -1,0 +1,342 @@
+// bssn_env.h -- stencil environments and result sinks for bssn_rhs_point.
+#pragma once
+#include "bssn_algebra.h"
+
+namespace amss {
+
+// ---------------------------------------------------------------------------
+// Interior environment: the point has +-3 room along every axis, so every
+// stencil is the full-order one (diff_new.f90:1084-1087, 1487-1507;
+// lopsidediff.f90:324-372 first branches; kodiss.f90:166-207).  Reads go
+// straight to memory through the vector L1/L2 as raw buffer loads: the field
+// base lives in a scalar buffer descriptor and the (uniform) stencil offset
+// in the scalar soffset, so every load of every field shares one per-lane
+// offset register -- with flat addressing each field and each runtime
+// stride would cost a 64-bit per-lane address pair.
+// Requires 3 + 3 sy + 3 sz <= p and field size < 4 GiB.
+struct InteriorEnv {
+  const FieldsIn& F;
+  const FieldsIn& M;
+  const Coef& C;
+  long p, sy, sz;
+
+  __device__ __forceinline__ double at(int v, long o) const {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void*)F.p[v], (short)0, (int)0xFFFFFFFF, 0x00020000);
+    const int q = 3 + 3 * (int)sy + 3 * (int)sz;  // most negative stencil offset, negated
+    const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, (unsigned)((p - q) * 8), (unsigned)((o + q) * 8), 0);
+    return __builtin_bit_cast(double, t);
+  }
+  __device__ __forceinline__ double val(int v) const { return at(v, 0); }
+  __device__ __forceinline__ double mat(int m) const { return M.p[m][p]; }
+  __device__ __forceinline__ void begin(int) const {}
+  __device__ __forceinline__ void end() const {}
+
+  __device__ __forceinline__ double d1(int v, long s) const {
+    return (at(v, -2 * s) - at(v, 2 * s)) + 8.0 * (at(v, s) - at(v, -s));
+  }
+  __device__ __forceinline__ void grad(int v, double d[3]) const {
+    d[0] = C.d1[0] * d1(v, 1);
+    d[1] = C.d1[1] * d1(v, sy);
+    d[2] = C.d1[2] * d1(v, sz);
+  }
+  __device__ __forceinline__ double row(int v, long o, long s) const {
+    return (at(v, o - 2 * s) - at(v, o + 2 * s)) + 8.0 * (at(v, o + s) - at(v, o - s));
+  }
+  __device__ __forceinline__ double mix(int v, long sa, long sb) const {
+    return (row(v, -2 * sb, sa) - row(v, 2 * sb, sa)) + 8.0 * (row(v, sb, sa) - row(v, -sb, sa));
+  }
+  __device__ __forceinline__ void hess(int v, double d[6]) const {
+    const double c = 30.0 * at(v, 0);
+    d[0] = C.d2[0] * (16.0 * (at(v, -1) + at(v, 1)) - (at(v, -2) + at(v, 2)) - c);
+    d[3] = C.d2[1] * (16.0 * (at(v, -sy) + at(v, sy)) - (at(v, -2 * sy) + at(v, 2 * sy)) - c);
+    d[5] = C.d2[2] * (16.0 * (at(v, -sz) + at(v, sz)) - (at(v, -2 * sz) + at(v, 2 * sz)) - c);
+    d[1] = C.dm[0] * mix(v, 1, sy);
+    d[2] = C.dm[1] * mix(v, 1, sz);
+    d[4] = C.dm[2] * mix(v, sy, sz);
+  }
+  // one upwind direction: beta>0 uses -3,-10,18,-6,1 on (-1..3), beta<0 the mirror
+  __device__ __forceinline__ double up1(int v, long s, double b, double c) const {
+    const double f0 = 10.0 * at(v, 0);
+    const double up = (at(v, 3 * s) - 3.0 * at(v, -s)) + (18.0 * at(v, s) - 6.0 * at(v, 2 * s)) - f0;
+    const double dn = (at(v, -3 * s) - 3.0 * at(v, s)) + (18.0 * at(v, -s) - 6.0 * at(v, -2 * s)) - f0;
+    // branch-free: beta>0 -> beta c up ; beta<0 -> -beta c dn ; beta==0 -> 0
+    return c * (fmax(b, 0.0) * up - fmin(b, 0.0) * dn);
+  }
+  __device__ __forceinline__ double adv(int v, double /*off*/, const double b[3]) const {
+    return up1(v, 1, b[0], C.d1[0]) + up1(v, sy, b[1], C.d1[1]) + up1(v, sz, b[2], C.d1[2]);
+  }
+  __device__ __forceinline__ double ko1(int v, long s, double c20) const {
+    return (at(v, -3 * s) + at(v, 3 * s)) - 6.0 * (at(v, -2 * s) + at(v, 2 * s)) +
+           15.0 * (at(v, -s) + at(v, s)) - c20;
+  }
+  __device__ __forceinline__ double ko(int v) const {
+    const double c20 = 20.0 * at(v, 0);
+    return C.ko[0] * ko1(v, 1, c20) + C.ko[1] * ko1(v, sy, c20) + C.ko[2] * ko1(v, sz, c20);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Edge environment: the reference's behaviour at every point of the array
+// (Symmetry = 0): the derivative order drops for all three directions
+// together when any axis lacks +-2 (then +-1) room, the last index of any
+// axis gets zero derivatives and no advection, the upwind stencil falls back
+// per direction, KO needs +-3 room on every axis, and the halo index -1
+// mirrors index 0 with the field's parity (symmetry_bd, fmisc.f90:877-901).
+struct EdgeEnv {
+  const FieldsIn& F;
+  const FieldsIn& M;
+  const Coef& C;
+  int i, j, k;
+  int n[3];
+  long sy, sz;
+  bool last, kook;
+  int ord;
+
+  __device__ EdgeEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, int i_, int j_, int k_,
+                     const int* n_, long sy_, long sz_)
+      : F(F_), M(M_), C(C_), i(i_), j(j_), k(k_), sy(sy_), sz(sz_) {
+    n[0] = n_[0];
+    n[1] = n_[1];
+    n[2] = n_[2];
+    last = (i == n[0] - 1) || (j == n[1] - 1) || (k == n[2] - 1);
+    auto rm = [&](int r) {
+      return i - r >= 0 && i + r <= n[0] - 1 && j - r >= 0 && j + r <= n[1] - 1 && k - r >= 0 && k + r <= n[2] - 1;
+    };
+    ord = last ? 0 : (rm(2) ? 2 : (rm(1) ? 1 : 0));
+    kook = rm(3);
+  }
+  __device__ double rd(int v, int a, int b, int c, double off) const {
+    double s = 1.0;
+    if (a < 0) { a = -a - 1; s *= parity(v, 0); }
+    if (b < 0) { b = -b - 1; s *= parity(v, 1); }
+    if (c < 0) { c = -c - 1; s *= parity(v, 2); }
+    return (F.p[v][a + b * sy + c * sz] + off) * s;
+  }
+  __device__ double at3(int v, int di, int dj, int dk) const { return rd(v, i + di, j + dj, k + dk, 0.0); }
+  __device__ double val(int v) const { return F.p[v][i + j * sy + k * sz]; }
+  __device__ double mat(int m) const { return M.p[m][i + j * sy + k * sz]; }
+  __device__ void begin(int) const {}
+  __device__ void end() const {}
+
+  __device__ double ax(int v, int axis, int m, double off = 0.0) const {
+    return axis == 0 ? rd(v, i + m, j, k, off) : (axis == 1 ? rd(v, i, j + m, k, off) : rd(v, i, j, k + m, off));
+  }
+  __device__ void grad(int v, double d[3]) const {
+    for (int a = 0; a < 3; ++a) {
+      if (ord == 2)
+        d[a] = C.d1[a] * ((ax(v, a, -2) - ax(v, a, 2)) + 8.0 * (ax(v, a, 1) - ax(v, a, -1)));
+      else if (ord == 1)
+        d[a] = C.d1lo[a] * (ax(v, a, 1) - ax(v, a, -1));
+      else
+        d[a] = 0.0;
+    }
+  }
+  __device__ void hess(int v, double d[6]) const {
+    for (int s = 0; s < 6; ++s) d[s] = 0.0;
+    if (ord == 0) return;
+    const double c = at3(v, 0, 0, 0);
+    if (ord == 2) {
+      for (int a = 0; a < 3; ++a)
+        d[sidx(a, a)] = C.d2[a] * (16.0 * (ax(v, a, -1) + ax(v, a, 1)) - (ax(v, a, -2) + ax(v, a, 2)) - 30.0 * c);
+      const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+      for (int q = 0; q < 3; ++q) {
+        double acc = 0.0;
+        const double w[5] = {1.0, -8.0, 0.0, 8.0, -1.0};
+        for (int bb = -2; bb <= 2; ++bb) {
+          if (bb == 0) continue;
+          double r = 0.0;
+          for (int aa = -2; aa <= 2; ++aa) {
+            if (aa == 0) continue;
+            int o[3] = {0, 0, 0};
+            o[pa[q]] = aa;
+            o[pb[q]] = bb;
+            r += w[aa + 2] * at3(v, o[0], o[1], o[2]);
+          }
+          acc += w[bb + 2] * r;
+        }
+        d[sidx(pa[q], pb[q])] = C.dm[q] * acc;
+      }
+    } else {
+      for (int a = 0; a < 3; ++a)
+        d[sidx(a, a)] = C.d2lo[a] * (ax(v, a, -1) + ax(v, a, 1) - 2.0 * c);
+      const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+      for (int q = 0; q < 3; ++q) {
+        int o[3] = {0, 0, 0};
+        auto f = [&](int a, int b) {
+          o[0] = o[1] = o[2] = 0;
+          o[pa[q]] = a;
+          o[pb[q]] = b;
+          return at3(v, o[0], o[1], o[2]);
+        };
+        d[sidx(pa[q], pb[q])] = C.dmlo[q] * ((f(-1, -1) - f(1, -1)) - (f(-1, 1) - f(1, 1)));
+      }
+    }
+  }
+  // lopsidediff.f90:321-481, one direction
+  __device__ double adv1(int v, int a, double b, double off) const {
+    const int pos = a == 0 ? i : (a == 1 ? j : k);
+    const int nn = n[a];
+    auto up = [&]() {
+      return -3.0 * ax(v, a, -1, off) - 10.0 * ax(v, a, 0, off) + 18.0 * ax(v, a, 1, off) - 6.0 * ax(v, a, 2, off) +
+             ax(v, a, 3, off);
+    };
+    auto dn = [&]() {
+      return -3.0 * ax(v, a, 1, off) - 10.0 * ax(v, a, 0, off) + 18.0 * ax(v, a, -1, off) - 6.0 * ax(v, a, -2, off) +
+             ax(v, a, -3, off);
+    };
+    auto ce = [&]() { return ax(v, a, -2, off) - 8.0 * ax(v, a, -1, off) + 8.0 * ax(v, a, 1, off) - ax(v, a, 2, off); };
+    const double c = C.d1[a];
+    if (b > 0.0) {
+      if (pos + 3 <= nn - 1) return b * c * up();
+      if (pos + 2 <= nn - 1) return b * c * ce();
+      if (pos + 1 <= nn - 1) return -b * c * dn();
+    } else if (b < 0.0) {
+      if (pos - 3 >= 0) return -b * c * dn();
+      if (pos - 2 >= 0) return b * c * ce();
+      if (pos - 1 >= 0) return b * c * up();
+    }
+    return 0.0;
+  }
+  __device__ double adv(int v, double off, const double b[3]) const {
+    if (last) return 0.0;
+    return adv1(v, 0, b[0], off) + adv1(v, 1, b[1], off) + adv1(v, 2, b[2], off);
+  }
+  __device__ double ko(int v) const {
+    if (!kook) return 0.0;
+    const double c20 = 20.0 * at3(v, 0, 0, 0);
+    double r = 0.0;
+    for (int a = 0; a < 3; ++a)
+      r += C.ko[a] * ((ax(v, a, -3) + ax(v, a, 3)) - 6.0 * (ax(v, a, -2) + ax(v, a, 2)) +
+                      15.0 * (ax(v, a, -1) + ax(v, a, 1)) - c20);
+    return r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Extra outputs of the reference ABI: physical Christoffels (18), physical
+// Ricci (6) and constraint residuals (7: ham, mom x3, Gam x3).
+struct AuxOut {
+  double* __restrict__ gam[18];
+  double* __restrict__ ric[6];
+  double* __restrict__ res[7];
+};
+
+// Plain RHS sink: k -> rhs arrays.
+struct RhsSink {
+  const FieldsOut& O;
+  const AuxOut& X;
+  long p;
+  __device__ __forceinline__ void put(int v, double k) { O.p[v][p] = k; }
+  __device__ __forceinline__ void gam(int a, double x) { X.gam[a][p] = x; }
+  __device__ __forceinline__ void ric(int a, double x) { X.ric[a][p] = x; }
+  __device__ __forceinline__ void res(int a, double x) { X.res[a][p] = x; }
+  __device__ __forceinline__ void finish() {}
+};
+
+// enforce_ga (enforce_algebra.f90:72-127) on one point, offsets in place:
+// rescale g to unit determinant, then remove the trace of A taken with the
+// cofactors of the rescaled metric.
+__device__ __forceinline__ void enforce_point(double d[6], double a[6]) {
+  double g[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) g[s] = d[s] + (sdiag(s) ? 1.0 : 0.0);
+  const double det = g[0] * g[3] * g[5] + g[1] * g[4] * g[2] + g[2] * g[1] * g[4] - g[2] * g[3] * g[2] -
+                     g[1] * g[1] * g[5] - g[0] * g[4] * g[4];
+  const double sc = 1.0 / pow(det, 1.0 / 3.0);
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    g[s] *= sc;
+    d[s] = sdiag(s) ? g[s] - 1.0 : g[s];
+  }
+  const double cxx = g[3] * g[5] - g[4] * g[4], cxy = g[2] * g[4] - g[1] * g[5], cxz = g[1] * g[4] - g[3] * g[2];
+  const double cyy = g[0] * g[5] - g[2] * g[2], cyz = g[1] * g[2] - g[0] * g[4], czz = g[0] * g[3] - g[1] * g[1];
+  const double trA = cxx * a[0] + cyy * a[3] + czz * a[5] + 2.0 * (cxy * a[1] + cxz * a[2] + cyz * a[4]);
+#pragma unroll
+  for (int s = 0; s < 6; ++s) a[s] -= (1.0 / 3.0) * g[s] * trA;
+}
+
+// RK4 sub-step sink (rungekutta4_rout.f90:110-128 fused into the RHS
+// epilogue), followed by the lower bound on chi (fmisc.f90:1644-1657) and,
+// for sub-steps 0-2, the enforce_ga that the next RHS call applies to its
+// input (bssn_class.C:3415-3428).  Stage 3 leaves the new state un-enforced
+// exactly as the reference does.
+struct Rk4Sink {
+  const FieldsIn& U0;
+  const FieldsOut& ACC;
+  const FieldsOut& OUT;
+  const AuxOut& X;
+  long p;
+  int stage;
+  double c;      // 0.5 dt, 0.5 dt, dt, dt/6
+  double tinny;  // chitiny - 1
+  double gA[12];
+
+  __device__ __forceinline__ void put(int v, double k) {
+    double kk = k;
+    if (stage == 0)
+      ACC.p[v][p] = k;
+    else if (stage < 3)
+      ACC.p[v][p] = ACC.p[v][p] + 2.0 * k;
+    else
+      kk = k + ACC.p[v][p];
+    double u = U0.p[v][p] + c * kk;
+    if (v == CHI) u = u < tinny ? tinny : u;
+    if (stage < 3 && v >= DXX && v <= AZZ)
+      gA[v - DXX] = u;
+    else
+      OUT.p[v][p] = u;
+  }
+  __device__ __forceinline__ void gam(int, double) {}
+  __device__ __forceinline__ void ric(int, double) {}
+  __device__ __forceinline__ void res(int a, double x) { X.res[a][p] = x; }
+  __device__ __forceinline__ void finish() {
+    if (stage < 3) {
+      enforce_point(gA, gA + 6);
+#pragma unroll
+      for (int s = 0; s < 12; ++s) OUT.p[DXX + s][p] = gA[s];
+    }
+  }
+};
+
+// Same epilogue with the sub-step fixed at compile time: the interior kernels
+// use this form, since a runtime stage makes the compiler duplicate the whole
+// point computation per stage branch (and blows the register budget).
+template <int STAGE>
+struct Rk4SinkT {
+  const FieldsIn& U0;
+  const FieldsOut& ACC;
+  const FieldsOut& OUT;
+  const AuxOut& X;
+  long p;
+  double c, tinny;
+  double gA[12];
+
+  __device__ __forceinline__ void put(int v, double k) {
+    double kk = k;
+    if (STAGE == 0)
+      ACC.p[v][p] = k;
+    else if (STAGE < 3)
+      ACC.p[v][p] = ACC.p[v][p] + 2.0 * k;
+    else
+      kk = k + ACC.p[v][p];
+    double u = U0.p[v][p] + c * kk;
+    if (v == CHI) u = u < tinny ? tinny : u;
+    if (STAGE < 3 && v >= DXX && v <= AZZ)
+      gA[v - DXX] = u;
+    else
+      OUT.p[v][p] = u;
+  }
+  __device__ __forceinline__ void gam(int, double) {}
+  __device__ __forceinline__ void ric(int, double) {}
+  __device__ __forceinline__ void res(int a, double x) { X.res[a][p] = x; }
+  __device__ __forceinline__ void finish() {
+    if (STAGE < 3) {
+      enforce_point(gA, gA + 6);
+#pragma unroll
+      for (int s = 0; s < 12; ++s) OUT.p[DXX + s][p] = gA[s];
+    }
+  }
+};
+
+}  // namespace amss

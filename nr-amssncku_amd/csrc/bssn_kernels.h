@@ -1,0 +1,64 @@
+// bssn_kernels.h -- launch interface of the gfx950 BSSN kernels.
+#pragma once
+#include "bssn_env.h"
+
+namespace amss {
+
+// Interior tile of the plain kernels: kTileX x kTileY points of one z-plane.
+constexpr int kTileX = 32;
+constexpr int kTileY = 8;
+constexpr int kRhsThreads = kTileX * kTileY;
+
+// An array of shape n (with ghosts), strides sy, sz, and a box [lo, hi).
+struct Box {
+  int n[3];
+  int lo[3];
+  int hi[3];
+  long sy, sz;
+};
+
+// Widths of the boundary layers filled after a sub-step, per face.
+struct Faces {
+  int lo[3], hi[3];
+};
+
+// One ghost-exchange box: source origin, destination origin, extent, strides.
+struct SegDesc {
+  int slo[3], dlo[3], ext[3];
+  long ssy, ssz, dsy, dsz;
+};
+
+struct Rk4Param {
+  int stage;     // 0 predictor, 1..3 correctors
+  double c;      // 0.5 dt, 0.5 dt, dt, dt/6
+  double tinny;  // chitiny - 1 (bssn_class.C:179)
+};
+
+// RHS of compute_rhs_bssn on the interior box (and, with edge, on the rest
+// of the array).  mat must point at 10 valid arrays (zeros for vacuum).
+void launch_rhs(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux, const Box& box,
+                const Coef& C, int gauge, int co, int* flag, bool edge, hipStream_t s);
+void launch_rhs_g0(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux, const Box& box,
+                   const Coef& C, int co, int* flag, bool edge, hipStream_t s);
+void launch_rhs_g2(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux, const Box& box,
+                   const Coef& C, int co, int* flag, bool edge, hipStream_t s);
+// RHS fused with RK4 sub-step R.stage (residuals written for stage 0 when want_res).
+void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
+                const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
+                int* flag, bool edge, hipStream_t s);
+void launch_rk4_g0(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
+                   const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
+                   bool edge, hipStream_t s);
+void launch_rk4_g2(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
+                   const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
+                   bool edge, hipStream_t s);
+void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,
+                       double tinny, int* flag, hipStream_t s);
+void launch_seg_copy(const FieldsIn& src, const FieldsOut& dst, const SegDesc& d, hipStream_t s);
+void launch_enforce(const FieldsOut& f, long n, hipStream_t s);
+void launch_lowerbound(double* chi, long n, double tinny, hipStream_t s);
+void launch_rk4_update(const double* f0, double* f1, double* frhs, long n, double dT, int stage, hipStream_t s);
+void launch_copy_box(const double* f0, double* f, const Box& box, hipStream_t s);
+int launch_sumsq(const double* f, const Box& box, double* part, int maxblocks, hipStream_t s);
+
+}  // namespace amss

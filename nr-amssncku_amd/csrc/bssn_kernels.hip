@@ -1,0 +1,174 @@
+// bssn_kernels.hip -- gfx950 kernels of the BSSN RHS + RK4 path.
+//
+//  The point kernels (RHS, RHS fused with RK4) live in bssn_kernels_impl.h
+//  and are instantiated per gauge in bssn_gauge0.hip / bssn_gauge2.hip.
+//  This file holds the small pointwise kernels and the dispatch:
+//  * ghost_fill : the boundary fill of a stage output; seg_copy: ghost exchange.
+//  * enforce, lowerbound, rk4_update, copy_box, l2 partial sums.
+#include "bssn_kernels_impl.h"
+
+namespace amss {
+
+// Boundary fill after an RK4 sub-step: every point within w.lo[a] of the low
+// face or w.hi[a] of the high face of axis a takes the sub-step's base state
+// U0, then the lower bound on chi and, for sub-steps 0-2, the enforce_ga the
+// next RHS applies to its input -- exactly the sequence rk4 -> copy ->
+// lowerboundset -> enforce_ga the reference runs on such points
+// (bssn_class.C:3119-3156, 3415-3525).
+__global__ void ghost_fill_kernel(FieldsIn u0, FieldsOut out, Box box, Faces w, int stage, double tinny, int* flag) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  if (t >= N) return;
+  const int i = t % box.n[0];
+  const int j = (t / box.n[0]) % box.n[1];
+  const int k = t / ((long)box.n[0] * box.n[1]);
+  if (i >= w.lo[0] && i < box.n[0] - w.hi[0] && j >= w.lo[1] && j < box.n[1] - w.hi[1] && k >= w.lo[2] &&
+      k < box.n[2] - w.hi[2])
+    return;
+  const long p = i + j * box.sy + k * box.sz;
+  if (bad_input(u0, p)) *flag = 1;
+  double ga[12];
+#pragma unroll
+  for (int s = 0; s < 12; ++s) ga[s] = u0.p[DXX + s][p];
+  if (stage < 3) enforce_point(ga, ga + 6);
+#pragma unroll
+  for (int s = 0; s < 12; ++s) out.p[DXX + s][p] = ga[s];
+  const double c = u0.p[CHI][p];
+  out.p[CHI][p] = c < tinny ? tinny : c;
+  out.p[TRK][p] = u0.p[TRK][p];
+#pragma unroll
+  for (int v = GAMX; v < NFIELD; ++v) out.p[v][p] = u0.p[v][p];
+}
+
+// Ghost exchange between patches of one level: copy all 24 fields of a box
+// from the source patch into the destination patch (Parallel::Sync for one
+// rank: transfer + data_packer, Parallel.C:3336-3423, 3508-3591, 3677-3775).
+__global__ void seg_copy_kernel(FieldsIn src, FieldsOut dst, SegDesc d) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long N = (long)d.ext[0] * d.ext[1] * d.ext[2];
+  if (t >= N) return;
+  const int a = t % d.ext[0], b = (t / d.ext[0]) % d.ext[1], c = t / ((long)d.ext[0] * d.ext[1]);
+  const long ps = (d.slo[0] + a) + (d.slo[1] + b) * d.ssy + (d.slo[2] + c) * d.ssz;
+  const long pd = (d.dlo[0] + a) + (d.dlo[1] + b) * d.dsy + (d.dlo[2] + c) * d.dsz;
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) dst.p[v][pd] = src.p[v][ps];
+}
+
+__global__ void enforce_kernel(FieldsOut f, long n) {
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double ga[12];
+#pragma unroll
+  for (int s = 0; s < 12; ++s) ga[s] = f.p[DXX + s][p];
+  enforce_point(ga, ga + 6);
+#pragma unroll
+  for (int s = 0; s < 12; ++s) f.p[DXX + s][p] = ga[s];
+}
+
+__global__ void lowerbound_kernel(double* chi, long n, double tinny) {
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double c = chi[p];
+  if (c < tinny) chi[p] = tinny;
+}
+
+// rungekutta4_rout on one field (rungekutta4_rout.f90:94-139).
+__global__ void rk4_update_kernel(const double* __restrict__ f0, double* __restrict__ f1, double* __restrict__ frhs,
+                                  long n, double dT, int stage) {
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (stage == 0) {
+    f1[p] = f0[p] + (0.5 * dT) * frhs[p];
+  } else if (stage == 1) {
+    const double k = f1[p];
+    frhs[p] = frhs[p] + 2.0 * k;
+    f1[p] = f0[p] + (0.5 * dT) * k;
+  } else if (stage == 2) {
+    const double k = f1[p];
+    frhs[p] = frhs[p] + 2.0 * k;
+    f1[p] = f0[p] + dT * k;
+  } else {
+    f1[p] = f0[p] + ((1.0 / 6.0) * dT) * (f1[p] + frhs[p]);
+  }
+}
+
+// f = f0 on the box [lo, hi] (inclusive) -- one face layer of sommerfeld_rout.
+__global__ void copy_box_kernel(const double* __restrict__ f0, double* __restrict__ f, Box box) {
+  const int nx = box.hi[0] - box.lo[0] + 1, ny = box.hi[1] - box.lo[1] + 1, nz = box.hi[2] - box.lo[2] + 1;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nx * ny * nz) return;
+  const int i = box.lo[0] + t % nx, j = box.lo[1] + (t / nx) % ny, k = box.lo[2] + t / ((long)nx * ny);
+  const long p = i + j * box.sy + k * box.sz;
+  f[p] = f0[p];
+}
+
+// Partial sums of f^2 over the box [lo, hi) per block (fmisc.f90:1263-1309).
+__global__ void __launch_bounds__(256) sumsq_kernel(const double* __restrict__ f, Box box, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int nx = box.hi[0] - box.lo[0], ny = box.hi[1] - box.lo[1], nz = box.hi[2] - box.lo[2];
+  const long N = (long)nx * ny * nz;
+  double s = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N; t += (long)gridDim.x * blockDim.x) {
+    const int i = box.lo[0] + t % nx, j = box.lo[1] + (t / nx) % ny, k = box.lo[2] + t / ((long)nx * ny);
+    const double v = f[i + j * box.sy + k * box.sz];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// --------------------------------------------------------------------------
+// Host-side launchers (stream-ordered, no synchronisation).
+
+void launch_rhs(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux, const Box& box,
+                const Coef& C, int gauge, int co, int* flag, bool edge, hipStream_t s) {
+  if (gauge == 2) launch_rhs_g2(in, mat, out, aux, box, C, co, flag, edge, s);
+  else launch_rhs_g0(in, mat, out, aux, box, C, co, flag, edge, s);
+}
+
+void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
+                const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
+                int* flag, bool edge, hipStream_t s) {
+  if (gauge == 2) launch_rk4_g2(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s);
+  else launch_rk4_g0(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s);
+}
+
+void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,
+                       double tinny, int* flag, hipStream_t s) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  ghost_fill_kernel<<<blocks_for(N, 256), 256, 0, s>>>(u0, out, box, w, stage, tinny, flag);
+}
+
+void launch_seg_copy(const FieldsIn& src, const FieldsOut& dst, const SegDesc& d, hipStream_t s) {
+  const long N = (long)d.ext[0] * d.ext[1] * d.ext[2];
+  if (N > 0) seg_copy_kernel<<<blocks_for(N, 256), 256, 0, s>>>(src, dst, d);
+}
+
+void launch_enforce(const FieldsOut& f, long n, hipStream_t s) {
+  enforce_kernel<<<blocks_for(n, 256), 256, 0, s>>>(f, n);
+}
+void launch_lowerbound(double* chi, long n, double tinny, hipStream_t s) {
+  lowerbound_kernel<<<blocks_for(n, 256), 256, 0, s>>>(chi, n, tinny);
+}
+void launch_rk4_update(const double* f0, double* f1, double* frhs, long n, double dT, int stage, hipStream_t s) {
+  rk4_update_kernel<<<blocks_for(n, 256), 256, 0, s>>>(f0, f1, frhs, n, dT, stage);
+}
+void launch_copy_box(const double* f0, double* f, const Box& box, hipStream_t s) {
+  const long N = (long)(box.hi[0] - box.lo[0] + 1) * (box.hi[1] - box.lo[1] + 1) * (box.hi[2] - box.lo[2] + 1);
+  if (N > 0) copy_box_kernel<<<blocks_for(N, 256), 256, 0, s>>>(f0, f, box);
+}
+int launch_sumsq(const double* f, const Box& box, double* part, int maxblocks, hipStream_t s) {
+  const long N = (long)(box.hi[0] - box.lo[0]) * (box.hi[1] - box.lo[1]) * (box.hi[2] - box.lo[2]);
+  int nb = (int)blocks_for(N, 256);
+  if (nb > maxblocks) nb = maxblocks;
+  if (nb < 1) nb = 1;
+  sumsq_kernel<<<nb, 256, 0, s>>>(f, box, part);
+  return nb;
+}
+
+}  // namespace amss

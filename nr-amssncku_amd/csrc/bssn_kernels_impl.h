@@ -1,0 +1,135 @@
+// bssn_kernels_impl.h -- kernel templates of the BSSN RHS + RK4 path.
+// Instantiated once per gauge in bssn_gauge0.hip / bssn_gauge2.hip so the
+// heavy point kernels compile in parallel translation units.
+#pragma once
+#include "bssn_kernels.h"
+
+namespace amss {
+
+#define AMSS_POINT_KERNEL __global__ void __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+
+__device__ __forceinline__ bool bad_input(const FieldsIn& F, long p) {
+  // the reference's NaN test covers the first 21 fields (bssn_rhs.f90:110-113)
+  bool b = false;
+#pragma unroll
+  for (int v = 0; v < 21; ++v) {
+    const double x = F.p[v][p];
+    b |= !(x == x);
+  }
+  return b;
+}
+
+__device__ __forceinline__ bool interior_index(const Box& box, int& i, int& j, int& k) {
+  i = box.lo[0] + blockIdx.x * kTileX + (threadIdx.x % kTileX);
+  j = box.lo[1] + blockIdx.y * kTileY + (threadIdx.x / kTileX);
+  k = box.lo[2] + blockIdx.z;
+  return i < box.hi[0] && j < box.hi[1];
+}
+__device__ __forceinline__ bool edge_index(const Box& box, int& i, int& j, int& k) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  if (t >= N) return false;
+  i = t % box.n[0];
+  j = (t / box.n[0]) % box.n[1];
+  k = t / ((long)box.n[0] * box.n[1]);
+  return !(i >= box.lo[0] && i < box.hi[0] && j >= box.lo[1] && j < box.hi[1] && k >= box.lo[2] && k < box.hi[2]);
+}
+
+// ---- plain RHS (the compute_rhs_bssn outputs) ------------------------------
+template <int GAUGE, bool CO>
+AMSS_POINT_KERNEL rhs_interior_kernel(FieldsIn in, FieldsIn mat, FieldsOut out, AuxOut aux, Box box, Coef C,
+                                      int* flag) {
+  int i, j, k;
+  if (!interior_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  if (bad_input(in, p)) *flag = 1;
+  InteriorEnv E{in, mat, C, p, box.sy, box.sz};
+  RhsSink S{out, aux, p};
+  bssn_rhs_point<GAUGE, CO, true, true>(E, S);
+}
+
+template <int GAUGE, bool CO>
+__global__ void __launch_bounds__(256) rhs_edge_kernel(FieldsIn in, FieldsIn mat, FieldsOut out, AuxOut aux,
+                                                       Box box, Coef C, int* flag) {
+  int i, j, k;
+  if (!edge_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  if (bad_input(in, p)) *flag = 1;
+  EdgeEnv E(in, mat, C, i, j, k, box.n, box.sy, box.sz);
+  RhsSink S{out, aux, p};
+  bssn_rhs_point<GAUGE, CO, true, true>(E, S);
+}
+
+// ---- RHS fused with the RK4 sub-step ----------------------------------------
+template <int GAUGE, int STAGE, bool CO>
+AMSS_POINT_KERNEL rk4_interior_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
+                                      Coef C, Rk4Param R, int* flag) {
+  int i, j, k;
+  if (!interior_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  if (bad_input(in, p)) *flag = 1;
+  InteriorEnv E{in, in, C, p, box.sy, box.sz};
+  Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  S.finish();
+}
+
+template <int GAUGE, bool CO>
+__global__ void __launch_bounds__(256) rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
+                                                       AuxOut aux, Box box, Coef C, Rk4Param R, int* flag) {
+  int i, j, k;
+  if (!edge_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  if (bad_input(in, p)) *flag = 1;
+  EdgeEnv E(in, in, C, i, j, k, box.n, box.sy, box.sz);
+  Rk4Sink S{u0, acc, out, aux, p, R.stage, R.c, R.tinny, {}};
+  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  S.finish();
+}
+
+inline dim3 interior_grid(const Box& b) {
+  return dim3((b.hi[0] - b.lo[0] + kTileX - 1) / kTileX, (b.hi[1] - b.lo[1] + kTileY - 1) / kTileY,
+              b.hi[2] - b.lo[2]);
+}
+inline bool box_empty(const Box& b) { return b.hi[0] <= b.lo[0] || b.hi[1] <= b.lo[1] || b.hi[2] <= b.lo[2]; }
+inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <int G>
+void launch_rhs_gauge(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux,
+                      const Box& box, const Coef& C, int co, int* flag, bool edge, hipStream_t s) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  if (co == 0) {
+    if (!box_empty(box)) rhs_interior_kernel<G, true><<<interior_grid(box), kRhsThreads, 0, s>>>(in, mat, out, aux, box, C, flag);
+    if (edge) rhs_edge_kernel<G, true><<<blocks_for(N, 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
+  } else {
+    if (!box_empty(box)) rhs_interior_kernel<G, false><<<interior_grid(box), kRhsThreads, 0, s>>>(in, mat, out, aux, box, C, flag);
+    if (edge) rhs_edge_kernel<G, false><<<blocks_for(N, 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
+  }
+}
+
+template <int G>
+void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
+                      const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
+                      bool edge, hipStream_t s) {
+  if (!box_empty(box)) {
+    const dim3 gr = interior_grid(box);
+    switch (R.stage) {
+      case 0:
+        if (want_res) rk4_interior_kernel<G, 0, true><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);
+        else rk4_interior_kernel<G, 0, false><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);
+        break;
+      case 1: rk4_interior_kernel<G, 1, false><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag); break;
+      case 2: rk4_interior_kernel<G, 2, false><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag); break;
+      default: rk4_interior_kernel<G, 3, false><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag); break;
+    }
+  }
+  if (edge) {
+    const long N = (long)box.n[0] * box.n[1] * box.n[2];
+    if (want_res && R.stage == 0)
+      rk4_edge_kernel<G, true><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);
+    else
+      rk4_edge_kernel<G, false><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);
+  }
+}
+
+}  // namespace amss
