@@ -1,0 +1,126 @@
+#!/usr/bin/env python3
+"""Benchmark of the BSSN RHS + RK4 hot path on one MI355X.
+
+Workload: the two-puncture configuration on one 128^3 patch (+3 ghost
+zones) over [-16,16]^3, GAUGE 0, KO eps 0.02, dt = h/4, frozen ghost zones.
+A "step" is one RK4 step = 4 right-hand-side evaluations of every interior
+point, run by the fused RHS + RK4 kernel of libamssbssn.so.
+
+Prints one JSON line.  value = interior points x RHS evaluations / wall
+time of the timed steps (Mpoints/s), inputs resident in HBM.  The roofline
+object uses the average duration of the fused kernel's launches, measured
+with HIP events on the level's stream during the timed region.  The CPU
+baseline times the reference Fortran (built from the reference sources,
+OpenMP-style z-slab threads) on the host cores on a short sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nr-amssncku_amd"))
+
+METRIC = "BSSN RHS Mpoints/s + % FP64/HBM roofline; batched vs plain; adjoint/forward"
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the fused kernel from the committed rocprof
+    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), if present."""
+    p = os.path.join(ROOT, "profiles", "rk4_interior_c1_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(cfg, nsteps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cpu_baseline as CB  # the oracle package's harness (CPU baseline leg only)
+    t, kind, cores = CB.time_reference_steps(cfg, nsteps=nsteps)
+    mpts = cfg.interior_pts * 4 / t / 1e6
+    sample = (f"{nsteps} RK4 step(s) of the same 128^3 workload after 1 warm-up, z-slab threads with 3 ghost "
+              f"planes, {'reference Fortran (amdflang -O3, its configured GAUGE 2)' if kind == 'reference' else 'C restatement'}; "
+              f"{t:.2f} s per step")
+    return dict(value=round(mpts, 3), unit="Mpoints/s", cores=cores, kind=kind, sample=sample)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128, help="interior points per axis")
+    ap.add_argument("--mode", choices=["plain", "batched", "graph"], default="plain")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design only)", file=sys.stderr)
+        return 2
+
+    import numpy as np  # noqa: F401
+    import amssbssn as A
+    from amssbssn import perf
+
+    cfg = A.config1(n=args.n)
+    mode = {"plain": A.MODE_PLAIN, "batched": A.MODE_BATCHED, "graph": A.MODE_GRAPH}[args.mode]
+    lv = A.Level.from_config(cfg, gauge=0)
+    pts = cfg.interior_pts
+
+    if args.warmup:
+        rc = lv.step(cfg.dt, args.warmup, mode)
+        assert rc == 0, f"warm-up step returned {rc}"
+    lv.set_timing(True)
+    t0 = time.perf_counter()
+    rc = lv.step(cfg.dt, args.steps, mode)   # synchronises at the end
+    wall = time.perf_counter() - t0
+    assert rc == 0, f"timed steps returned {rc}"
+    ms, nl = lv.kernel_ms()
+    lv.set_timing(False)
+
+    value = pts * 4 * args.steps / wall / 1e6
+    ktime = sum(ms) / 1e3
+    launches = sum(nl)
+    flops = perf.FLOPS_PER_PT_RHS * pts * launches
+    nbytes = sum(b * pts * n for b, n in zip(perf.BYTES_PER_PT_STAGE, nl))
+    ach_gbs = nbytes / ktime / 1e9
+    ach_tf = flops / ktime / 1e12
+    ai = perf.FLOPS_PER_PT_RHS / (sum(perf.BYTES_PER_PT_STAGE) / 4)
+    ridge = perf.PEAK_FP64_TFLOPS * 1e12 / (perf.PEAK_HBM_GBS * 1e9)
+    traffic = traffic_from_profiles()
+    roof = dict(bound="hbm" if ai < ridge else "fp64", achieved=round(ach_gbs, 1), peak=perf.PEAK_HBM_GBS,
+                unit="GB/s", frac=round(ach_gbs / perf.PEAK_HBM_GBS, 4), traffic=traffic,
+                kernel="rk4_interior_kernel (fused RHS + RK4 sub-step, stages 0-3)",
+                avg_launch_ms=round(sum(ms) / launches, 4), launches=launches,
+                algorithmic_bytes_per_launch=int(nbytes / launches),
+                algorithmic_flops_per_launch=int(flops / launches),
+                fp64=dict(achieved=round(ach_tf, 3), peak=perf.PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                          frac=round(ach_tf / perf.PEAK_FP64_TFLOPS, 4)),
+                intensity_flop_per_byte=round(ai, 2), ridge_flop_per_byte=round(ridge, 2))
+    if roof["bound"] == "fp64":
+        roof.update(achieved=round(ach_tf, 3), peak=perf.PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                    frac=round(ach_tf / perf.PEAK_FP64_TFLOPS, 4))
+    out = dict(metric=METRIC, value=round(value, 2), unit="Mpoints/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(wall * 1e3 / args.steps, 3), higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic: analytic two-puncture (Brill-Lindquist) initial data, Philox A_ij noise",
+               config=dict(workload=f"two-puncture BSSN RK4 step, one {args.n}^3 patch + 3 ghosts over [-16,16]^3",
+                           interior_points=pts, rhs_evals_per_step=4, gauge=0, ko_eps=cfg.eps, dt=cfg.dt,
+                           mode=args.mode, stage_kernel_ms=[round(m / max(n, 1), 4) for m, n in zip(ms, nl)]),
+               roofline=roof)
+    lv.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
+        out["cpu_baseline"]["gpu_speedup"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -156,6 +156,11 @@ int amss_level_step(amss_level *L, double dT, int nsteps, int mode, int want_ham
 int amss_level_ham_l2(amss_level *L, double *out);
 /* The stream the level's work is ordered on (a hipStream_t). */
 void *amss_level_stream(amss_level *L);
+/* HIP-event timing of the fused RHS + RK4 interior kernel launches: enable
+ * (resets the totals) or disable; then read the summed milliseconds and the
+ * launch counts per sub-step 0..3 since the last enable. */
+int amss_level_timing(amss_level *L, int enable);
+int amss_level_kernel_ms(amss_level *L, double *ms4, long *launches4);
 
 #ifdef __cplusplus
 }

@@ -86,3 +86,13 @@ class Level:
 
     def stream(self) -> int:
         return lib().amss_level_stream(self.h)
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().amss_level_timing(self.h, 1 if on else 0), "amss_level_timing")
+
+    def kernel_ms(self):
+        """(summed ms, launch count) per RK4 sub-step of the fused interior kernel."""
+        ms = (C.c_double * 4)()
+        n = (C.c_long * 4)()
+        check(lib().amss_level_kernel_ms(self.h, ms, n), "amss_level_kernel_ms")
+        return list(ms), list(n)

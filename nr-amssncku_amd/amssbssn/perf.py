@@ -1,0 +1,23 @@
+"""Work and roofline constants of the BSSN RHS + RK4 path on MI355X.
+
+FLOPS_PER_PT_RHS is the operation count of the reference algorithm as
+written (bssn_rhs.f90 with its stencil routines, GAUGE 0) for one RHS
+evaluation at an interior point: add/sub/mul/div one each, counted by
+running the C restatement (bit-identical to the reference) with a counting
+scalar type (oracle/flopcount.cpp): 4394 add/sub + 2826 mul + 376 div.
+The predictor's co == 0 call adds the constraint residuals (8492 total).
+
+BYTES_PER_PT_STAGE are the compulsory HBM bytes of the fused RHS + RK4
+kernel per interior point and RK4 sub-step: stage 0 reads the state and
+writes the accumulator and the stage output (72 FP64 fields); stages 1-2
+read the stage input, base state and accumulator and write accumulator and
+output (120); stage 3 reads input, base and accumulator and writes the new
+state (96).  Halo re-reads are not compulsory and not counted.
+"""
+FLOPS_PER_PT_RHS = 7596
+FLOPS_PER_PT_RHS_CO0 = 8492
+BYTES_PER_PT_STAGE = (72 * 8, 120 * 8, 120 * 8, 96 * 8)
+
+# MI355X published peaks
+PEAK_HBM_GBS = 8000.0        # HBM3E, GB/s
+PEAK_FP64_TFLOPS = 78.6      # FP64 vector (= FP64 matrix on CDNA4), TFLOP/s

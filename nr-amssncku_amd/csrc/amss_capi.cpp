@@ -451,4 +451,14 @@ int amss_level_ham_l2(amss_level* L, double* out) {
 
 void* amss_level_stream(amss_level* L) { return L ? (void*)L->L->stream() : nullptr; }
 
+int amss_level_timing(amss_level* L, int enable) {
+  if (!L) return fail(AMSS_ERR_ARG, "amss_level_timing: null level");
+  return L->L->set_timing(enable != 0);
+}
+
+int amss_level_kernel_ms(amss_level* L, double* ms4, long* n4) {
+  if (!L || !ms4 || !n4) return fail(AMSS_ERR_ARG, "amss_level_kernel_ms: null argument");
+  return L->L->kernel_ms(ms4, n4);
+}
+
 }  // extern "C"

@@ -88,6 +88,9 @@ class BssnLevel {
   int step(double dT, int nsteps, int mode, bool want_ham);
   int ham_l2(double* out);
   hipStream_t stream() const { return stream_; }
+  // HIP-event timing of the fused RHS+RK4 interior launches (per stage)
+  int set_timing(bool on);
+  int kernel_ms(double* ms4, long* n4);
   int npatch() const { return (int)patches_.size(); }
   long npts(int patch) const;
 
@@ -119,6 +122,12 @@ class BssnLevel {
   double* dpart_ = nullptr;
   int nparts_ = 4096;
   bool ham_valid_ = false;
+  bool timing_ = false;
+  struct Ev { hipEvent_t a, b; int stage; };
+  std::vector<Ev> ev_;       // event pairs of the current step() call
+  size_t ev_used_ = 0;
+  double ms_[4] = {0, 0, 0, 0};
+  long nl_[4] = {0, 0, 0, 0};
   std::string err_;
 };
 

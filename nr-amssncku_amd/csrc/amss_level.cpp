@@ -149,6 +149,10 @@ BssnLevel::~BssnLevel() {
       delete l;
     }
   for (var* v : vars_) delete v;
+  for (Ev& e : ev_) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
   if (dflag_) (void)hipFree(dflag_);
   if (dpart_) (void)hipFree(dpart_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -265,8 +269,27 @@ int BssnLevel::Step(double dT, bool want_res) {
       for (int a = 0; a < 7; ++a, c = c->next) aux.res[a] = b.fgfs[c->data->sgfn];
       Rk4Param R{it, cst[it], tinny};
       const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
+      Ev* ev = nullptr;
+      if (timing_) {
+        if (ev_used_ == ev_.size()) {
+          Ev e{};
+          RCHK(hipEventCreate(&e.a));
+          RCHK(hipEventCreate(&e.b));
+          ev_.push_back(e);
+        }
+        ev = &ev_[ev_used_++];
+        ev->stage = it;
+        RCHK(hipEventRecord(ev->a, stream_));
+      }
       launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, box,
-                 coef_[pi], R, gauge_, want_res && it == 0, dflag_, edge, stream_);
+                 coef_[pi], R, gauge_, want_res && it == 0, dflag_, false, stream_);
+      if (ev) RCHK(hipEventRecord(ev->b, stream_));
+      if (edge) {
+        Box all = box;
+        for (int a = 0; a < 3; ++a) { all.lo[a] = 3; all.hi[a] = b.shape[a] - 3; }
+        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, all,
+                   coef_[pi], R, gauge_, want_res && it == 0, dflag_, true, stream_, /*interior=*/false);
+      }
       if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
         Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
         launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
@@ -304,15 +327,40 @@ int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
     return AMSS_ERR_UNSUPPORTED;
   }
   RCHK(hipMemsetAsync(dflag_, 0, sizeof(int), stream_));
+  ev_used_ = 0;
   for (int n = 0; n < nsteps; ++n)
     if (int r = Step(dT, want_ham && n == nsteps - 1)) return r;
   int flag = 0;
   RCHK(hipMemcpyAsync(&flag, dflag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
   RCHK(hipStreamSynchronize(stream_));
+  for (size_t i = 0; i < ev_used_; ++i) {
+    float ms = 0.f;
+    RCHK(hipEventElapsedTime(&ms, ev_[i].a, ev_[i].b));
+    ms_[ev_[i].stage] += ms;
+    nl_[ev_[i].stage] += 1;
+  }
+  ev_used_ = 0;
   if (want_ham) ham_valid_ = true;
   if (flag) {
     err_ = "NaN found in a right-hand-side input";
     return AMSS_ERR_NONFINITE;
+  }
+  return 0;
+}
+
+int BssnLevel::set_timing(bool on) {
+  timing_ = on;
+  for (int a = 0; a < 4; ++a) {
+    ms_[a] = 0.0;
+    nl_[a] = 0;
+  }
+  return 0;
+}
+
+int BssnLevel::kernel_ms(double* ms4, long* n4) {
+  for (int a = 0; a < 4; ++a) {
+    ms4[a] = ms_[a];
+    n4[a] = nl_[a];
   }
   return 0;
 }

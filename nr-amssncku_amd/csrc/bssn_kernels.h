@@ -45,13 +45,13 @@ void launch_rhs_g2(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out
 // RHS fused with RK4 sub-step R.stage (residuals written for stage 0 when want_res).
 void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                 const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
-                int* flag, bool edge, hipStream_t s);
+                int* flag, bool edge, hipStream_t s, bool interior = true);
 void launch_rk4_g0(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                    const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                   bool edge, hipStream_t s);
+                   bool edge, hipStream_t s, bool interior);
 void launch_rk4_g2(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                    const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                   bool edge, hipStream_t s);
+                   bool edge, hipStream_t s, bool interior);
 void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,
                        double tinny, int* flag, hipStream_t s);
 void launch_seg_copy(const FieldsIn& src, const FieldsOut& dst, const SegDesc& d, hipStream_t s);

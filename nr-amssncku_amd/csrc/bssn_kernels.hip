@@ -133,9 +133,9 @@ void launch_rhs(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, c
 
 void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                 const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
-                int* flag, bool edge, hipStream_t s) {
-  if (gauge == 2) launch_rk4_g2(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s);
-  else launch_rk4_g0(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s);
+                int* flag, bool edge, hipStream_t s, bool interior) {
+  if (gauge == 2) launch_rk4_g2(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior);
+  else launch_rk4_g0(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior);
 }
 
 void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,

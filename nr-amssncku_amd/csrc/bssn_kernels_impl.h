@@ -110,8 +110,8 @@ void launch_rhs_gauge(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& 
 template <int G>
 void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                       const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                      bool edge, hipStream_t s) {
-  if (!box_empty(box)) {
+                      bool edge, hipStream_t s, bool interior) {
+  if (interior && !box_empty(box)) {
     const dim3 gr = interior_grid(box);
     switch (R.stage) {
       case 0:

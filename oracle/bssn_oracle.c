@@ -264,6 +264,10 @@ void orc_symmetry_bd(int ord, const int *ex, const double *f, double *fhout, con
 enum { CHI, TRK, DXX, GXY, GXZ, DYY, GYZ, DZZ, AXX, AXY, AXZ, AYY, AYZ, AZZ,
        GAMX, GAMY, GAMZ, LAP, BETAX, BETAY, BETAZ, DTSFX, DTSFY, DTSFZ };
 
+/* Restrict the point loop to one linear index (operation counting only;
+ * -1 = every point). */
+long orc_only_point = -1;
+
 int orc_compute_rhs_bssn(const int *ex, double T, const double *X, const double *Y, const double *Z,
                          const double *const *st, double *const *rhs, const double *const *mat,
                          double *const *gamo, double *const *rico, double *const *res, int symmetry,
@@ -290,6 +294,7 @@ int orc_compute_rhs_bssn(const int *ex, double T, const double *X, const double 
     for (int j = 0; j < ex[1]; j++)
     for (int i = 0; i < ex[0]; i++) {
         const size_t p = IX(ex, i, j, k);
+        if (orc_only_point >= 0 && (long)p != orc_only_point) continue;
         const double chi = st[CHI][p], trK = st[TRK][p];
         const double dxx = st[DXX][p], gxy = st[GXY][p], gxz = st[GXZ][p];
         const double dyy = st[DYY][p], gyz = st[GYZ][p], dzz = st[DZZ][p];

@@ -1,0 +1,137 @@
+"""Writes the golden fixtures of the BSSN RHS + RK4 path from the reference
+Fortran build (oracle/_ref, built from the reference sources in this
+container; its configured gauge is GAUGE 2).  The fixtures are data only:
+inputs are regenerated from the recorded seeds / analytic initial data and
+the stored arrays are the reference's outputs.
+
+  python tests/golden/make_golden.py small      # RHS and stencil vectors
+  python tests/golden/make_golden.py c0         # one RK4 step, 48^3 single puncture
+  python tests/golden/make_golden.py c1small    # 100 RK4 steps, 32^3 two punctures
+  python tests/golden/make_golden.py c1full     # 100 RK4 steps, 128^3 two punctures (minutes)
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "nr-amssncku_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle as O  # noqa: E402
+from helpers import grid, random_state  # noqa: E402
+
+NSAMPLE = 512
+
+
+def sample_index(shape, seed=123):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    return np.sort(rng.choice(n, size=min(NSAMPLE, n), replace=False))
+
+
+def summary(U, idx):
+    """Per field: max|f|, sum f^2, and the values at the sample indices."""
+    return dict(maxabs=np.array([np.max(np.abs(u)) for u in U]),
+                sumsq=np.array([np.sum(u * u) for u in U]),
+                samples=np.array([u.ravel()[idx] for u in U]))
+
+
+def make_small(ref):
+    nx, ny, nz = 8, 9, 10
+    X, Y, Z, _ = grid(nx, ny, nz)
+    out = dict(X=X, Y=Y, Z=Z)
+    for tag, seed, dtsf in (("a", 101, True), ("b", 102, False)):
+        st = random_state((nz, ny, nx), seed, dtsf=dtsf)
+        out[f"{tag}_state"] = np.array(st)
+        for co in (0, 1):
+            o = ref.compute_rhs(st, X, Y, Z, 0.02, co=co)
+            assert o["gont"] == 0
+            out[f"{tag}_co{co}_rhs"] = np.array(o["rhs"])
+            out[f"{tag}_co{co}_gam"] = np.array(o["gam"])
+            out[f"{tag}_co{co}_ric"] = np.array(o["ric"])
+            if co == 0:
+                out[f"{tag}_co{co}_res"] = np.array(o["res"])
+    # stencils with parities, and symmetric grids touching the z = 0 / octant planes
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal((nz, ny, nx))
+    r0 = rng.standard_normal((nz, ny, nx))
+    b = [rng.standard_normal((nz, ny, nx)) for _ in range(3)]
+    out["st_f"], out["st_r0"], out["st_b"] = f, r0, np.array(b)
+    for sym in (0, 1, 2):
+        if sym == 0:
+            Xs, Ys, Zs = X, Y, Z
+        else:
+            Xs = (np.arange(nx) + 0.5) * 0.3
+            Ys = (np.arange(ny) + 0.5) * 0.27
+            Zs = (np.arange(nz) + 0.5) * 0.33
+        out[f"sym{sym}_XYZ"] = np.concatenate([Xs, Ys, Zs])
+        for par in ((1, 1, 1), (-1, 1, -1)):
+            tag = f"sym{sym}_p{'p' if par[0] > 0 else 'm'}"
+            out[f"{tag}_d1"] = np.array(ref.fderivs(f, Xs, Ys, Zs, par, sym))
+            out[f"{tag}_d2"] = np.array(ref.fdderivs(f, Xs, Ys, Zs, par, sym))
+            out[f"{tag}_lop"] = ref.lopsided(f, r0, b[0], b[1], b[2], Xs, Ys, Zs, par, sym)
+            out[f"{tag}_ko"] = ref.kodis(f, r0, Xs, Ys, Zs, par, 0.3, sym)
+    np.savez_compressed(os.path.join(HERE, "ref_rhs_stencils_8x9x10.npz"), **out)
+
+
+def run_steps(ref, cfg, nsteps, nthreads=None, ham_steps=()):
+    import cpu_baseline as CB
+    p = cfg.patches[0]
+    U = cfg.initial_state(0)
+    X, Y, Z = p.coords()
+    hams = {}
+    if p.npts > 200000:
+        stp = CB.SlabStepper(ref, p, cfg.dt, cfg.eps, cfg.chitiny, gauge=None, nthreads=nthreads, gw=p.gw)
+        for n in range(nsteps):
+            stp.step(U)
+        stp.close()
+    else:
+        for n in range(nsteps):
+            want = (n + 1) in ham_steps
+            g, ham = ref.step(U, X, Y, Z, p.bbox, cfg.dt, cfg.eps, cfg.chitiny, bc=0, gw=p.gw, want_ham=want)
+            assert g == 0
+            if want:
+                gw = p.gw
+                hams[n + 1] = float(np.sqrt(np.sum(ham[gw:-gw, gw:-gw, gw:-gw] ** 2) * np.prod(p.h)))
+    return U, hams
+
+
+def main(what):
+    import amssbssn as A
+    ref = O.Ref()
+    t0 = time.time()
+    if what == "small":
+        make_small(ref)
+    elif what == "c0":
+        cfg = A.config0()
+        U, hams = run_steps(ref, cfg, 1, ham_steps=(1,))
+        idx = sample_index(U[0].shape)
+        np.savez_compressed(os.path.join(HERE, "ref_c0_step1_gauge2.npz"), idx=idx, ham_l2=hams[1],
+                            **summary(U, idx))
+    elif what == "c1small":
+        cfg = A.config1(n=32, steps=100)
+        U, hams = run_steps(ref, cfg, 100, ham_steps=(1, 50, 100))
+        idx = sample_index(U[0].shape)
+        np.savez_compressed(os.path.join(HERE, "ref_c1_32cubed_step100_gauge2.npz"), idx=idx,
+                            ham_steps=np.array(sorted(hams)), ham_l2=np.array([hams[k] for k in sorted(hams)]),
+                            **summary(U, idx))
+    elif what == "c1full":
+        cfg = A.config1(n=128, steps=100)
+        U, _ = run_steps(ref, cfg, 100)
+        idx = sample_index(U[0].shape)
+        np.savez_compressed(os.path.join(HERE, "ref_c1_128cubed_step100_gauge2.npz"), idx=idx, **summary(U, idx))
+    print(json.dumps(dict(fixture=what, seconds=round(time.time() - t0, 1))))
+
+
+if __name__ == "__main__":
+    # the reference keeps its temporaries as automatic (stack) arrays
+    import threading
+    threading.stack_size(1024 * 1024 * 1024)
+    th = threading.Thread(target=main, args=(sys.argv[1] if len(sys.argv) > 1 else "small",))
+    th.start()
+    th.join()

@@ -193,3 +193,65 @@ def test_two_patches_equal_one_patch():
         b = A1[v][gw:-gw, gw:-gw, gw:-gw]
         assert np.array_equal(w[:, :, : n // 2], a), v
         assert np.array_equal(w[:, :, n // 2:], b), v
+
+
+# ---------------------------------------------------------------------------
+# Against the reference Fortran's golden vectors (tests/golden, GAUGE 2: the
+# reference build's configured gauge).  Per field: relative difference at the
+# sampled points (scaled by the field's max |f|), max |f| and sum f^2.
+def _gold(name):
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+    if not os.path.exists(p):
+        pytest.skip(f"fixture {name} not generated")
+    return np.load(p)
+
+
+def _vs_gold(G, g, tol):
+    idx = g["idx"]
+    worst = 0.0
+    for v, u in enumerate(G):
+        m = float(g["maxabs"][v])
+        d = float(np.max(np.abs(u.ravel()[idx] - g["samples"][v])))
+        e = d / m if m > 0 else d
+        worst = max(worst, e)
+        assert e <= tol, (v, e)
+        if m > 0:
+            assert abs(float(np.max(np.abs(u))) / m - 1) <= tol, v
+            assert abs(float(np.sum(u * u)) / float(g["sumsq"][v]) - 1) <= 2 * tol, v
+    return worst
+
+
+def test_c0_one_step_vs_reference_vectors():
+    import amssbssn as A
+    g = _gold("ref_c0_step1_gauge2.npz")
+    cfg = A.config0()
+    lv = A.Level.from_config(cfg, gauge=2)
+    assert lv.step(cfg.dt, 1, want_ham=True) == 0
+    worst = _vs_gold(lv.get_state(0), g, 1e-12)
+    assert abs(lv.ham_l2() / float(g["ham_l2"]) - 1) < 1e-12
+    lv.close()
+    print(f"C0 one step vs reference: worst {worst:.2e}")
+
+
+def test_c1_32cubed_100_steps_vs_reference_vectors():
+    import amssbssn as A
+    g = _gold("ref_c1_32cubed_step100_gauge2.npz")
+    cfg = A.config1(n=32)
+    lv = A.Level.from_config(cfg, gauge=2)
+    assert lv.step(cfg.dt, 100, want_ham=True) == 0
+    worst = _vs_gold(lv.get_state(0), g, 1e-9)
+    assert abs(lv.ham_l2() / float(g["ham_l2"][-1]) - 1) < 1e-9
+    lv.close()
+    print(f"C1 32^3, 100 steps vs reference: worst {worst:.2e}")
+
+
+def test_c1_128cubed_100_steps_vs_reference_vectors():
+    import amssbssn as A
+    g = _gold("ref_c1_128cubed_step100_gauge2.npz")
+    cfg = A.config1(n=128)
+    lv = A.Level.from_config(cfg, gauge=2)
+    assert lv.step(cfg.dt, 100) == 0
+    worst = _vs_gold(lv.get_state(0), g, 1e-9)
+    lv.close()
+    print(f"C1 128^3, 100 steps vs reference: worst {worst:.2e}")

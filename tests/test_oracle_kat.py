@@ -1,0 +1,171 @@
+"""Closed-form checks of the oracle, chosen so that a dropped term, a wrong
+sign or index, or a transposed operand fails: polynomial exactness of every
+stencil, the KO operator on x^6, the RK4 amplification factor, the algebraic
+constraints after enforce_ga, the RHS of flat space and of a pure B^i
+field, reflection symmetry of the puncture RHS, and Philox known answers."""
+import numpy as np
+import pytest
+
+from helpers import grid, random_state
+
+
+def mesh(X, Y, Z):
+    z, y, x = np.meshgrid(Z, Y, X, indexing="ij")
+    return x, y, z
+
+
+def test_fderivs_exact_on_quartics(orc):
+    X, Y, Z, _ = grid(11, 12, 13)
+    x, y, z = mesh(X, Y, Z)
+    f = x ** 3 - 2 * x ** 2 * y + y ** 4 + z ** 4 + x * y * z
+    fx, fy, fz = orc.fderivs(f, X, Y, Z, (1, 1, 1))
+    ex = (3 * x ** 2 - 4 * x * y + y * z, -2 * x ** 2 + 4 * y ** 3 + x * z, 4 * z ** 3 + x * y)
+    s = np.s_[2:-3, 2:-3, 2:-3]  # 4th-order points (+-2 room, last index excluded)
+    for a, b in zip((fx, fy, fz), ex):
+        assert np.max(np.abs(a[s] - b[s])) < 1e-11 * np.max(np.abs(b[s]))
+    # 2nd-order points (exactly +-1 room on some axis) are exact on quadratics
+    q = 2 * x ** 2 - x * y + 3 * z ** 2
+    qx, qy, qz = orc.fderivs(q, X, Y, Z, (1, 1, 1))
+    edge = np.s_[1, 1:-1, 1:-1]
+    assert np.allclose(qx[edge], (4 * x - y)[edge], rtol=0, atol=1e-12)
+    assert np.allclose(qz[edge], (6 * z)[edge], rtol=0, atol=1e-12)
+    # zero where an axis lacks +-1 room, and on the last index
+    assert np.all(fx[0] == 0) and np.all(fx[:, :, -1] == 0)
+
+
+def test_fdderivs_exact(orc):
+    X, Y, Z, _ = grid(11, 12, 13)
+    x, y, z = mesh(X, Y, Z)
+    f = x ** 5 + y ** 5 + z ** 4 + x ** 3 * y ** 2 + x ** 2 * z ** 3 + y ** 4 * z ** 4
+    d = orc.fdderivs(f, X, Y, Z, (1, 1, 1))
+    ex = (20 * x ** 3 + 6 * x * y ** 2 + 2 * z ** 3,          # xx
+          6 * x ** 2 * y,                                     # xy
+          6 * x * z ** 2,                                     # xz
+          20 * y ** 3 + 2 * x ** 3 + 12 * y ** 2 * z ** 4,    # yy
+          16 * y ** 3 * z ** 3,                               # yz
+          12 * z ** 2 + 6 * x ** 2 * z + 12 * y ** 4 * z ** 2)  # zz
+    s = np.s_[2:-3, 2:-3, 2:-3]
+    for a, b in zip(d, ex):
+        assert np.max(np.abs(a[s] - b[s])) < 1e-10 * np.max(np.abs(b[s]))
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_lopsided_exact_on_quartics(orc, sign):
+    X, Y, Z, _ = grid(12, 12, 12)
+    x, y, z = mesh(X, Y, Z)
+    f = x ** 4 + 2 * y ** 3 * x - z ** 4 + x * y * z
+    bx, by, bz = (sign * c * np.ones_like(x) for c in (0.7, -1.3, 0.4))
+    r = orc.lopsided(f, np.zeros_like(f), bx, by, bz, X, Y, Z, (1, 1, 1))
+    grad = (4 * x ** 3 + 2 * y ** 3 + y * z, 6 * y ** 2 * x + x * z, -4 * z ** 3 + x * y)
+    ex = bx * grad[0] + by * grad[1] + bz * grad[2]
+    s = np.s_[3:-3, 3:-3, 3:-3]  # full upwind room along every axis
+    assert np.max(np.abs(r[s] - ex[s])) < 1e-10 * np.max(np.abs(ex[s]))
+
+
+def test_kodis_annihilates_quintics_and_gives_720_h5_on_x6(orc):
+    h = 0.3
+    X, Y, Z, _ = grid(12, 12, 12, h=(h, h, h))
+    x, y, z = mesh(X, Y, Z)
+    eps = 0.3
+    p5 = x ** 5 - 3 * y ** 4 * z + z ** 5 + x * y * z
+    r = orc.kodis(p5, np.zeros_like(p5), X, Y, Z, (1, 1, 1), eps)
+    s = np.s_[3:-3, 3:-3, 3:-3]
+    assert np.max(np.abs(r[s])) < 1e-9
+    r6 = orc.kodis(x ** 6, np.zeros_like(x), X, Y, Z, (1, 1, 1), eps)
+    assert np.allclose(r6[s], eps / 64 * 720 * h ** 5, rtol=1e-9, atol=0)
+    # only where every axis has +-3 room
+    assert np.all(r6[:3] == 0) and np.all(r6[:, :, -3:] == 0)
+
+
+def test_rk4_amplification_factor(orc):
+    """The reference's four-call RK4 sequence (bssn_class.C:3119-3123, 3487-3491)
+    on f' = lambda f reproduces 1 + z + z^2/2 + z^3/6 + z^4/24."""
+    lam, dt = -0.7, 0.3
+    sh = (2, 2, 2)
+    U0 = np.ones(sh)
+    pre, cor, acc = np.zeros(sh), np.zeros(sh), np.zeros(sh)
+    acc[:] = lam * U0
+    orc.rk4(U0, pre, acc, dt, 0)
+    for it in (1, 2, 3):
+        cor[:] = lam * pre
+        orc.rk4(U0, cor, acc, dt, it)
+        if it < 3:
+            pre, cor = cor, pre
+    zz = lam * dt
+    assert np.allclose(cor, 1 + zz + zz ** 2 / 2 + zz ** 3 / 6 + zz ** 4 / 24, rtol=1e-15, atol=0)
+
+
+def test_enforce_ga_restores_algebraic_constraints(orc):
+    st = random_state((6, 7, 8), seed=3, amp=0.1)
+    orc.enforce_ga(st)
+    g = [st[2] + 1, st[3], st[4], st[5] + 1, st[6], st[7] + 1]
+    det = g[0] * g[3] * g[5] + 2 * g[1] * g[4] * g[2] - g[2] ** 2 * g[3] - g[1] ** 2 * g[5] - g[0] * g[4] ** 2
+    assert np.max(np.abs(det - 1)) < 1e-14
+    cof = [g[3] * g[5] - g[4] ** 2, g[2] * g[4] - g[1] * g[5], g[1] * g[4] - g[2] * g[3],
+           g[0] * g[5] - g[2] ** 2, g[1] * g[2] - g[0] * g[4], g[0] * g[3] - g[1] ** 2]
+    A = st[8:14]
+    tr = cof[0] * A[0] + cof[3] * A[3] + cof[5] * A[5] + 2 * (cof[1] * A[1] + cof[2] * A[2] + cof[4] * A[4])
+    assert np.max(np.abs(tr)) < 1e-15
+
+
+@pytest.mark.parametrize("gauge", [0, 2])
+def test_flat_space_rhs_is_zero(orc, gauge):
+    X, Y, Z, _ = grid(10, 10, 10)
+    st = [np.zeros((10, 10, 10)) for _ in range(24)]
+    if gauge == 2:  # GAUGE 2's damping is 0/0 at chi = 1; perturb chi slightly
+        st[0][:] = -1e-3
+    o = orc.compute_rhs(st, X, Y, Z, 0.02, co=0, gauge=gauge)
+    tol = 0.0 if gauge == 0 else 1e-18  # constant chi - 1 != 0: stencil sums cancel to rounding
+    for v in range(24):
+        assert np.max(np.abs(o["rhs"][v])) <= tol, v
+    assert np.max(np.abs(o["res"][0])) <= tol
+
+
+def test_gauge0_B_driver_closed_form(orc):
+    """Uniform B^i on flat space: beta^i_rhs = 3/4 B^i, B^i_rhs = -eta B^i with eta = 2."""
+    X, Y, Z, _ = grid(10, 10, 10)
+    st = [np.zeros((10, 10, 10)) for _ in range(24)]
+    c = (0.3, -0.2, 0.05)
+    for a in range(3):
+        st[21 + a][:] = c[a]
+    o = orc.compute_rhs(st, X, Y, Z, 0.02, co=1, gauge=0)
+    s = np.s_[3:-3, 3:-3, 3:-3]
+    for a in range(3):
+        assert np.all(o["rhs"][18 + a][s] == 0.75 * c[a])
+        assert np.all(o["rhs"][21 + a][s] == -2.0 * c[a])
+
+
+def test_puncture_rhs_reflection_symmetry(orc):
+    """Single puncture at the centre of a symmetric grid: each RHS field has
+    the reflection parity of its variable (bssn_class.C:183-361)."""
+    import oracle
+    import amssbssn as A
+    g = A.cube_patch(14, 4.0)
+    cfg = A.Config("sym", [g], dt=0.1, steps=1, punctures=[(1.0, 0.0, 0.0, 0.0)])
+    st = cfg.initial_state(0)
+    rng = np.random.default_rng(0)
+    # add a reflection-symmetric shift so advection and the Lie terms are exercised
+    X, Y, Z = g.coords()
+    x, y, z = mesh(X, Y, Z)
+    bump = np.exp(-(x ** 2 + y ** 2 + z ** 2) / 4)
+    st[18], st[19], st[20] = 0.1 * x * bump, 0.1 * y * bump, 0.1 * z * bump
+    o = orc.compute_rhs(st, X, Y, Z, 0.02, co=1, gauge=0)
+    s = np.s_[3:-3, 3:-3, 3:-3]  # [3, n-4]: full stencils, symmetric under i -> n-1-i
+    for v in range(24):
+        r = o["rhs"][v][s]
+        m = max(float(np.max(np.abs(r))), 1e-300)
+        for axis, flip in ((2, np.s_[:, :, ::-1]), (1, np.s_[:, ::-1, :]), (0, np.s_[::-1, :, :])):
+            par = oracle.PARITY[v][2 - axis]
+            assert np.max(np.abs(r[flip] - par * r)) <= 1e-12 * m, (v, axis)
+
+
+def test_philox_known_answers(orc):
+    import amssbssn.grid as G
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in kat:
+        assert tuple(orc.philox(ctr, key)) == want
+        got = G.philox4x32_10(*[np.array([c], np.uint64) for c in ctr], *key)
+        assert tuple(int(w[0]) for w in got) == want
