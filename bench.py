@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128, help="interior points per axis")
     ap.add_argument("--mode", choices=["plain", "batched", "graph"], default="plain")
+    ap.add_argument("--variant", type=int, choices=[0, 1], default=None,
+                    help="interior kernel: 0 direct stencil reads, 1 LDS-staged tiles (default: the level's)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -71,6 +73,9 @@ def main():
     cfg = A.config1(n=args.n)
     mode = {"plain": A.MODE_PLAIN, "batched": A.MODE_BATCHED, "graph": A.MODE_GRAPH}[args.mode]
     lv = A.Level.from_config(cfg, gauge=0)
+    if args.variant is not None:
+        lv.set_variant(args.variant)
+    variant = lv.variant()
     pts = cfg.interior_pts
 
     if args.warmup:
@@ -96,7 +101,8 @@ def main():
     traffic = traffic_from_profiles()
     roof = dict(bound="hbm" if ai < ridge else "fp64", achieved=round(ach_gbs, 1), peak=perf.PEAK_HBM_GBS,
                 unit="GB/s", frac=round(ach_gbs / perf.PEAK_HBM_GBS, 4), traffic=traffic,
-                kernel="rk4_interior_kernel (fused RHS + RK4 sub-step, stages 0-3)",
+                kernel=("rk4_lds_kernel" if variant == 1 else "rk4_interior_kernel") +
+                " (fused RHS + RK4 sub-step, stages 0-3)",
                 avg_launch_ms=round(sum(ms) / launches, 4), launches=launches,
                 algorithmic_bytes_per_launch=int(nbytes / launches),
                 algorithmic_flops_per_launch=int(flops / launches),
@@ -112,7 +118,7 @@ def main():
                data="synthetic: analytic two-puncture (Brill-Lindquist) initial data, Philox A_ij noise",
                config=dict(workload=f"two-puncture BSSN RK4 step, one {args.n}^3 patch + 3 ghosts over [-16,16]^3",
                            interior_points=pts, rhs_evals_per_step=4, gauge=0, ko_eps=cfg.eps, dt=cfg.dt,
-                           mode=args.mode, stage_kernel_ms=[round(m / max(n, 1), 4) for m, n in zip(ms, nl)]),
+                           mode=args.mode, variant=variant, stage_kernel_ms=[round(m / max(n, 1), 4) for m, n in zip(ms, nl)]),
                roofline=roof)
     lv.close()
     if not args.no_cpu_baseline:

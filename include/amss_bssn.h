@@ -156,6 +156,14 @@ int amss_level_step(amss_level *L, double dT, int nsteps, int mode, int want_ham
 int amss_level_ham_l2(amss_level *L, double *out);
 /* The stream the level's work is ordered on (a hipStream_t). */
 void *amss_level_stream(amss_level *L);
+/* Interior kernel of the fused RHS + RK4 sub-step: 0 reads every stencil
+ * point through the vector L1/L2 (raw buffer loads); 1 stages a halo tile of
+ * each field group in LDS (8x8x4-point tiles, double-buffered, filled by
+ * direct global->LDS loads; the default).  Both give the same results to
+ * rounding. */
+int amss_level_set_variant(amss_level *L, int variant);
+/* The interior kernel variant in use (-1 for a null level). */
+int amss_level_variant(amss_level *L);
 /* HIP-event timing of the fused RHS + RK4 interior kernel launches: enable
  * (resets the totals) or disable; then read the summed milliseconds and the
  * launch counts per sub-step 0..3 since the last enable. */

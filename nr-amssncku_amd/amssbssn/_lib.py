@@ -38,6 +38,8 @@ SIGNATURES = {
     "amss_level_ham_l2": ([C.c_void_p, _dp], C.c_int),
     "amss_level_stream": ([C.c_void_p], C.c_void_p),
     "amss_level_timing": ([C.c_void_p, C.c_int], C.c_int),
+    "amss_level_set_variant": ([C.c_void_p, C.c_int], C.c_int),
+    "amss_level_variant": ([C.c_void_p], C.c_int),
     "amss_level_kernel_ms": ([C.c_void_p, _dp, C.POINTER(C.c_long)], C.c_int),
 }
 

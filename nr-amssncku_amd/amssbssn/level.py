@@ -87,6 +87,13 @@ class Level:
     def stream(self) -> int:
         return lib().amss_level_stream(self.h)
 
+    def set_variant(self, variant: int) -> None:
+        """Interior kernel: 0 direct L1/L2 stencil reads, 1 LDS-staged tiles."""
+        check(lib().amss_level_set_variant(self.h, variant), "amss_level_set_variant")
+
+    def variant(self) -> int:
+        return lib().amss_level_variant(self.h)
+
     def set_timing(self, on: bool) -> None:
         check(lib().amss_level_timing(self.h, 1 if on else 0), "amss_level_timing")
 

@@ -451,6 +451,15 @@ int amss_level_ham_l2(amss_level* L, double* out) {
 
 void* amss_level_stream(amss_level* L) { return L ? (void*)L->L->stream() : nullptr; }
 
+int amss_level_set_variant(amss_level* L, int variant) {
+  if (!L) return fail(AMSS_ERR_ARG, "amss_level_set_variant: null level");
+  const int r = L->L->set_variant(variant);
+  if (r) return fail(r, L->L->err());
+  return 0;
+}
+
+int amss_level_variant(amss_level* L) { return L ? L->L->variant() : -1; }
+
 int amss_level_timing(amss_level* L, int enable) {
   if (!L) return fail(AMSS_ERR_ARG, "amss_level_timing: null level");
   return L->L->set_timing(enable != 0);

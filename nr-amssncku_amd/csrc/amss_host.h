@@ -90,6 +90,9 @@ class BssnLevel {
   hipStream_t stream() const { return stream_; }
   // HIP-event timing of the fused RHS+RK4 interior launches (per stage)
   int set_timing(bool on);
+  // interior kernel variant: 0 direct L1/L2 reads, 1 LDS-staged tiles
+  int set_variant(int v);
+  int variant() const { return variant_; }
   int kernel_ms(double* ms4, long* n4);
   int npatch() const { return (int)patches_.size(); }
   long npts(int patch) const;
@@ -123,6 +126,7 @@ class BssnLevel {
   int nparts_ = 4096;
   bool ham_valid_ = false;
   bool timing_ = false;
+  int variant_ = 1;
   struct Ev { hipEvent_t a, b; int stage; };
   std::vector<Ev> ev_;       // event pairs of the current step() call
   size_t ev_used_ = 0;

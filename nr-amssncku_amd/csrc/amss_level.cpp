@@ -267,7 +267,7 @@ int BssnLevel::Step(double dT, bool want_res) {
       for (int a = 0; a < 6; ++a) aux.ric[a] = nullptr;
       MyList<var>* c = ConsList;
       for (int a = 0; a < 7; ++a, c = c->next) aux.res[a] = b.fgfs[c->data->sgfn];
-      Rk4Param R{it, cst[it], tinny};
+      Rk4Param R{it, variant_, cst[it], tinny};
       const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
       Ev* ev = nullptr;
       if (timing_) {
@@ -345,6 +345,15 @@ int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
     err_ = "NaN found in a right-hand-side input";
     return AMSS_ERR_NONFINITE;
   }
+  return 0;
+}
+
+int BssnLevel::set_variant(int v) {
+  if (v != 0 && v != 1) {
+    err_ = "kernel variant must be 0 (direct) or 1 (LDS tiles)";
+    return AMSS_ERR_ARG;
+  }
+  variant_ = v;
   return 0;
 }
 

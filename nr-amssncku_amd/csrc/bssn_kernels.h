@@ -30,6 +30,7 @@ struct SegDesc {
 
 struct Rk4Param {
   int stage;     // 0 predictor, 1..3 correctors
+  int variant;   // interior kernel: 0 direct (L1/L2 reads), 1 LDS-staged tiles
   double c;      // 0.5 dt, 0.5 dt, dt, dt/6
   double tinny;  // chitiny - 1 (bssn_class.C:179)
 };

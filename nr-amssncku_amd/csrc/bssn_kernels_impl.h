@@ -3,6 +3,7 @@
 // heavy point kernels compile in parallel translation units.
 #pragma once
 #include "bssn_kernels.h"
+#include "bssn_lds.h"
 
 namespace amss {
 
@@ -74,6 +75,32 @@ AMSS_POINT_KERNEL rk4_interior_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, F
   S.finish();
 }
 
+// LDS-staged variant: 8x8x4-point tiles, halo tiles of every field group in
+// LDS (double-buffered, filled by direct global->LDS loads).
+constexpr int kLdsTX = 8, kLdsTY = 8, kLdsTZ = 4;
+template <int GAUGE, int STAGE, bool CO>
+AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
+                                 Coef C, Rk4Param R, int* flag) {
+  using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
+  __shared__ __attribute__((aligned(16))) double tiles[2 * Tl::NPAD];
+  const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
+  const int i0 = box.lo[0] + blockIdx.x * kLdsTX, j0 = box.lo[1] + blockIdx.y * kLdsTY;
+  const int k0 = box.lo[2] + blockIdx.z * kLdsTZ;
+  const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+  // the launch covers whole tiles only (remainder slabs go to the direct kernel)
+  const long p = i + j * box.sy + k * box.sz;
+  if (bad_input(in, p)) *flag = 1;
+  const int N = box.n[0] * box.n[1] * box.n[2];
+  lds_double* t0 = (lds_double*)tiles;
+  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + Tl::NPAD, p,
+                                  (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
+                                  (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N};
+  E.prologue();
+  Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  S.finish();
+}
+
 template <int GAUGE, bool CO>
 __global__ void __launch_bounds__(256) rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
                                                        AuxOut aux, Box box, Coef C, Rk4Param R, int* flag) {
@@ -111,7 +138,34 @@ template <int G>
 void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                       const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
                       bool edge, hipStream_t s, bool interior) {
-  if (interior && !box_empty(box)) {
+  if (interior && !box_empty(box) && R.variant == 1) {
+    // LDS kernel on the largest tile-aligned sub-box, the direct kernel on the rest
+    Box core = box;
+    const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
+    for (int a = 0; a < 3; ++a) core.hi[a] = box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a];
+    if (!box_empty(core)) {
+      const dim3 gr((core.hi[0] - core.lo[0]) / kLdsTX, (core.hi[1] - core.lo[1]) / kLdsTY,
+                    (core.hi[2] - core.lo[2]) / kLdsTZ);
+      const int nt = kLdsTX * kLdsTY * kLdsTZ;
+      switch (R.stage) {
+        case 0:
+          if (want_res) rk4_lds_kernel<G, 0, true><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag);
+          else rk4_lds_kernel<G, 0, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag);
+          break;
+        case 1: rk4_lds_kernel<G, 1, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break;
+        case 2: rk4_lds_kernel<G, 2, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break;
+        default: rk4_lds_kernel<G, 3, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break;
+      }
+    }
+    Rk4Param Rd = R;
+    Rd.variant = 0;
+    for (int a = 0; a < 3; ++a) {  // remainder slab [core.hi[a], box.hi[a]) along axis a
+      Box rem = box;
+      for (int c = 0; c < a; ++c) rem.hi[c] = core.hi[c];
+      rem.lo[a] = core.hi[a];
+      if (!box_empty(rem)) launch_rk4_gauge<G>(in, u0, acc, out, aux, rem, C, Rd, want_res, flag, false, s, true);
+    }
+  } else if (interior && !box_empty(box)) {
     const dim3 gr = interior_grid(box);
     switch (R.stage) {
       case 0:

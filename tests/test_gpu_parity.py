@@ -118,9 +118,10 @@ def test_pointwise_abi_entry_points_match_oracle(fa, orc):
     assert np.array_equal(g, o)
 
 
-def _level_vs_oracle(orc, cfg, gauge, bc, nsteps, tol):
+def _level_vs_oracle(orc, cfg, gauge, bc, nsteps, tol, variant=0):
     import amssbssn as A
     lv = A.Level.from_config(cfg, gauge=gauge, bc=bc)
+    lv.set_variant(variant)
     p = cfg.patches[0]
     X, Y, Z = p.coords()
     U = cfg.initial_state(0)
@@ -134,12 +135,16 @@ def _level_vs_oracle(orc, cfg, gauge, bc, nsteps, tol):
     return max(errs)
 
 
+# variant 0: one thread per point, stencils straight from HBM; variant 1: LDS
+# tiles of 8x8x4 points on the tile-aligned part of the interior, variant 0 on
+# the remainder slabs (n = 18 leaves a 2-point remainder on every axis).
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("bc", [0, 1, 2])
-def test_one_rk4_step_matches_oracle(orc, bc):
+def test_one_rk4_step_matches_oracle(orc, bc, variant):
     import amssbssn as A
     cfg = A.config1(n=18)
-    worst = _level_vs_oracle(orc, cfg, 0, bc, 1, 1e-12)
-    print(f"bc {bc}: worst {worst:.2e}")
+    worst = _level_vs_oracle(orc, cfg, 0, bc, 1, 1e-12, variant)
+    print(f"bc {bc} variant {variant}: worst {worst:.2e}")
 
 
 def test_one_rk4_step_single_puncture_small(orc):
@@ -149,17 +154,36 @@ def test_one_rk4_step_single_puncture_small(orc):
     _level_vs_oracle(orc, cfg, 0, 0, 1, 1e-12)
 
 
-def test_twenty_steps_match_oracle(orc):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_twenty_steps_match_oracle(orc, variant):
     import amssbssn as A
     cfg = A.config1(n=16)
-    worst = _level_vs_oracle(orc, cfg, 0, 0, 20, 1e-10)
-    print(f"20 steps: worst {worst:.2e}")
+    worst = _level_vs_oracle(orc, cfg, 0, 0, 20, 1e-10, variant)
+    print(f"20 steps variant {variant}: worst {worst:.2e}")
 
 
-def test_gauge2_steps_match_oracle(orc):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gauge2_steps_match_oracle(orc, variant):
     import amssbssn as A
     cfg = A.config1(n=14)
-    _level_vs_oracle(orc, cfg, 2, 0, 5, 1e-10)
+    _level_vs_oracle(orc, cfg, 2, 0, 5, 1e-10, variant)
+
+
+def test_lds_variant_matches_direct_kernel():
+    """Both interior kernels evaluate the same expression tree per point (the
+    compiler may contract different products into FMAs), so a 24^3 patch
+    (three whole tiles on x and y, six on z) agrees to rounding."""
+    import amssbssn as A
+    cfg = A.config1(n=24)
+    out = []
+    for variant in (0, 1):
+        lv = A.Level.from_config(cfg, gauge=0)
+        lv.set_variant(variant)
+        assert lv.step(cfg.dt, 2) == 0
+        out.append(lv.get_state(0))
+        lv.close()
+    for v in range(24):
+        assert rel_err(out[1][v], out[0][v]) <= 1e-13, v
 
 
 def test_two_patches_equal_one_patch():
@@ -234,11 +258,13 @@ def test_c0_one_step_vs_reference_vectors():
     print(f"C0 one step vs reference: worst {worst:.2e}")
 
 
-def test_c1_32cubed_100_steps_vs_reference_vectors():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_c1_32cubed_100_steps_vs_reference_vectors(variant):
     import amssbssn as A
     g = _gold("ref_c1_32cubed_step100_gauge2.npz")
     cfg = A.config1(n=32)
     lv = A.Level.from_config(cfg, gauge=2)
+    lv.set_variant(variant)
     assert lv.step(cfg.dt, 100, want_ham=True) == 0
     worst = _vs_gold(lv.get_state(0), g, 1e-9)
     assert abs(lv.ham_l2() / float(g["ham_l2"][-1]) - 1) < 1e-9
@@ -246,11 +272,13 @@ def test_c1_32cubed_100_steps_vs_reference_vectors():
     print(f"C1 32^3, 100 steps vs reference: worst {worst:.2e}")
 
 
-def test_c1_128cubed_100_steps_vs_reference_vectors():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_c1_128cubed_100_steps_vs_reference_vectors(variant):
     import amssbssn as A
     g = _gold("ref_c1_128cubed_step100_gauge2.npz")
     cfg = A.config1(n=128)
     lv = A.Level.from_config(cfg, gauge=2)
+    lv.set_variant(variant)
     assert lv.step(cfg.dt, 100) == 0
     worst = _vs_gold(lv.get_state(0), g, 1e-9)
     lv.close()
