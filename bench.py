@@ -49,6 +49,47 @@ def cpu_baseline(cfg, nsteps):
     return dict(value=round(mpts, 3), unit="Mpoints/s", cores=cores, kind=kind, sample=sample)
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: FETCH_SIZE and WRITE_SIZE
+    of its dispatches (mib MiB each) give the counters' scale on this GPU."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel (copy_ would be a DMA memcpy)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    print(json.dumps({"copy_probe_mib": mib, "dispatches": 5}), flush=True)
+    return 0
+
+
+def batching(steps, warmup):
+    """The 37-patch uneven level (extents multiples of 8 in [24,96], seed 1,
+    packed in [-24,24]^3): the same RK4 steps with one launch per patch and
+    kind of work (plain), with the batched tile-list launches, and replayed
+    from hipGraphs."""
+    import amssbssn as A
+    cfg = A.config2()
+    lv = A.Level.from_config(cfg, gauge=0)
+    pts = cfg.interior_pts
+    choice, table = lv.autotune()
+    out = dict(workload=f"{len(cfg.patches)} uneven patches (multiples of 8 in [24,96], seed 1) in [-24,24]^3",
+               interior_points=pts, steps=steps, tune=choice)
+    for name, mode in (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED), ("graph", A.MODE_GRAPH)):
+        assert lv.step(cfg.dt, warmup, mode) == 0
+        t0 = time.perf_counter()
+        assert lv.step(cfg.dt, steps, mode) == 0
+        dt = time.perf_counter() - t0
+        out[name + "_mpoints_s"] = round(pts * 4 * steps / dt / 1e6, 2)
+        out[name + "_ms_per_step"] = round(dt * 1e3 / steps, 3)
+    out["graph_vs_plain"] = round(out["graph_mpoints_s"] / out["plain_mpoints_s"], 3)
+    out["batched_vs_plain"] = round(out["batched_mpoints_s"] / out["plain_mpoints_s"], 3)
+    out["tune_table_ms"] = [[t[0], t[1], t[2], round(t[3], 4)] for t in table]
+    lv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,9 +99,15 @@ def main():
     ap.add_argument("--mode", choices=["plain", "batched", "graph"], default="plain")
     ap.add_argument("--variant", type=int, choices=[0, 1], default=None,
                     help="interior kernel: 0 direct stencil reads, 1 LDS-staged tiles (default: the level's)")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="timed steps of each mode on the 37-patch level (0 skips the batching comparison)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (calibrates rocprof's FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design only)", file=sys.stderr)
@@ -121,6 +168,8 @@ def main():
                            mode=args.mode, variant=variant, stage_kernel_ms=[round(m / max(n, 1), 4) for m, n in zip(ms, nl)]),
                roofline=roof)
     lv.close()
+    if args.batch_steps > 0:
+        out["batching"] = batching(args.batch_steps, 2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
         out["cpu_baseline"]["gpu_speedup"] = round(value / out["cpu_baseline"]["value"], 1)

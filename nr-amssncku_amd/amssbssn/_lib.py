@@ -40,6 +40,9 @@ SIGNATURES = {
     "amss_level_timing": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_set_variant": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_variant": ([C.c_void_p], C.c_int),
+    "amss_level_autotune": ([C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "amss_level_set_tune": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
+    "amss_level_get_tune": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "amss_level_kernel_ms": ([C.c_void_p, _dp, C.POINTER(C.c_long)], C.c_int),
 }
 

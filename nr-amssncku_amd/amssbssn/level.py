@@ -94,6 +94,31 @@ class Level:
     def variant(self) -> int:
         return lib().amss_level_variant(self.h)
 
+    TILE_SHAPES = ((8, 8, 4), (4, 8, 8), (8, 4, 8))
+
+    def autotune(self):
+        """Time every batched-kernel candidate (tile shape x persistent grid x
+        tile order) on the current state, keep the fastest.  Returns
+        (choice dict, list of (shape, grid, remap, ms))."""
+        n = 3 * 3 * 2
+        ms = (C.c_double * n)()
+        ch = (C.c_int * 3)()
+        check(lib().amss_level_autotune(self.h, ms, n, ch), "amss_level_autotune")
+        cands = [(s, g, r) for s in range(3) for g in range(3) for r in range(2)]
+        return self._tune_dict(ch), [(s, g, r, float(t)) for (s, g, r), t in zip(cands, ms)]
+
+    def set_tune(self, shape: int, grid: int, remap: int) -> None:
+        check(lib().amss_level_set_tune(self.h, shape, grid, remap), "amss_level_set_tune")
+
+    def get_tune(self):
+        ch = (C.c_int * 3)()
+        check(lib().amss_level_get_tune(self.h, ch), "amss_level_get_tune")
+        return self._tune_dict(ch)
+
+    def _tune_dict(self, ch):
+        return dict(tile="x".join(map(str, self.TILE_SHAPES[ch[0]])), shape=int(ch[0]), grid=int(ch[1]),
+                    xcd_order=bool(ch[2]))
+
     def set_timing(self, on: bool) -> None:
         check(lib().amss_level_timing(self.h, 1 if on else 0), "amss_level_timing")
 

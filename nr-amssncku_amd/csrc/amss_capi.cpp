@@ -441,7 +441,36 @@ int amss_level_copy(amss_level* L, int patch, int list, int field, double* ptr, 
 
 int amss_level_step(amss_level* L, double dT, int nsteps, int mode, int want_ham) {
   if (!L) return fail(AMSS_ERR_ARG, "amss_level_step: null level");
-  return L->L->step(dT, nsteps, mode, want_ham != 0);
+  const int r = L->L->step(dT, nsteps, mode, want_ham != 0);
+  if (r) return fail(r, L->L->err());
+  return 0;
+}
+
+int amss_level_autotune(amss_level* L, double* ms, int nms, int* choice) {
+  if (!L) return fail(AMSS_ERR_ARG, "amss_level_autotune: null level");
+  const int r = L->L->autotune(ms, ms ? nms : 0);
+  if (r) return fail(r, L->L->err());
+  if (choice) {
+    choice[0] = L->L->tune().shape;
+    choice[1] = L->L->tune().grid;
+    choice[2] = L->L->tune().remap;
+  }
+  return 0;
+}
+
+int amss_level_set_tune(amss_level* L, int shape, int grid, int remap) {
+  if (!L) return fail(AMSS_ERR_ARG, "amss_level_set_tune: null level");
+  const int r = L->L->set_tune(BatchTune{shape, grid, remap});
+  if (r) return fail(r, L->L->err());
+  return 0;
+}
+
+int amss_level_get_tune(amss_level* L, int* choice) {
+  if (!L || !choice) return fail(AMSS_ERR_ARG, "amss_level_get_tune: null argument");
+  choice[0] = L->L->tune().shape;
+  choice[1] = L->L->tune().grid;
+  choice[2] = L->L->tune().remap;
+  return 0;
 }
 
 int amss_level_ham_l2(amss_level* L, double* out) {

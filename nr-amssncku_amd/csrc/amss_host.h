@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "bssn_kernels.h"
+#include "bssn_batch.h"
 
 namespace amss {
 
@@ -78,6 +78,7 @@ struct GhostSeg {
 
 class BssnLevel {
  public:
+  struct Ev { hipEvent_t a, b; int stage; };
   BssnLevel(int npatch, const int* shapes, const double* bboxes, int gauge, double eps, double chitiny, int bc,
             int gw);
   ~BssnLevel();
@@ -94,6 +95,11 @@ class BssnLevel {
   int set_variant(int v);
   int variant() const { return variant_; }
   int kernel_ms(double* ms4, long* n4);
+  // batched-step tile shape / grid / order: time every candidate on a
+  // sub-step of the current state and keep the fastest; or set one
+  int autotune(double* ms, int nms);
+  int set_tune(const BatchTune& t);
+  const BatchTune& tune() const { return tune_; }
   int npatch() const { return (int)patches_.size(); }
   long npts(int patch) const;
 
@@ -108,7 +114,13 @@ class BssnLevel {
  private:
   int Step(double dT, bool want_res);  // one RK4 step, one launch per patch and sub-step
   int StepBatched(double dT, bool want_res);
+  int StepGraph(double dT);
+  void rotate_lists();  // the end-of-step list rotation (bssn_class.C:3845-3859)
+  int slot_of(MyList<var>* l) const;
+  int time_event_begin(int stage, Ev** ev);
   void build_ghost_segments();
+  void build_batch_tables();
+  void clear_graphs();
   void sync(MyList<var>* list, int stage);
   FieldsIn list_in(const Block& b, MyList<var>* l) const;
   FieldsOut list_out(const Block& b, MyList<var>* l) const;
@@ -127,9 +139,28 @@ class BssnLevel {
   bool ham_valid_ = false;
   bool timing_ = false;
   int variant_ = 1;
-  struct Ev { hipEvent_t a, b; int stage; };
   std::vector<Ev> ev_;       // event pairs of the current step() call
   size_t ev_used_ = 0;
+  // batched step: device tables of patches, tiles (per tile shape), boundary
+  // boxes and exchange boxes; interior remainders that no whole tile covers
+  std::vector<PatchDesc> hpd_;
+  PatchDesc* dpd_ = nullptr;
+  std::vector<TileRef> htiles_[kNumTileShapes];
+  TileRef* dtiles_[kNumTileShapes] = {};
+  std::vector<std::pair<int, Box>> rem_[kNumTileShapes];
+  std::vector<FillBox> hfill_;
+  FillBox* dfill_ = nullptr;
+  long fill_max_ = 0;
+  std::vector<SegRef> hseg_;
+  SegRef* dseg_ = nullptr;
+  long seg_max_ = 0;
+  long maxn_ = 0;
+  int ncu_ = 256;
+  BatchTune tune_{0, 0, 1};
+  bool tuned_ = false;
+  bool capturing_ = false;
+  struct Graph { int key; double dT; hipGraphExec_t exec; };
+  std::vector<Graph> graphs_;
   double ms_[4] = {0, 0, 0, 0};
   long nl_[4] = {0, 0, 0, 0};
   std::string err_;

@@ -134,6 +134,12 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
   LCHK(hipMalloc(&dflag_, sizeof(int)));
   LCHK(hipMalloc(&dpart_, nparts_ * sizeof(double)));
   build_ghost_segments();
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.multiProcessorCount > 0)
+    ncu_ = prop.multiProcessorCount;
+  build_batch_tables();
 }
 
 BssnLevel::~BssnLevel() {
@@ -153,8 +159,14 @@ BssnLevel::~BssnLevel() {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
   }
+  clear_graphs();
   if (dflag_) (void)hipFree(dflag_);
   if (dpart_) (void)hipFree(dpart_);
+  if (dpd_) (void)hipFree(dpd_);
+  for (TileRef* t : dtiles_)
+    if (t) (void)hipFree(t);
+  if (dfill_) (void)hipFree(dfill_);
+  if (dseg_) (void)hipFree(dseg_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -270,17 +282,7 @@ int BssnLevel::Step(double dT, bool want_res) {
       Rk4Param R{it, variant_, cst[it], tinny};
       const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
       Ev* ev = nullptr;
-      if (timing_) {
-        if (ev_used_ == ev_.size()) {
-          Ev e{};
-          RCHK(hipEventCreate(&e.a));
-          RCHK(hipEventCreate(&e.b));
-          ev_.push_back(e);
-        }
-        ev = &ev_[ev_used_++];
-        ev->stage = it;
-        RCHK(hipEventRecord(ev->a, stream_));
-      }
+      if (int r = time_event_begin(it, &ev)) return r;
       launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, box,
                  coef_[pi], R, gauge_, want_res && it == 0, dflag_, false, stream_);
       if (ev) RCHK(hipEventRecord(ev->b, stream_));
@@ -309,11 +311,7 @@ int BssnLevel::Step(double dT, bool want_res) {
     if (it > 0 && it < 3)
       for (Block* b : blocks_) b->swapList(SynchList_pre, SynchList_cor);  // :3765-3781
   }
-  // rotate: the corrector output becomes the state (:3845-3859)
-  for (Block* b : blocks_) {
-    b->swapList(StateList, SynchList_cor);
-    b->swapList(OldStateList, SynchList_cor);
-  }
+  rotate_lists();
   if (hipGetLastError() != hipSuccess) {
     err_ = "kernel launch failed";
     return AMSS_ERR_HIP;
@@ -321,15 +319,369 @@ int BssnLevel::Step(double dT, bool want_res) {
   return 0;
 }
 
-int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
-  if (mode != 0) {
-    err_ = "batched modes are not built into this library";
-    return AMSS_ERR_UNSUPPORTED;
+// rotate: the corrector output becomes the state, the state the old state
+// (bssn_class.C:3845-3859)
+void BssnLevel::rotate_lists() {
+  for (Block* b : blocks_) {
+    b->swapList(StateList, SynchList_cor);
+    b->swapList(OldStateList, SynchList_cor);
   }
+}
+
+int BssnLevel::time_event_begin(int stage, Ev** ev) {
+  *ev = nullptr;
+  if (!timing_ || capturing_) return 0;
+  if (ev_used_ == ev_.size()) {
+    Ev e{};
+    RCHK(hipEventCreate(&e.a));
+    RCHK(hipEventCreate(&e.b));
+    ev_.push_back(e);
+  }
+  *ev = &ev_[ev_used_++];
+  (*ev)->stage = stage;
+  RCHK(hipEventRecord((*ev)->a, stream_));
+  return 0;
+}
+
+// A list's slot in every Block's slab: lists move as whole 24-field runs.
+int BssnLevel::slot_of(MyList<var>* l) const {
+  const Block& b = *blocks_[0];
+  return (int)((b.fgfs[l->data->sgfn] - b.slab) / ((long)NFIELD * b.npts()));
+}
+
+// Device tables of the batched step: patch views, the interior tiles of every
+// patch for each tile shape (whole tiles only; what is left of an interior
+// goes to the direct kernel), the boundary boxes of every patch and the
+// inter-patch exchange boxes.
+void BssnLevel::build_batch_tables() {
+  const int np = (int)blocks_.size();
+  hpd_.clear();
+  maxn_ = 0;
+  for (Block* b : blocks_) {
+    PatchDesc d;
+    d.slab = b->slab;
+    d.n = b->npts();
+    d.sy = b->shape[0];
+    d.sz = b->shape[0] * b->shape[1];
+    for (int a = 0; a < 3; ++a) d.shape[a] = b->shape[a];
+    hpd_.push_back(d);
+    maxn_ = std::max(maxn_, d.n);
+  }
+  LCHK(hipMalloc(&dpd_, np * sizeof(PatchDesc)));
+  LCHK(hipMemcpy(dpd_, hpd_.data(), np * sizeof(PatchDesc), hipMemcpyHostToDevice));
+  for (int sh = 0; sh < kNumTileShapes; ++sh) {
+    const int T[3] = {tile_dim(sh, 0), tile_dim(sh, 1), tile_dim(sh, 2)};
+    htiles_[sh].clear();
+    rem_[sh].clear();
+    for (int pi = 0; pi < np; ++pi) {
+      const Block& b = *blocks_[pi];
+      Box box;
+      for (int a = 0; a < 3; ++a) {
+        box.n[a] = b.shape[a];
+        box.lo[a] = 3;
+        box.hi[a] = b.shape[a] - 3;
+      }
+      box.sy = b.shape[0];
+      box.sz = (long)b.shape[0] * b.shape[1];
+      int core[3];
+      for (int a = 0; a < 3; ++a) core[a] = 3 + (box.hi[a] - 3) / T[a] * T[a];
+      if (core[0] > 3 && core[1] > 3 && core[2] > 3) {
+        if (b.shape[0] > 65535 || b.shape[1] > 65535 || b.shape[2] > 65535 || pi > 65535) {
+          err_ = "patch too large for the tile list";
+          return;
+        }
+        for (int k = 3; k < core[2]; k += T[2])
+          for (int j = 3; j < core[1]; j += T[1])
+            for (int i = 3; i < core[0]; i += T[0])
+              htiles_[sh].push_back(TileRef{(unsigned short)pi, (unsigned short)i, (unsigned short)j,
+                                            (unsigned short)k});
+      } else {
+        for (int a = 0; a < 3; ++a) core[a] = 3;
+      }
+      for (int a = 0; a < 3; ++a) {  // remainder slab [core[a], hi[a]) along a
+        Box r = box;
+        for (int c = 0; c < a; ++c) r.hi[c] = core[c];
+        r.lo[a] = core[a];
+        if (r.hi[0] > r.lo[0] && r.hi[1] > r.lo[1] && r.hi[2] > r.lo[2]) rem_[sh].push_back({pi, r});
+      }
+    }
+    if ((long)htiles_[sh].size() > (1L << 31) - 1) { err_ = "too many tiles"; return; }
+    if (!htiles_[sh].empty()) {
+      LCHK(hipMalloc(&dtiles_[sh], htiles_[sh].size() * sizeof(TileRef)));
+      LCHK(hipMemcpy(dtiles_[sh], htiles_[sh].data(), htiles_[sh].size() * sizeof(TileRef),
+                     hipMemcpyHostToDevice));
+    }
+  }
+  // boundary boxes: the points outside [w.lo, n - w.hi) of each patch, as at
+  // most 6 disjoint slabs (z faces whole, then y faces, then x faces)
+  hfill_.clear();
+  fill_max_ = 0;
+  if (bc_ != AMSS_BC_NONE) {
+    for (int pi = 0; pi < np; ++pi) {
+      const Block& b = *blocks_[pi];
+      int wl[3], wh[3];
+      for (int a = 0; a < 3; ++a) {
+        if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
+          wl[a] = wh[a] = gw_;
+        } else {
+          const double d = b.getdX(a);
+          wh[a] = std::fabs(b.X[a][b.shape[a] - 1] - patches_[pi].bbox[3 + a]) < d ? 1 : 0;
+          wl[a] = std::fabs(b.X[a][0] - patches_[pi].bbox[a]) < d ? 1 : 0;
+        }
+      }
+      int lo[3] = {0, 0, 0}, hi[3] = {b.shape[0], b.shape[1], b.shape[2]};
+      for (int a = 2; a >= 0; --a) {
+        for (int side = 0; side < 2; ++side) {
+          FillBox f;
+          f.patch = pi;
+          for (int c = 0; c < 3; ++c) {
+            f.lo[c] = lo[c];
+            f.ext[c] = hi[c] - lo[c];
+          }
+          if (side == 0) {
+            f.ext[a] = wl[a];
+          } else {
+            f.lo[a] = b.shape[a] - wh[a];
+            f.ext[a] = wh[a];
+          }
+          if (f.ext[0] > 0 && f.ext[1] > 0 && f.ext[2] > 0) {
+            hfill_.push_back(f);
+            fill_max_ = std::max(fill_max_, (long)f.ext[0] * f.ext[1] * f.ext[2]);
+          }
+        }
+        lo[a] = wl[a];
+        hi[a] = b.shape[a] - wh[a];
+      }
+    }
+  }
+  hseg_.clear();
+  seg_max_ = 0;
+  for (const GhostSeg& g : segs_) {
+    SegRef r;
+    r.src = g.src;
+    r.dst = g.dst;
+    for (int a = 0; a < 3; ++a) {
+      r.slo[a] = g.slo[a];
+      r.dlo[a] = g.dlo[a];
+      r.ext[a] = g.ext[a];
+    }
+    hseg_.push_back(r);
+    seg_max_ = std::max(seg_max_, (long)g.ext[0] * g.ext[1] * g.ext[2]);
+  }
+  if (hfill_.size() > 65535 || hseg_.size() > 65535 || blocks_.size() > 65535) {
+    err_ = "too many boundary or exchange boxes";
+    return;
+  }
+  if (!hfill_.empty()) {
+    LCHK(hipMalloc(&dfill_, hfill_.size() * sizeof(FillBox)));
+    LCHK(hipMemcpy(dfill_, hfill_.data(), hfill_.size() * sizeof(FillBox), hipMemcpyHostToDevice));
+  }
+  if (!hseg_.empty()) {
+    LCHK(hipMalloc(&dseg_, hseg_.size() * sizeof(SegRef)));
+    LCHK(hipMemcpy(dseg_, hseg_.data(), hseg_.size() * sizeof(SegRef), hipMemcpyHostToDevice));
+  }
+}
+
+// One RK4 step of the whole level, one launch per kind of work and sub-step
+// (same order of operations as Step).
+int BssnLevel::StepBatched(double dT, bool want_res) {
+  const double tinny = chitiny_ - 1.0;
+  const double cst[4] = {0.5 * dT, 0.5 * dT, dT, (1.0 / 6.0) * dT};
+  const int np = (int)blocks_.size();
+  const int sh = tune_.shape;
+  launch_enforce_patches(dpd_, np, maxn_, slot_of(StateList), stream_);
+  for (int it = 0; it < 4; ++it) {
+    MyList<var>* in = it == 0 ? StateList : SynchList_pre;
+    MyList<var>* out = it == 0 ? SynchList_pre : SynchList_cor;
+    const Roles m{slot_of(in), slot_of(StateList), slot_of(RHSList), slot_of(out)};
+    const Rk4Param R{it, 1, cst[it], tinny};
+    const bool res = want_res && it == 0;
+    Ev* ev = nullptr;
+    if (int r = time_event_begin(it, &ev)) return r;
+    launch_rk4_tiles(gauge_, dpd_, dtiles_[sh], 0, (int)htiles_[sh].size(), m, coef_[0], R, res, tune_, dflag_,
+                     stream_);
+    if (ev) RCHK(hipEventRecord(ev->b, stream_));
+    AuxOut aux;
+    for (int a = 0; a < 18; ++a) aux.gam[a] = nullptr;
+    for (int a = 0; a < 6; ++a) aux.ric[a] = nullptr;
+    Rk4Param Rd = R;
+    Rd.variant = 0;
+    for (const auto& pr : rem_[sh]) {
+      const Block& b = *blocks_[pr.first];
+      for (int a = 0; a < 7; ++a) aux.res[a] = b.fgfs[kResidualBase + a];
+      launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, pr.second,
+                 coef_[pr.first], Rd, gauge_, res, dflag_, false, stream_, true);
+    }
+    if (bc_ != AMSS_BC_FROZEN_GHOSTS || gw_ < 3) {
+      for (int pi = 0; pi < np; ++pi) {
+        const Block& b = *blocks_[pi];
+        Box all;
+        for (int a = 0; a < 3; ++a) {
+          all.n[a] = b.shape[a];
+          all.lo[a] = 3;
+          all.hi[a] = b.shape[a] - 3;
+        }
+        all.sy = b.shape[0];
+        all.sz = (long)b.shape[0] * b.shape[1];
+        for (int a = 0; a < 7; ++a) aux.res[a] = b.fgfs[kResidualBase + a];
+        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, all,
+                   coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false);
+      }
+    }
+    launch_fill_boxes(dpd_, dfill_, (int)hfill_.size(), (int)std::min(fill_max_, 1L << 30), m, it, tinny, dflag_,
+                      stream_);
+    launch_seg_boxes(dpd_, dseg_, (int)hseg_.size(), (int)std::min(seg_max_, 1L << 30), m.out, stream_);
+    if (it > 0 && it < 3)
+      for (Block* b : blocks_) b->swapList(SynchList_pre, SynchList_cor);
+  }
+  rotate_lists();
+  if (hipGetLastError() != hipSuccess) {
+    err_ = "kernel launch failed";
+    return AMSS_ERR_HIP;
+  }
+  return 0;
+}
+
+// One batched step replayed from a hipGraph.  The list slots return to the
+// same assignment every 3 steps, so at most 3 graphs (keyed by the slot
+// assignment at the start of the step, and dT) cover a run.
+int BssnLevel::StepGraph(double dT) {
+  const int key = (((slot_of(StateList) * 5 + slot_of(OldStateList)) * 5 + slot_of(SynchList_pre)) * 5 +
+                   slot_of(SynchList_cor)) * 5 + slot_of(RHSList);
+  for (const Graph& g : graphs_)
+    if (g.key == key && g.dT == dT) {
+      RCHK(hipGraphLaunch(g.exec, stream_));
+      // the pre/cor swaps of sub-steps 1 and 2 cancel; only the rotation is left
+      rotate_lists();
+      return 0;
+    }
+  hipGraph_t graph = nullptr;
+  RCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  capturing_ = true;
+  const int r = StepBatched(dT, false);
+  capturing_ = false;
+  const hipError_t e = hipStreamEndCapture(stream_, &graph);
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess) {
+    err_ = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+    return AMSS_ERR_HIP;
+  }
+  Graph g{key, dT, nullptr};
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    err_ = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+    return AMSS_ERR_HIP;
+  }
+  graphs_.push_back(g);
+  RCHK(hipGraphLaunch(g.exec, stream_));
+  return 0;
+}
+
+void BssnLevel::clear_graphs() {
+  for (Graph& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+  graphs_.clear();
+}
+
+int BssnLevel::set_tune(const BatchTune& t) {
+  if (t.shape < 0 || t.shape >= kNumTileShapes || t.grid < 0 || (t.remap != 0 && t.remap != 1)) {
+    err_ = "bad tile shape, grid or order";
+    return AMSS_ERR_ARG;
+  }
+  tune_ = t;
+  tuned_ = true;
+  clear_graphs();
+  return 0;
+}
+
+// Candidates: 3 tile shapes x {one workgroup per tile, persistent with 1x and
+// 2x the resident workgroups (2 per CU at 2 waves per SIMD)} x {list order,
+// XCD-contiguous order}.  Each is timed (best of 2 after a warm-up) on
+// sub-step 0 of the current state, which writes only the RHS and stage lists
+// that the next step overwrites anyway.  ms (may be NULL) receives the times
+// in candidate order.
+int BssnLevel::autotune(double* ms, int nms) {
+  const double tinny = chitiny_ - 1.0;
+  const Roles m{slot_of(StateList), slot_of(StateList), slot_of(RHSList), slot_of(SynchList_pre)};
+  const Rk4Param R{0, 1, 0.0, tinny};
+  hipEvent_t e0, e1;
+  RCHK(hipEventCreate(&e0));
+  RCHK(hipEventCreate(&e1));
+  double best = 1e300;
+  BatchTune pick = tune_;
+  int c = 0;
+  AuxOut aux;
+  for (int a = 0; a < 18; ++a) aux.gam[a] = nullptr;
+  for (int a = 0; a < 6; ++a) aux.ric[a] = nullptr;
+  Rk4Param Rd = R;
+  Rd.variant = 0;
+  for (int sh = 0; sh < kNumTileShapes; ++sh)
+    for (int gm = 0; gm < 3; ++gm)
+      for (int rm = 0; rm < 2; ++rm, ++c) {
+        const BatchTune t{sh, gm * 2 * ncu_, rm};
+        float tbest = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+          RCHK(hipEventRecord(e0, stream_));
+          launch_rk4_tiles(gauge_, dpd_, dtiles_[sh], 0, (int)htiles_[sh].size(), m, coef_[0], R, false, t, dflag_,
+                           stream_);
+          for (const auto& pr : rem_[sh]) {
+            const Block& b = *blocks_[pr.first];
+            for (int a = 0; a < 7; ++a) aux.res[a] = b.fgfs[kResidualBase + a];
+            launch_rk4(list_in(b, StateList), list_in(b, StateList), list_out(b, RHSList),
+                       list_out(b, SynchList_pre), aux, pr.second, coef_[pr.first], Rd, gauge_, false, dflag_, false,
+                       stream_, true);
+          }
+          RCHK(hipEventRecord(e1, stream_));
+          RCHK(hipEventSynchronize(e1));
+          float t_ms = 0.f;
+          RCHK(hipEventElapsedTime(&t_ms, e0, e1));
+          if (rep > 0) tbest = std::min(tbest, t_ms);
+        }
+        if (ms && c < nms) ms[c] = tbest;
+        if (tbest < best) {
+          best = tbest;
+          pick = t;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (hipGetLastError() != hipSuccess) {
+    err_ = "kernel launch failed";
+    return AMSS_ERR_HIP;
+  }
+  tune_ = pick;
+  tuned_ = true;
+  clear_graphs();
+  return 0;
+}
+
+int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
+  if (mode < 0 || mode > 2) {
+    err_ = "mode must be 0 (per patch), 1 (batched) or 2 (batched, hipGraph)";
+    return AMSS_ERR_ARG;
+  }
+  if (nsteps < 0) {
+    err_ = "negative step count";
+    return AMSS_ERR_ARG;
+  }
+  if (mode > 0 && !tuned_)
+    if (int r = autotune(nullptr, 0)) return r;
   RCHK(hipMemsetAsync(dflag_, 0, sizeof(int), stream_));
   ev_used_ = 0;
-  for (int n = 0; n < nsteps; ++n)
-    if (int r = Step(dT, want_ham && n == nsteps - 1)) return r;
+  for (int n = 0; n < nsteps; ++n) {
+    const bool res = want_ham && n == nsteps - 1;
+    int r = 0;
+    if (mode == 0)
+      r = Step(dT, res);
+    else if (mode == 1 || res)
+      r = StepBatched(dT, res);
+    else
+      r = StepGraph(dT);
+    if (r) return r;
+  }
   int flag = 0;
   RCHK(hipMemcpyAsync(&flag, dflag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
   RCHK(hipStreamSynchronize(stream_));

@@ -1,0 +1,89 @@
+// bssn_batch.h -- one refinement level advanced as a whole: every patch's
+// interior is cut into LDS tiles and one launch per RK4 sub-step walks the
+// tile list of all patches (persistent workgroups striding over the list), so
+// small and uneven patches fill the GPU together.  Ghost fill, inter-patch
+// ghost exchange and enforce_ga run as one launch each over box tables.
+//
+// Field addressing: a Block keeps its 5 time levels as consecutive runs of 24
+// fields in one slab (list slot s, field v at slab + (24 s + v) n) and the 7
+// residuals after them; Block::swapList only permutes which slot a list
+// names, so a launch carries the slot of each role instead of 96 pointers.
+#pragma once
+#include "bssn_kernels.h"
+
+namespace amss {
+
+// Device view of one Block.
+struct PatchDesc {
+  double* slab;  // (5 * 24 + 7) * n doubles
+  long n;        // points per grid function
+  int sy, sz;    // strides (shape[0], shape[0] * shape[1])
+  int shape[3];
+};
+
+// One tile of the interior: patch index and the array index of its first point.
+struct TileRef {
+  unsigned short patch, i0, j0, k0;
+};
+
+// List slot (0..4) of each role of a fused sub-step.
+struct Roles {
+  int in, u0, acc, out;
+};
+constexpr int kResidualBase = 5 * NFIELD;  // first residual grid function of a slab
+
+// A box of points of one patch (ghost fill) or a copy between patches.
+struct FillBox {
+  int patch;
+  int lo[3], ext[3];
+};
+struct SegRef {
+  int src, dst;
+  int slo[3], dlo[3], ext[3];
+};
+
+// Tile shapes of the batched kernel (points per tile along x, y, z).
+constexpr int kNumTileShapes = 3;
+__host__ __device__ constexpr int tile_dim(int shape, int a) {
+  // 0: 8x8x4, 1: 4x8x8, 2: 8x4x8 -- 256 threads each
+  return shape == 0 ? (a == 2 ? 4 : 8) : shape == 1 ? (a == 0 ? 4 : 8) : (a == 1 ? 4 : 8);
+}
+
+// Workgroup t runs on XCD t % 8 (round-robin dispatch over the 8 XCDs of an
+// MI355X).  Giving XCD x the contiguous run of tiles [start(x), start(x+1))
+// keeps neighbouring tiles, whose halos overlap, behind the same L2.
+__host__ __device__ __forceinline__ int xcd_spread(int t, int n) {
+  const int x = t & 7, l = t >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + l;
+}
+
+struct BatchTune {
+  int shape;   // tile shape index
+  int grid;    // workgroups per launch (persistent); 0 = one per tile
+  int remap;   // 1 = XCD-contiguous tile order
+};
+
+// Fused RHS + RK4 sub-step over tiles [t0, t0 + ntiles) of the tile list.
+void launch_rk4_tiles(int gauge, const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
+                      const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune, int* flag,
+                      hipStream_t s);
+#define AMSS_DECL_TILES(G, S)                                                                             \
+  void launch_rk4_tiles_g##G##_s##S(const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m, \
+                                    const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune,  \
+                                    int* flag, hipStream_t s);
+AMSS_DECL_TILES(0, 0)
+AMSS_DECL_TILES(0, 1)
+AMSS_DECL_TILES(0, 2)
+AMSS_DECL_TILES(2, 0)
+AMSS_DECL_TILES(2, 1)
+AMSS_DECL_TILES(2, 2)
+#undef AMSS_DECL_TILES
+// Frozen / copied boundary points of every patch (boxes of points of u0 -> out).
+void launch_fill_boxes(const PatchDesc* P, const FillBox* B, int nbox, int maxpts, const Roles& m, int stage,
+                       double tinny, int* flag, hipStream_t s);
+// Inter-patch ghost exchange of list slot `slot`.
+void launch_seg_boxes(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, int slot, hipStream_t s);
+// enforce_ga on every point of list slot `slot` of every patch.
+void launch_enforce_patches(const PatchDesc* P, int npatch, long maxn, int slot, hipStream_t s);
+
+}  // namespace amss

@@ -1,0 +1,75 @@
+// bssn_batch_impl.h -- the tile-list kernel of the batched level step,
+// instantiated per gauge and tile shape in bssn_batch_g*_s*.hip.
+#pragma once
+#include "bssn_batch.h"
+#include "bssn_kernels_impl.h"
+
+namespace amss {
+
+// Persistent workgroups stride over the tile list; each tile runs the same
+// LDS-staged point computation as rk4_lds_kernel, with the fields of its
+// patch addressed through the slab and the role slots.  The LDS buffers need
+// no extra barrier between tiles: the next tile's prologue fills buffer 0,
+// last read by group 22, which every wave finished before the barrier that
+// opened group 23.
+template <int GAUGE, int STAGE, bool CO, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(2, 2)))
+rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T, int t0, int ntiles, Roles m,
+                 Coef C, Rk4Param R, int remap, int* flag) {
+  using Tl = LdsTile<TX, TY, TZ>;
+  __shared__ __attribute__((aligned(16))) double tiles[2 * Tl::NPAD];
+  lds_double* b0 = (lds_double*)tiles;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // keep the compiler from hoisting per-thread LDS addresses out of the
+    // loop (hundreds of loop-invariant addresses would occupy VGPRs)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+    const int ltv = (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3));
+    const TileRef r = T[t0 + (remap ? xcd_spread(t, ntiles) : t)];
+    const PatchDesc d = P[r.patch];
+    const long p = (r.i0 + tx) + (long)(r.j0 + ty) * d.sy + (long)(r.k0 + tz) * d.sz;
+    const long ln = (long)NFIELD * d.n;
+    const SlabFields in{d.slab + m.in * ln, d.n}, u0{d.slab + m.u0 * ln, d.n};
+    const SlabFields acc{d.slab + m.acc * ln, d.n}, out{d.slab + m.out * ln, d.n};
+    const SlabAux aux{d.slab + kResidualBase * d.n, d.n};
+    if (bad_input(in, p)) *flag = 1;
+    LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + Tl::NPAD, p, ltv,
+                                     (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz, d.sy, d.sz, (int)d.n, tid};
+    E.prologue();
+    Rk4SinkT<STAGE, SlabFields, SlabFields, SlabAux> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+    bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+    S.finish();
+  }
+}
+
+template <int G, int TX, int TY, int TZ>
+void launch_rk4_tiles_shape(const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
+                            const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune, int* flag,
+                            hipStream_t s) {
+  if (ntiles <= 0) return;
+  const unsigned g = (unsigned)(tune.grid > 0 && tune.grid < ntiles ? tune.grid : ntiles);
+  const int nt = TX * TY * TZ;
+  switch (R.stage) {
+    case 0:
+      if (want_res)
+        rk4_tiles_kernel<G, 0, true, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag);
+      else
+        rk4_tiles_kernel<G, 0, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag);
+      break;
+    case 1: rk4_tiles_kernel<G, 1, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag); break;
+    case 2: rk4_tiles_kernel<G, 2, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag); break;
+    default: rk4_tiles_kernel<G, 3, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag); break;
+  }
+}
+
+// One translation unit per gauge and tile shape (they compile in parallel).
+#define AMSS_BATCH_TU(G, S)                                                                                  \
+  void launch_rk4_tiles_g##G##_s##S(const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m, \
+                                    const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune,  \
+                                    int* flag, hipStream_t s) {                                              \
+    launch_rk4_tiles_shape<G, tile_dim(S, 0), tile_dim(S, 1), tile_dim(S, 2)>(P, T, t0, ntiles, m, C, R,      \
+                                                                              want_res, tune, flag, s);      \
+  }
+
+}  // namespace amss

@@ -37,8 +37,22 @@ __host__ __device__ constexpr int parity(int v, int axis) {
   return (odd[v] >> axis) & 1 ? -1 : 1;
 }
 
-struct FieldsIn { const double* __restrict__ p[NFIELD]; };
-struct FieldsOut { double* __restrict__ p[NFIELD]; };
+// The 24 fields of one time level as explicit pointers (kernel arguments).
+struct FieldsIn {
+  const double* __restrict__ p[NFIELD];
+  __device__ __forceinline__ const double* ptr(int v) const { return p[v]; }
+};
+struct FieldsOut {
+  double* __restrict__ p[NFIELD];
+  __device__ __forceinline__ double* ptr(int v) const { return p[v]; }
+};
+// The 24 fields of one time level inside a Block's slab: field v starts at
+// base + v * n (a Block keeps its lists as consecutive 24-field runs).
+struct SlabFields {
+  double* base;
+  long n;
+  __device__ __forceinline__ double* ptr(int v) const { return base + v * n; }
+};
 
 // Finite-difference coefficients of one patch (host-computed from
 // dX = X[1]-X[0] etc., bssn_rhs.f90:142-144).

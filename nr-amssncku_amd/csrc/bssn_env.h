@@ -220,6 +220,13 @@ struct AuxOut {
   double* __restrict__ gam[18];
   double* __restrict__ ric[6];
   double* __restrict__ res[7];
+  __device__ __forceinline__ double* resp(int a) const { return res[a]; }
+};
+// The 7 residual fields inside a Block's slab: residual a at base + a * n.
+struct SlabAux {
+  double* base;
+  long n;
+  __device__ __forceinline__ double* resp(int a) const { return base + a * n; }
 };
 
 // Plain RHS sink: k -> rhs arrays.
@@ -302,12 +309,12 @@ struct Rk4Sink {
 // Same epilogue with the sub-step fixed at compile time: the interior kernels
 // use this form, since a runtime stage makes the compiler duplicate the whole
 // point computation per stage branch (and blows the register budget).
-template <int STAGE>
+template <int STAGE, class FI = FieldsIn, class FO = FieldsOut, class AX = AuxOut>
 struct Rk4SinkT {
-  const FieldsIn& U0;
-  const FieldsOut& ACC;
-  const FieldsOut& OUT;
-  const AuxOut& X;
+  const FI& U0;
+  const FO& ACC;
+  const FO& OUT;
+  const AX& X;
   long p;
   double c, tinny;
   double gA[12];
@@ -315,26 +322,26 @@ struct Rk4SinkT {
   __device__ __forceinline__ void put(int v, double k) {
     double kk = k;
     if (STAGE == 0)
-      ACC.p[v][p] = k;
+      ACC.ptr(v)[p] = k;
     else if (STAGE < 3)
-      ACC.p[v][p] = ACC.p[v][p] + 2.0 * k;
+      ACC.ptr(v)[p] = ACC.ptr(v)[p] + 2.0 * k;
     else
-      kk = k + ACC.p[v][p];
-    double u = U0.p[v][p] + c * kk;
+      kk = k + ACC.ptr(v)[p];
+    double u = U0.ptr(v)[p] + c * kk;
     if (v == CHI) u = u < tinny ? tinny : u;
     if (STAGE < 3 && v >= DXX && v <= AZZ)
       gA[v - DXX] = u;
     else
-      OUT.p[v][p] = u;
+      OUT.ptr(v)[p] = u;
   }
   __device__ __forceinline__ void gam(int, double) {}
   __device__ __forceinline__ void ric(int, double) {}
-  __device__ __forceinline__ void res(int a, double x) { X.res[a][p] = x; }
+  __device__ __forceinline__ void res(int a, double x) { X.resp(a)[p] = x; }
   __device__ __forceinline__ void finish() {
     if (STAGE < 3) {
       enforce_point(gA, gA + 6);
 #pragma unroll
-      for (int s = 0; s < 12; ++s) OUT.p[DXX + s][p] = gA[s];
+      for (int s = 0; s < 12; ++s) OUT.ptr(DXX + s)[p] = gA[s];
     }
   }
 };

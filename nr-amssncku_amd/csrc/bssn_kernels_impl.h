@@ -2,19 +2,20 @@
 // Instantiated once per gauge in bssn_gauge0.hip / bssn_gauge2.hip so the
 // heavy point kernels compile in parallel translation units.
 #pragma once
-#include "bssn_kernels.h"
+#include "bssn_batch.h"
 #include "bssn_lds.h"
 
 namespace amss {
 
 #define AMSS_POINT_KERNEL __global__ void __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 
-__device__ __forceinline__ bool bad_input(const FieldsIn& F, long p) {
+template <class FI>
+__device__ __forceinline__ bool bad_input(const FI& F, long p) {
   // the reference's NaN test covers the first 21 fields (bssn_rhs.f90:110-113)
   bool b = false;
 #pragma unroll
   for (int v = 0; v < 21; ++v) {
-    const double x = F.p[v][p];
+    const double x = F.ptr(v)[p];
     b |= !(x == x);
   }
   return b;
@@ -76,7 +77,9 @@ AMSS_POINT_KERNEL rk4_interior_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, F
 }
 
 // LDS-staged variant: 8x8x4-point tiles, halo tiles of every field group in
-// LDS (double-buffered, filled by direct global->LDS loads).
+// LDS (double-buffered, filled by direct global->LDS loads).  A 1-D grid of
+// tiles in XCD-contiguous order: consecutive tiles, whose halos overlap, run
+// behind the same L2.
 constexpr int kLdsTX = 8, kLdsTY = 8, kLdsTZ = 4;
 template <int GAUGE, int STAGE, bool CO>
 AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
@@ -84,8 +87,10 @@ AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, Fields
   using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
   __shared__ __attribute__((aligned(16))) double tiles[2 * Tl::NPAD];
   const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
-  const int i0 = box.lo[0] + blockIdx.x * kLdsTX, j0 = box.lo[1] + blockIdx.y * kLdsTY;
-  const int k0 = box.lo[2] + blockIdx.z * kLdsTZ;
+  const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
+  const int b = xcd_spread(blockIdx.x, gridDim.x);
+  const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;
+  const int k0 = box.lo[2] + (b / (gx * gy)) * kLdsTZ;
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   // the launch covers whole tiles only (remainder slabs go to the direct kernel)
   const long p = i + j * box.sy + k * box.sz;
@@ -94,7 +99,8 @@ AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, Fields
   lds_double* t0 = (lds_double*)tiles;
   LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + Tl::NPAD, p,
                                   (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
-                                  (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N};
+                                  (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N,
+                                  (int)threadIdx.x};
   E.prologue();
   Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
   bssn_rhs_point<GAUGE, CO, false, false>(E, S);
@@ -144,8 +150,8 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
     const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
     for (int a = 0; a < 3; ++a) core.hi[a] = box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a];
     if (!box_empty(core)) {
-      const dim3 gr((core.hi[0] - core.lo[0]) / kLdsTX, (core.hi[1] - core.lo[1]) / kLdsTY,
-                    (core.hi[2] - core.lo[2]) / kLdsTZ);
+      const unsigned gr = (unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
+                          ((core.hi[2] - core.lo[2]) / kLdsTZ);
       const int nt = kLdsTX * kLdsTY * kLdsTZ;
       switch (R.stage) {
         case 0:

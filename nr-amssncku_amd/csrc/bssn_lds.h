@@ -39,13 +39,13 @@ struct LdsTile {
   static_assert(NT % 64 == 0, "whole waves");
 };
 
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, class FI = FieldsIn>
 struct LdsEnv {
   using T = LdsTile<TX, TY, TZ>;
   static constexpr long SY = T::LX, SZ = (long)T::LX * T::LY;
 
-  const FieldsIn& F;
-  const FieldsIn& M;
+  const FI& F;
+  const FI& M;
   const Coef& C;
   lds_double* buf0;   // two LDS tile buffers
   lds_double* buf1;
@@ -54,6 +54,7 @@ struct LdsEnv {
   int gbase;      // global index of tile element (0,0,0)
   int sy, sz;     // global strides
   int gmax;       // number of points of one field
+  int tid;        // thread index in the workgroup
   int g = 0;      // next group to open
 
   // Stream the halo tile of field v into buffer b: every lane issues ROUNDS
@@ -62,11 +63,11 @@ struct LdsEnv {
   // address) and the loop has no bounds branches -- branches here split the
   // point kernel into many blocks and wreck its register allocation.
   __device__ __forceinline__ void issue(int v, lds_double* b) const {
-    const double* base = F.p[v];
-    const int wave = threadIdx.x >> 6;
+    const double* base = F.ptr(v);
+    const int wave = tid >> 6;
 #pragma unroll
     for (int r = 0; r < T::ROUNDS; ++r) {
-      const int c = r * T::NT + (int)threadIdx.x;
+      const int c = r * T::NT + tid;
       const int e = 2 * c;
       const int x = e % T::LX, y = (e / T::LX) % T::LY, z = e / (T::LX * T::LY);
       int gi = gbase + x + y * sy + z * sz;  // < 2^29 points per field
@@ -89,8 +90,8 @@ struct LdsEnv {
   __device__ __forceinline__ void end() const {}
 
   __device__ __forceinline__ double at(int, long o) const { return cur()[lt + o]; }
-  __device__ __forceinline__ double val(int v) const { return F.p[v][p]; }
-  __device__ __forceinline__ double mat(int m) const { return M.p[m][p]; }
+  __device__ __forceinline__ double val(int v) const { return F.ptr(v)[p]; }
+  __device__ __forceinline__ double mat(int m) const { return M.ptr(m)[p]; }
 
   __device__ __forceinline__ double d1(int v, long s) const {
     return (at(v, -2 * s) - at(v, 2 * s)) + 8.0 * (at(v, s) - at(v, -s));
