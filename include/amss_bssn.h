@@ -184,6 +184,27 @@ int amss_level_autotune(amss_level *L, double *ms, int nms, int *choice);
 int amss_level_set_tune(amss_level *L, int shape, int grid, int remap);
 int amss_level_get_tune(amss_level *L, int *choice);
 
+/* ---- discrete adjoint of one RK4 step (GAUGE 0, vacuum, frozen ghosts) ----
+ * No counterpart in the reference: the reverse of bssn_class::Step
+ * (bssn_class.C:3042-3163, 3401-3526) for one Block with ghost width 3 and
+ * frozen ghost zones, for gradients of a scalar of the evolved state.
+ * All pointers are 24 device arrays of the Block's full shape ex, in the
+ * reference field order; calls are ordered on `stream` (a hipStream_t, NULL
+ * for the default stream) and do not synchronise.  The object holds the
+ * stage states and work arrays (about 520 grid functions).
+ *  forward: U -> Unext (one RK4 step, the adjoint's own evaluation of it);
+ *  step:    lam = dJ/dU^{n+1} in, dJ/dU^n out, U = the state at t^n;
+ *  rhs:     out = dJ/dU of sum_p lam(p) . rhs(U)(p) over the points with
+ *           +-3 room on every axis (the vector-Jacobian product of
+ *           compute_rhs_bssn_ there). */
+typedef struct amss_adjoint amss_adjoint;
+amss_adjoint *amss_adjoint_create(const int *ex, const double *h, double eps, double chitiny, int gauge);
+void amss_adjoint_destroy(amss_adjoint *A);
+int amss_adjoint_forward(amss_adjoint *A, double dT, const double *const *U, double *const *Unext, void *stream);
+int amss_adjoint_step(amss_adjoint *A, double dT, const double *const *U, double *const *lam, void *stream);
+int amss_adjoint_rhs(amss_adjoint *A, const double *const *U, const double *const *lam, double *const *out,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/amss_bssn.h"
+#include "amss_adjoint.h"
 #include "amss_host.h"
 
 namespace amss {
@@ -497,6 +498,60 @@ int amss_level_timing(amss_level* L, int enable) {
 int amss_level_kernel_ms(amss_level* L, double* ms4, long* n4) {
   if (!L || !ms4 || !n4) return fail(AMSS_ERR_ARG, "amss_level_kernel_ms: null argument");
   return L->L->kernel_ms(ms4, n4);
+}
+
+// ---- step adjoint ----------------------------------------------------------
+struct amss_adjoint {
+  AdjointStep* A;
+};
+
+amss_adjoint* amss_adjoint_create(const int* ex, const double* h, double eps, double chitiny, int gauge) {
+  if (!ex || !h) {
+    fail(AMSS_ERR_ARG, "amss_adjoint_create: null argument");
+    return nullptr;
+  }
+  if (gauge != 0) {
+    fail(AMSS_ERR_UNSUPPORTED, "amss_adjoint_create: the adjoint covers GAUGE 0 only");
+    return nullptr;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (!(h[a] > 0.0)) {
+      fail(AMSS_ERR_ARG, "amss_adjoint_create: grid spacing must be positive");
+      return nullptr;
+    }
+  auto* A = new AdjointStep(ex, make_coef(h, eps > 0 ? eps : 0.0), chitiny);
+  if (!A->ok()) {
+    fail(AMSS_ERR_ARG, A->err());
+    delete A;
+    return nullptr;
+  }
+  return new amss_adjoint{A};
+}
+
+void amss_adjoint_destroy(amss_adjoint* A) {
+  if (!A) return;
+  (void)hipDeviceSynchronize();
+  delete A->A;
+  delete A;
+}
+
+int amss_adjoint_forward(amss_adjoint* A, double dT, const double* const* U, double* const* Unext, void* stream) {
+  if (!A || !U) return fail(AMSS_ERR_ARG, "amss_adjoint_forward: null argument");
+  const int r = A->A->forward(dT, U, Unext, (hipStream_t)stream);
+  return r ? fail(r, A->A->err()) : 0;
+}
+
+int amss_adjoint_step(amss_adjoint* A, double dT, const double* const* U, double* const* lam, void* stream) {
+  if (!A || !U || !lam) return fail(AMSS_ERR_ARG, "amss_adjoint_step: null argument");
+  const int r = A->A->backward(dT, U, lam, (hipStream_t)stream);
+  return r ? fail(r, A->A->err()) : 0;
+}
+
+int amss_adjoint_rhs(amss_adjoint* A, const double* const* U, const double* const* lam, double* const* out,
+                     void* stream) {
+  if (!A || !U || !lam || !out) return fail(AMSS_ERR_ARG, "amss_adjoint_rhs: null argument");
+  const int r = A->A->rhs_vjp(U, lam, out, (hipStream_t)stream);
+  return r ? fail(r, A->A->err()) : 0;
 }
 
 }  // extern "C"

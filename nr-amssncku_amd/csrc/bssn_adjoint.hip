@@ -1,0 +1,332 @@
+// bssn_adjoint.hip -- kernels of the discrete adjoint of one RK4 step
+// (GAUGE 0, vacuum, Symmetry 0, frozen ghost zones of width 3):
+//   * adj_point_kernel: reverse of the pointwise algebra -> feature bars,
+//   * adj_gather_kernel: transposed stencils (first/second derivatives,
+//     upwind advection, KO) -> dJ/d(input state) at every point,
+//   * the forward re-evaluation of the step's stages, and the pointwise
+//     reverse of the RK4 combination, chi's lower bound and enforce_ga.
+#include "bssn_adjoint.h"
+#include "bssn_adjoint_launch.h"
+#include "bssn_kernels_impl.h"
+
+namespace amss {
+
+namespace {
+
+__device__ __forceinline__ bool inside(const Box& b, int i, int j, int k) {
+  return i >= b.lo[0] && i < b.hi[0] && j >= b.lo[1] && j < b.hi[1] && k >= b.lo[2] && k < b.hi[2];
+}
+
+// k only (no Christoffel / Ricci / residual outputs)
+struct KSink {
+  const FieldsOut& O;
+  long p;
+  __device__ __forceinline__ void put(int v, double k) { O.p[v][p] = k; }
+  __device__ __forceinline__ void gam(int, double) {}
+  __device__ __forceinline__ void ric(int, double) {}
+  __device__ __forceinline__ void res(int, double) {}
+};
+
+__global__ void __launch_bounds__(kRhsThreads) rhs_k_kernel(FieldsIn in, FieldsOut out, Box box, Coef C) {
+  int i, j, k;
+  if (!interior_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  InteriorEnv E{in, in, C, p, box.sy, box.sz};
+  KSink S{out, p};
+  bssn_rhs_point<0, false, false, false>(E, S);
+}
+
+__global__ void __launch_bounds__(256) adj_point_kernel(FieldsIn in, FieldsIn lam, double* bars, Box box, Coef C) {
+  int i, j, k;
+  if (!interior_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  InteriorEnv E{in, in, C, p, box.sy, box.sz};
+  double l[NFIELD];
+  for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
+  const AdjSink O{bars, (long)box.n[0] * box.n[1] * box.n[2], p};
+  bssn_rhs_adjoint_point(E, l, O);
+}
+
+// dJ/dU(q) = value bar + sum over the output points p whose stencils read q
+// of (stencil weight) x (feature bar at p); bars and lam exist at interior
+// points only.  blockIdx.y = field.
+__global__ void __launch_bounds__(256) adj_gather_kernel(FieldsIn in, FieldsIn lam, const double* __restrict__ bars,
+                                                         FieldsOut out, Box box, Coef C, int accumulate) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= N) return;
+  const int v = blockIdx.y;
+  const int i = (int)(q % box.n[0]), j = (int)((q / box.n[0]) % box.n[1]), k = (int)(q / ((long)box.n[0] * box.n[1]));
+  auto rd = [&](const double* f, int di, int dj, int dk) -> double {
+    return inside(box, i + di, j + dj, k + dk) ? f[q + di + dj * box.sy + dk * box.sz] : 0.0;
+  };
+  auto sh = [](int a, int o, int& di, int& dj, int& dk) {
+    di = a == 0 ? o : 0;
+    dj = a == 1 ? o : 0;
+    dk = a == 2 ? o : 0;
+  };
+  double acc = inside(box, i, j, k) ? bars[v * N + q] : 0.0;
+  const int gs = adj_grad_slot(v);
+  if (gs >= 0) {
+    for (int a = 0; a < 3; ++a) {
+      const double* b = bars + adj_grad_index(gs, a) * N;
+      int d[4][3];
+      const int os[4] = {2, 1, -1, -2};
+      for (int t = 0; t < 4; ++t) sh(a, os[t], d[t][0], d[t][1], d[t][2]);
+      acc += C.d1[a] * ((rd(b, d[0][0], d[0][1], d[0][2]) - rd(b, d[3][0], d[3][1], d[3][2])) +
+                        8.0 * (rd(b, d[2][0], d[2][1], d[2][2]) - rd(b, d[1][0], d[1][1], d[1][2])));
+    }
+  }
+  const int hs = adj_hess_slot(v);
+  if (hs >= 0) {
+    const int ta[6] = {0, 0, 0, 1, 1, 2}, tb[6] = {0, 1, 2, 1, 2, 2};
+    for (int t = 0; t < 6; ++t) {
+      const double* b = bars + adj_hess_index(hs, t) * N;
+      const int a = ta[t], c = tb[t];
+      if (a == c) {
+        int d1[3], d2[3], d3[3], d4[3];
+        sh(a, -1, d1[0], d1[1], d1[2]);
+        sh(a, 1, d2[0], d2[1], d2[2]);
+        sh(a, -2, d3[0], d3[1], d3[2]);
+        sh(a, 2, d4[0], d4[1], d4[2]);
+        acc += C.d2[a] * (16.0 * (rd(b, d1[0], d1[1], d1[2]) + rd(b, d2[0], d2[1], d2[2])) -
+                          (rd(b, d3[0], d3[1], d3[2]) + rd(b, d4[0], d4[1], d4[2])) - 30.0 * rd(b, 0, 0, 0));
+      } else {
+        const double dm = t == 1 ? C.dm[0] : (t == 2 ? C.dm[1] : C.dm[2]);
+        const int os[4] = {-2, -1, 1, 2};
+        const double w[4] = {1.0, -8.0, 8.0, -1.0};  // d1 weights at offsets -2, -1, 1, 2
+        double s = 0.0;
+        for (int x = 0; x < 4; ++x)
+          for (int y = 0; y < 4; ++y) {
+            int di = 0, dj = 0, dk = 0;
+            if (a == 0) di += os[x];
+            if (a == 1) dj += os[x];
+            if (c == 1) dj += os[y];
+            if (c == 2) dk += os[y];
+            s += w[x] * w[y] * rd(b, di, dj, dk);
+          }
+        acc += dm * s;
+      }
+    }
+  }
+  // advection: sum over output points p = q - o e_a
+  const double* L = lam.p[v];
+  for (int a = 0; a < 3; ++a) {
+    const double* beta = in.p[BETAX + a];
+    const double up[7] = {0.0, 0.0, -3.0, -10.0, 18.0, -6.0, 1.0};  // offsets -3..3 from p
+    double s = 0.0;
+    for (int o = -3; o <= 3; ++o) {
+      int di, dj, dk;
+      sh(a, -o, di, dj, dk);
+      if (!inside(box, i + di, j + dj, k + dk)) continue;
+      const long pp = q + di + dj * box.sy + dk * box.sz;
+      const double b = beta[pp], l = L[pp];
+      // up weight at offset o; the downwind stencil is its mirror
+      s += up[o + 3] * fmax(b, 0.0) * l - up[3 - o] * fmin(b, 0.0) * l;
+    }
+    acc += C.d1[a] * s;
+  }
+  // Kreiss-Oliger (symmetric)
+  for (int a = 0; a < 3; ++a) {
+    const double kw[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+    double s = 0.0;
+    for (int o = -3; o <= 3; ++o) {
+      int di, dj, dk;
+      sh(a, o, di, dj, dk);
+      s += kw[o + 3] * rd(L, di, dj, dk);
+    }
+    acc += C.ko[a] * s;
+  }
+  if (accumulate)
+    out.p[v][q] += acc;
+  else
+    out.p[v][q] = acc;
+}
+
+// ---- enforce_ga and its vector-Jacobian product ------------------------------
+__device__ void enforce_vjp(const double din[6], const double ain[6], double gb[6], double ab[6]) {
+  constexpr double W[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};
+  double g[6];
+  for (int s = 0; s < 6; ++s) g[s] = din[s] + (sdiag(s) ? 1.0 : 0.0);
+  const double det = g[0] * g[3] * g[5] + g[1] * g[4] * g[2] + g[2] * g[1] * g[4] - g[2] * g[3] * g[2] -
+                     g[1] * g[1] * g[5] - g[0] * g[4] * g[4];
+  const double sc = 1.0 / pow(det, 1.0 / 3.0);
+  double p[6];
+  for (int s = 0; s < 6; ++s) p[s] = g[s] * sc;
+  const double c[6] = {p[3] * p[5] - p[4] * p[4], p[2] * p[4] - p[1] * p[5], p[1] * p[4] - p[3] * p[2],
+                       p[0] * p[5] - p[2] * p[2], p[1] * p[2] - p[0] * p[4], p[0] * p[3] - p[1] * p[1]};
+  double trA = 0.0;
+  for (int t = 0; t < 6; ++t) trA += W[t] * c[t] * ain[t];
+  // outputs: d'_s = p_s - diag, a'_s = a_s - p_s trA / 3
+  double pb[6], abo[6], trA_b = 0.0;
+  for (int s = 0; s < 6; ++s) {
+    pb[s] = gb[s] - (1.0 / 3.0) * trA * ab[s];
+    trA_b -= (1.0 / 3.0) * p[s] * ab[s];
+    abo[s] = ab[s];
+  }
+  double cb[6];
+  for (int t = 0; t < 6; ++t) {
+    abo[t] += trA_b * W[t] * c[t];
+    cb[t] = trA_b * W[t] * ain[t];
+  }
+  pb[3] += cb[0] * p[5]; pb[5] += cb[0] * p[3]; pb[4] -= 2.0 * cb[0] * p[4];
+  pb[2] += cb[1] * p[4]; pb[4] += cb[1] * p[2]; pb[1] -= cb[1] * p[5]; pb[5] -= cb[1] * p[1];
+  pb[1] += cb[2] * p[4]; pb[4] += cb[2] * p[1]; pb[3] -= cb[2] * p[2]; pb[2] -= cb[2] * p[3];
+  pb[0] += cb[3] * p[5]; pb[5] += cb[3] * p[0]; pb[2] -= 2.0 * cb[3] * p[2];
+  pb[1] += cb[4] * p[2]; pb[2] += cb[4] * p[1]; pb[0] -= cb[4] * p[4]; pb[4] -= cb[4] * p[0];
+  pb[0] += cb[5] * p[3]; pb[3] += cb[5] * p[0]; pb[1] -= 2.0 * cb[5] * p[1];
+  // p = g det(g)^(-1/3): d det / d g_t = W_t cof_t(g)
+  const double C0[6] = {g[3] * g[5] - g[4] * g[4], g[2] * g[4] - g[1] * g[5], g[1] * g[4] - g[3] * g[2],
+                        g[0] * g[5] - g[2] * g[2], g[1] * g[2] - g[0] * g[4], g[0] * g[3] - g[1] * g[1]};
+  double dot = 0.0;
+  for (int s = 0; s < 6; ++s) dot += pb[s] * g[s];
+  for (int t = 0; t < 6; ++t) {
+    gb[t] = sc * pb[t] - dot * (1.0 / 3.0) * (sc / det) * W[t] * C0[t];
+    ab[t] = abo[t];
+  }
+}
+
+// Forward stage s of the step at every point:
+//   interior: acc (s = 0: k; 1, 2: += 2k), z = Y0 + c k (s = 3: Y0 + c (k + acc)),
+//             Ynext = LB(z), enforce_ga for s < 3;
+//   boundary: Ynext = the frozen-ghost fill of Y0 (LB on chi, enforce_ga for s < 3).
+__global__ void __launch_bounds__(256) fwd_stage_kernel(FieldsIn Y0, FieldsIn kk, FieldsOut acc, FieldsOut z,
+                                                        FieldsOut Yn, Box box, int stage, double c, double tinny) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= N) return;
+  const int i = (int)(q % box.n[0]), j = (int)((q / box.n[0]) % box.n[1]), k = (int)(q / ((long)box.n[0] * box.n[1]));
+  double u[NFIELD];
+  if (inside(box, i, j, k)) {
+    for (int v = 0; v < NFIELD; ++v) {
+      const double kv = kk.p[v][q];
+      double w;
+      if (stage == 0) {
+        acc.p[v][q] = kv;
+        w = Y0.p[v][q] + c * kv;
+      } else if (stage < 3) {
+        acc.p[v][q] = acc.p[v][q] + 2.0 * kv;
+        w = Y0.p[v][q] + c * kv;
+      } else {
+        w = Y0.p[v][q] + c * (kv + acc.p[v][q]);
+      }
+      z.p[v][q] = w;
+      u[v] = w;
+    }
+  } else {
+    for (int v = 0; v < NFIELD; ++v) u[v] = Y0.p[v][q];
+  }
+  u[CHI] = u[CHI] < tinny ? tinny : u[CHI];
+  if (stage < 3) enforce_point(u + DXX, u + AXX);
+  for (int v = 0; v < NFIELD; ++v) Yn.p[v][q] = u[v];
+}
+
+// Reverse of the last combination: lam = dJ/dU^{n+1} ->
+//   interior: lz = lam (chi masked by the lower bound), lY0 = lz,
+//             lk3 = c lz, lk0 = c lz, lk1 = lk2 = 2 c lz;
+//   boundary: lY0 = lam through the stage-3 fill (chi masked), lk = 0.
+__global__ void __launch_bounds__(256) adj_final_kernel(FieldsIn lam, FieldsIn z3, FieldsIn Y0, FieldsOut lY0,
+                                                        FieldsOut lk0, FieldsOut lk1, FieldsOut lk2, FieldsOut lk3,
+                                                        Box box, double c, double tinny) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= N) return;
+  const int i = (int)(q % box.n[0]), j = (int)((q / box.n[0]) % box.n[1]), k = (int)(q / ((long)box.n[0] * box.n[1]));
+  const bool in = inside(box, i, j, k);
+  for (int v = 0; v < NFIELD; ++v) {
+    double l = lam.p[v][q];
+    if (v == CHI && (in ? z3.p[CHI][q] : Y0.p[CHI][q]) < tinny) l = 0.0;
+    lY0.p[v][q] = l;
+    lk0.p[v][q] = in ? c * l : 0.0;
+    lk1.p[v][q] = in ? 2.0 * c * l : 0.0;
+    lk2.p[v][q] = in ? 2.0 * c * l : 0.0;
+    lk3.p[v][q] = in ? c * l : 0.0;
+  }
+}
+
+// Reverse of stage s < 3 given lY = dJ/dY_{s+1} (RHS adjoint of stage s+1):
+//   interior: Y = E(LB(z)): lz = LB' E'^T lY; lY0 += lz; lk_s += c lz;
+//   boundary: Y = fill(Y0): lY0 += LB' E'(Y0)^T lY.
+__global__ void __launch_bounds__(256) adj_stage_kernel(FieldsIn lY, FieldsIn z, FieldsIn Y0, FieldsOut lY0,
+                                                        FieldsOut lk, Box box, double c, double tinny) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= N) return;
+  const int i = (int)(q % box.n[0]), j = (int)((q / box.n[0]) % box.n[1]), k = (int)(q / ((long)box.n[0] * box.n[1]));
+  const bool in = inside(box, i, j, k);
+  const FieldsIn& W = in ? z : Y0;
+  double l[NFIELD], w[NFIELD];
+  for (int v = 0; v < NFIELD; ++v) {
+    l[v] = lY.p[v][q];
+    w[v] = W.p[v][q];
+  }
+  enforce_vjp(w + DXX, w + AXX, l + DXX, l + AXX);
+  if (w[CHI] < tinny) l[CHI] = 0.0;
+  for (int v = 0; v < NFIELD; ++v) {
+    lY0.p[v][q] += l[v];
+    if (in) lk.p[v][q] += c * l[v];
+  }
+}
+
+// dJ/dU^n = E'(U^n)^T lY0 (the step's opening enforce_ga on every point).
+__global__ void __launch_bounds__(256) adj_open_kernel(FieldsIn U, FieldsIn lY0, FieldsOut lam, long N) {
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= N) return;
+  double l[NFIELD], d[6], a[6];
+  for (int v = 0; v < NFIELD; ++v) l[v] = lY0.p[v][q];
+  for (int s = 0; s < 6; ++s) {
+    d[s] = U.p[DXX + s][q];
+    a[s] = U.p[AXX + s][q];
+  }
+  enforce_vjp(d, a, l + DXX, l + AXX);
+  for (int v = 0; v < NFIELD; ++v) lam.p[v][q] = l[v];
+}
+
+__global__ void __launch_bounds__(256) copy_enforce_kernel(FieldsIn U, FieldsOut Y, long N) {
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= N) return;
+  double u[NFIELD];
+  for (int v = 0; v < NFIELD; ++v) u[v] = U.p[v][q];
+  enforce_point(u + DXX, u + AXX);
+  for (int v = 0; v < NFIELD; ++v) Y.p[v][q] = u[v];
+}
+
+unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, const Coef& C, hipStream_t s) {
+  rhs_k_kernel<<<interior_grid(box), kRhsThreads, 0, s>>>(in, out, box, C);
+}
+void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, const Box& box, const Coef& C,
+                      hipStream_t s) {
+  adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, box, C);
+}
+void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, const double* bars, const FieldsOut& out,
+                       const Box& box, const Coef& C, bool accumulate, hipStream_t s) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  adj_gather_kernel<<<dim3(nblk(N), NFIELD), 256, 0, s>>>(in, lam, bars, out, box, C, accumulate ? 1 : 0);
+}
+void adj_launch_fwd_stage(const FieldsIn& Y0, const FieldsIn& k, const FieldsOut& acc, const FieldsOut& z,
+                          const FieldsOut& Yn, const Box& box, int stage, double c, double tinny, hipStream_t s) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  fwd_stage_kernel<<<nblk(N), 256, 0, s>>>(Y0, k, acc, z, Yn, box, stage, c, tinny);
+}
+void adj_launch_final(const FieldsIn& lam, const FieldsIn& z3, const FieldsIn& Y0, const FieldsOut& lY0,
+                      const FieldsOut* lk, const Box& box, double c, double tinny, hipStream_t s) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  adj_final_kernel<<<nblk(N), 256, 0, s>>>(lam, z3, Y0, lY0, lk[0], lk[1], lk[2], lk[3], box, c, tinny);
+}
+void adj_launch_stage(const FieldsIn& lY, const FieldsIn& z, const FieldsIn& Y0, const FieldsOut& lY0,
+                      const FieldsOut& lk, const Box& box, double c, double tinny, hipStream_t s) {
+  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  adj_stage_kernel<<<nblk(N), 256, 0, s>>>(lY, z, Y0, lY0, lk, box, c, tinny);
+}
+void adj_launch_open(const FieldsIn& U, const FieldsIn& lY0, const FieldsOut& lam, long N, hipStream_t s) {
+  adj_open_kernel<<<nblk(N), 256, 0, s>>>(U, lY0, lam, N);
+}
+void adj_launch_copy_enforce(const FieldsIn& U, const FieldsOut& Y, long N, hipStream_t s) {
+  copy_enforce_kernel<<<nblk(N), 256, 0, s>>>(U, Y, N);
+}
+
+}  // namespace amss

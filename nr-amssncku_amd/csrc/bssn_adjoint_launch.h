@@ -1,0 +1,24 @@
+// bssn_adjoint_launch.h -- launchers of the adjoint kernels (bssn_adjoint.hip).
+#pragma once
+#include "bssn_kernels.h"
+
+namespace amss {
+
+// RHS k at the interior points (no auxiliary outputs), GAUGE 0.
+void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, const Coef& C, hipStream_t s);
+// Reverse of the pointwise algebra: feature bars (kAdjFeatures x N) at interior points.
+void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, const Box& box, const Coef& C,
+                      hipStream_t s);
+// Transposed stencils: out (= or +=) dJ/d(in) at every point.
+void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, const double* bars, const FieldsOut& out,
+                       const Box& box, const Coef& C, bool accumulate, hipStream_t s);
+void adj_launch_fwd_stage(const FieldsIn& Y0, const FieldsIn& k, const FieldsOut& acc, const FieldsOut& z,
+                          const FieldsOut& Yn, const Box& box, int stage, double c, double tinny, hipStream_t s);
+void adj_launch_final(const FieldsIn& lam, const FieldsIn& z3, const FieldsIn& Y0, const FieldsOut& lY0,
+                      const FieldsOut* lk, const Box& box, double c, double tinny, hipStream_t s);
+void adj_launch_stage(const FieldsIn& lY, const FieldsIn& z, const FieldsIn& Y0, const FieldsOut& lY0,
+                      const FieldsOut& lk, const Box& box, double c, double tinny, hipStream_t s);
+void adj_launch_open(const FieldsIn& U, const FieldsIn& lY0, const FieldsOut& lam, long N, hipStream_t s);
+void adj_launch_copy_enforce(const FieldsIn& U, const FieldsOut& Y, long N, hipStream_t s);
+
+}  // namespace amss

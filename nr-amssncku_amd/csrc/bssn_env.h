@@ -66,6 +66,17 @@ struct InteriorEnv {
   __device__ __forceinline__ double adv(int v, double /*off*/, const double b[3]) const {
     return up1(v, 1, b[0], C.d1[0]) + up1(v, sy, b[1], C.d1[1]) + up1(v, sz, b[2], C.d1[2]);
   }
+  // the two one-sided stencils of every direction, unscaled (adjoint use)
+  __device__ __forceinline__ void updn(int v, double up[3], double dn[3]) const {
+    const long st[3] = {1, sy, sz};
+    const double f0 = 10.0 * at(v, 0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const long s = st[a];
+      up[a] = (at(v, 3 * s) - 3.0 * at(v, -s)) + (18.0 * at(v, s) - 6.0 * at(v, 2 * s)) - f0;
+      dn[a] = (at(v, -3 * s) - 3.0 * at(v, s)) + (18.0 * at(v, -s) - 6.0 * at(v, -2 * s)) - f0;
+    }
+  }
   __device__ __forceinline__ double ko1(int v, long s, double c20) const {
     return (at(v, -3 * s) + at(v, 3 * s)) - 6.0 * (at(v, -2 * s) + at(v, 2 * s)) +
            15.0 * (at(v, -s) + at(v, s)) - c20;

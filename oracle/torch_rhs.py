@@ -242,6 +242,61 @@ def rhs(U, h, eps, gauge=0):
     return out
 
 
+def enforce(U):
+    """enforce_ga (enforce_algebra.f90:72-127) on every point: unit
+    determinant of the conformal metric, trace-free A (trace taken with the
+    cofactors of the rescaled metric)."""
+    g = [U[DXX + s] + (1.0 if DIAG[s] else 0.0) for s in range(6)]
+    det = (g[0] * g[3] * g[5] + g[1] * g[4] * g[2] + g[2] * g[1] * g[4] - g[2] * g[3] * g[2] -
+           g[1] * g[1] * g[5] - g[0] * g[4] * g[4])
+    sc = 1.0 / torch.pow(det, 1.0 / 3.0)
+    p = [x * sc for x in g]
+    cof = [p[3] * p[5] - p[4] * p[4], p[2] * p[4] - p[1] * p[5], p[1] * p[4] - p[3] * p[2],
+           p[0] * p[5] - p[2] * p[2], p[1] * p[2] - p[0] * p[4], p[0] * p[3] - p[1] * p[1]]
+    A = [U[AXX + s] for s in range(6)]
+    trA = sum(WSYM[s] * cof[s] * A[s] for s in range(6))
+    out = list(U)
+    for s in range(6):
+        out[DXX + s] = p[s] - (1.0 if DIAG[s] else 0.0)
+        out[AXX + s] = A[s] - (1.0 / 3.0) * p[s] * trA
+    return out
+
+
+def step(U, dt, h, eps, chitiny=1e-5, gauge=0):
+    """One RK4 step of bssn_class::Step for one block with frozen ghost
+    zones of width 3 (bssn_class.C:3042-3163, 3401-3526): enforce_ga on the
+    state, four right-hand sides, rungekutta4_rout's combinations
+    (rungekutta4_rout.f90:110-128), lowerboundset on chi (fmisc.f90:1644-1657)
+    and the next call's enforce_ga on each stage output; ghost points take
+    the state's values (with the same bound and enforce)."""
+    tinny = chitiny - 1.0
+    I = (slice(3, -3), slice(3, -3), slice(3, -3))
+    Y0 = enforce(U)
+    c = (0.5 * dt, 0.5 * dt, dt, dt / 6.0)
+    Y = Y0
+    acc = None
+    for st in range(4):
+        k = rhs(Y, h, eps, gauge)
+        if st == 0:
+            acc = list(k)
+        elif st < 3:
+            acc = [a + 2.0 * x for a, x in zip(acc, k)]
+        z = [Y0[v][I] + c[st] * (k[v] + acc[v] if st == 3 else k[v]) for v in range(24)]
+        z[CHI] = torch.where(z[CHI] < tinny, torch.full_like(z[CHI], tinny), z[CHI])
+        ghost = list(Y0)
+        ghost[CHI] = torch.where(Y0[CHI] < tinny, torch.full_like(Y0[CHI], tinny), Y0[CHI])
+        if st < 3:
+            z = enforce(z)
+            ghost = enforce(ghost)
+        Yn = []
+        for v in range(24):
+            t = ghost[v].clone()
+            t[I] = z[v]
+            Yn.append(t)
+        Y = Yn
+    return Y
+
+
 def interior_vjp(U_np, lam_np, h, eps, gauge=0):
     """d/dU of sum_v sum_p lam_v(p) rhs_v(p) over the interior points, by
     torch autograd.  U_np: 24 arrays [nz, ny, nx]; lam_np: 24 arrays of the
@@ -254,4 +309,4 @@ def interior_vjp(U_np, lam_np, h, eps, gauge=0):
     return [o.detach().numpy() for o in out], gr
 
 
-__all__ = ["rhs", "interior_vjp", "Stencils"]
+__all__ = ["rhs", "enforce", "step", "interior_vjp", "Stencils"]

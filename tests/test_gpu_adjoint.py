@@ -1,0 +1,151 @@
+"""The hand-written adjoint kernels: the RHS vector-Jacobian product against
+torch autograd of the PyTorch restatement, the adjoint's forward step
+against the level's fused step, and the step adjoint against central
+finite differences of the step itself."""
+import numpy as np
+import pytest
+
+from helpers import grid, random_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(st):
+    import torch
+    return torch.tensor(np.stack(st), dtype=torch.float64, device="cuda").contiguous()
+
+
+def test_rhs_vjp_matches_torch_autograd():
+    import torch
+    import torch_rhs
+    from amssbssn.adjoint import StepAdjoint
+    nx, ny, nz = 14, 15, 16
+    X, Y, Z, _ = grid(nx, ny, nz)
+    h = (X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0])
+    st = random_state((nz, ny, nx), seed=31, amp=0.05)
+    rng = np.random.default_rng(3)
+    lam_i = [rng.standard_normal((nz - 6, ny - 6, nx - 6)) for _ in range(24)]
+    _, ref = torch_rhs.interior_vjp(st, lam_i, h, 0.02)
+    lam = [np.zeros((nz, ny, nx)) for _ in range(24)]
+    for v in range(24):
+        lam[v][3:-3, 3:-3, 3:-3] = lam_i[v]
+    A = StepAdjoint((nx, ny, nz), h, eps=0.02)
+    U, L = _dev(st), _dev(lam)
+    out = torch.zeros_like(U)
+    A.rhs_vjp(U, L, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    A.close()
+    worst = 0.0
+    for v in range(24):
+        e = rel_err(got[v], ref[v])
+        worst = max(worst, e)
+        assert e <= 1e-12, (v, e)
+    print(f"RHS VJP vs torch autograd: worst {worst:.2e}")
+
+
+def _perturbed_puncture(n=16, seed=41, amp=1e-3, shift=(0.0, 0.0, 0.0)):
+    import amssbssn as A
+    cfg = A.config1(n=n)
+    st = cfg.initial_state(0)
+    rng = np.random.default_rng(seed)
+    for v in range(24):
+        st[v] = st[v] + amp * rng.standard_normal(st[v].shape)
+    for a in range(3):
+        st[18 + a] = st[18 + a] + shift[a]
+    p = cfg.patches[0]
+    return cfg, p, st
+
+
+def test_adjoint_forward_matches_level_step():
+    import torch
+    import amssbssn as A
+    from amssbssn.adjoint import StepAdjoint
+    cfg, p, st = _perturbed_puncture()
+    lv = A.Level(cfg.patches, eps=cfg.eps)
+    lv.set_state(0, st)
+    assert lv.step(cfg.dt, 1) == 0
+    ref = lv.get_state(0)
+    lv.close()
+    adj = StepAdjoint(p.shape, p.h, eps=cfg.eps)
+    U = _dev(st)
+    Un = torch.empty_like(U)
+    adj.forward(cfg.dt, U, Un)
+    torch.cuda.synchronize()
+    got = Un.cpu().numpy()
+    adj.close()
+    for v in range(24):
+        assert rel_err(got[v], ref[v]) <= 1e-13, v
+
+
+@pytest.mark.parametrize("nsteps", [1, 3])
+def test_step_adjoint_matches_central_differences(nsteps):
+    """J = <w, U^{n+nsteps}>: <dJ/dU^n, e> from the adjoint against the
+    central difference of J along a random direction e."""
+    import torch
+    from amssbssn.adjoint import StepAdjoint
+    # a shift well away from zero: the upwind stencil's switch at beta = 0 is
+    # a kink that a finite difference of 1e-6 would straddle at some points
+    cfg, p, st = _perturbed_puncture(shift=(0.03, -0.02, 0.025))
+    rng = np.random.default_rng(7)
+    w = _dev([rng.standard_normal(st[0].shape) for _ in range(24)])
+    e = _dev([rng.standard_normal(st[0].shape) for _ in range(24)])
+    adj = StepAdjoint(p.shape, p.h, eps=cfg.eps)
+    U0 = _dev(st)
+
+    def run(U):
+        traj = [U]
+        for _ in range(nsteps):
+            nxt = torch.empty_like(U)
+            adj.forward(cfg.dt, traj[-1], nxt)
+            traj.append(nxt)
+        return traj
+
+    traj = run(U0)
+    lam = w.clone()
+    for n in range(nsteps - 1, -1, -1):
+        adj.step(cfg.dt, traj[n], lam)
+    torch.cuda.synchronize()
+    lin = float(torch.sum(lam * e))
+    eps = 1e-6
+
+    def J(s):
+        return float(torch.sum(w * run(U0 + s * e)[-1]))
+    fd = (J(eps) - J(-eps)) / (2 * eps)
+    adj.close()
+    print(f"{nsteps} step(s): adjoint {lin:.12e} fd {fd:.12e} rel {abs(fd - lin) / abs(lin):.2e}")
+    assert abs(fd - lin) <= 1e-6 * abs(lin)
+
+
+def test_step_adjoint_matches_torch_autograd_of_the_step():
+    """One full step (enforce, four RHS, RK4 combination, chi bound, ghost
+    fill) differentiated by torch autograd on the CPU against the adjoint
+    kernels, on an 18 x 19 x 20 block with generic shift signs."""
+    import torch
+    import torch_rhs
+    from amssbssn.adjoint import StepAdjoint
+    nx, ny, nz = 18, 19, 20
+    X, Y, Z, _ = grid(nx, ny, nz)
+    h = (X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0])
+    st = random_state((nz, ny, nx), seed=33, amp=0.03)
+    rng = np.random.default_rng(9)
+    w = [rng.standard_normal((nz, ny, nx)) for _ in range(24)]
+    dt = 0.25 * min(h)
+    U = [torch.tensor(u, requires_grad=True) for u in st]
+    out = torch_rhs.step(U, dt, h, 0.02)
+    ref = torch.autograd.grad(out, U, grad_outputs=[torch.tensor(x) for x in w])
+    adj = StepAdjoint((nx, ny, nz), h, eps=0.02)
+    Ud, lam = _dev(st), _dev(w)
+    nxt = torch.empty_like(Ud)
+    adj.forward(dt, Ud, nxt)
+    adj.step(dt, Ud, lam)
+    torch.cuda.synchronize()
+    fwd, got = nxt.cpu().numpy(), lam.cpu().numpy()
+    adj.close()
+    worst = 0.0
+    for v in range(24):
+        assert rel_err(fwd[v], out[v].detach().numpy()) <= 1e-12, ("forward", v)
+        e = rel_err(got[v], ref[v].numpy())
+        worst = max(worst, e)
+        assert e <= 1e-11, (v, e)
+    print(f"step adjoint vs torch autograd: worst {worst:.2e}")

@@ -45,3 +45,21 @@ def test_torch_vjp_matches_central_differences():
         return sum(float(np.sum(l * o.numpy())) for l, o in zip(lam, out))
     fd = (J(1e-5) - J(-1e-5)) / 2e-5
     assert abs(fd - lin) <= 1e-6 * abs(lin)
+
+
+def test_torch_step_matches_oracle_step(orc):
+    """The PyTorch restatement of one RK4 step (frozen ghosts, width 3)
+    against the C oracle's step."""
+    import torch
+    import torch_rhs
+    nx, ny, nz = 18, 19, 20
+    X, Y, Z, bbox = grid(nx, ny, nz)
+    h = (X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0])
+    st = random_state((nz, ny, nx), seed=33, amp=0.03)
+    dt = 0.25 * min(h)
+    U = [u.copy() for u in st]
+    orc.step(U, X, Y, Z, bbox, dt, 0.02, 1e-5, gauge=0, bc=0, gw=3)
+    T = torch_rhs.step([torch.tensor(u) for u in st], dt, h, 0.02)
+    for v in range(24):
+        m = max(float(np.max(np.abs(U[v]))), 1e-300)
+        assert float(np.max(np.abs(T[v].numpy() - U[v]))) <= 1e-13 * m, v
