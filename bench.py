@@ -90,6 +90,30 @@ def batching(steps, warmup):
     return out
 
 
+def adjoint_leg(steps, checkpoint=20):
+    """The 64^3 differentiation workload (config-1 data over [-16,16]^3,
+    parameters: masses, x positions, 16^3 spline coefficients on A_xx): J and
+    dJ/dtheta through the checkpointed step adjoint, the wall time of the
+    backward pass over the forward pass, and a central difference of dJ/dm1."""
+    import numpy as np
+    import amssbssn.gradient as G
+    G.value_and_grad(G.config3(64, steps=2, checkpoint=1))  # load kernels, warm caches
+    cfg = G.config3(64, steps=steps, checkpoint=checkpoint)
+    J, g, tm = G.value_and_grad(cfg)
+    theta = cfg.theta0()
+    h = 1e-6
+    tp, tmn = theta.copy(), theta.copy()
+    tp[0] += h
+    tmn[0] -= h
+    fd = (G.value(cfg, tp) - G.value(cfg, tmn)) / (2 * h)
+    return dict(workload=f"64^3 patch + 3 ghosts over [-16,16]^3, {steps} RK4 steps, checkpoint every {checkpoint}; "
+                         "J = shell integral of alpha chi (r = 6, width 1); theta = (m1, m2, x1, x2, 16^3 A_xx spline)",
+                J=J, dJ_dtheta_head=[float(x) for x in g[:4]], spline_grad_norm=float(np.linalg.norm(g[4:])),
+                forward_s=round(tm["forward_s"], 4), backward_s=round(tm["backward_s"], 4),
+                adjoint_over_forward=round(tm["backward_s"] / tm["forward_s"], 3),
+                fd_dJ_dm1=fd, fd_rel_err_m1=abs(fd - g[0]) / abs(g[0]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +125,8 @@ def main():
                     help="interior kernel: 0 direct stencil reads, 1 LDS-staged tiles (default: the level's)")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="timed steps of each mode on the 37-patch level (0 skips the batching comparison)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the 64^3 gradient run (0 skips the adjoint/forward measurement)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -170,6 +196,8 @@ def main():
     lv.close()
     if args.batch_steps > 0:
         out["batching"] = batching(args.batch_steps, 2)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
         out["cpu_baseline"]["gpu_speedup"] = round(value / out["cpu_baseline"]["value"], 1)

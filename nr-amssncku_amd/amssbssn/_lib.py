@@ -63,6 +63,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AmssError(f"libamssbssn.so is not built ({LIB_PATH}); run `make -C nr-amssncku_amd/csrc`")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7.  Once
+    # torch is loaded, the library's dependency on that soname binds to
+    # torch's copy; loaded the other way round, torch would later bring in a
+    # second runtime that finds no GPU.  So load torch first when it exists.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, (args, res) in SIGNATURES.items():
         f = getattr(L, name)

@@ -42,6 +42,7 @@ class DiffConfig:
     spline_seed: int = 3
     a_noise: float = 1e-8
     noise_seed: int = 0
+    shift0: tuple = (0.0, 0.0, 0.0)  # constant initial shift (0 in the benchmark)
     shell_r: float = 6.0
     shell_width: float = 1.0
 
@@ -109,6 +110,9 @@ class InitialData:
         fields[CHI] = psi ** -4 - 1.0
         fields[LAP] = psi ** -2 - 1.0
         fields[AXX] = fields[AXX] + axx
+        for a in range(3):
+            if self.cfg.shift0[a]:
+                fields[18 + a] = fields[18 + a] + self.cfg.shift0[a]
         return torch.stack(fields).contiguous()
 
 
