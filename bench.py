@@ -28,9 +28,10 @@ METRIC = "BSSN RHS Mpoints/s + % FP64/HBM roofline; batched vs plain; adjoint/fo
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch of the fused kernel from the committed rocprof
-    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), if present."""
-    p = os.path.join(ROOT, "profiles", "rk4_interior_c1_traffic.json")
+    """HBM bytes per launch of the fused LDS kernel on this workload, from the
+    committed rocprof PMC summary (request-size-resolved reads + WRITE_SIZE,
+    calibrated against a copy kernel of known size), if present."""
+    p = os.path.join(ROOT, "profiles", "rk4_lds_c1_128cubed_traffic.json")
     try:
         with open(p) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -171,7 +172,7 @@ def main():
     ach_tf = flops / ktime / 1e12
     ai = perf.FLOPS_PER_PT_RHS / (sum(perf.BYTES_PER_PT_STAGE) / 4)
     ridge = perf.PEAK_FP64_TFLOPS * 1e12 / (perf.PEAK_HBM_GBS * 1e9)
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles() if (variant == 1 and args.n == 128) else None
     roof = dict(bound="hbm" if ai < ridge else "fp64", achieved=round(ach_gbs, 1), peak=perf.PEAK_HBM_GBS,
                 unit="GB/s", frac=round(ach_gbs / perf.PEAK_HBM_GBS, 4), traffic=traffic,
                 kernel=("rk4_lds_kernel" if variant == 1 else "rk4_interior_kernel") +
