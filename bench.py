@@ -77,7 +77,9 @@ def batching(steps, warmup):
     choice, table = lv.autotune()
     out = dict(workload=f"{len(cfg.patches)} uneven patches (multiples of 8 in [24,96], seed 1) in [-24,24]^3",
                interior_points=pts, steps=steps, tune=choice)
-    for name, mode in (("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED), ("graph", A.MODE_GRAPH)):
+    for name, mode in (("plain_direct", A.MODE_PLAIN), ("plain", A.MODE_PLAIN), ("batched", A.MODE_BATCHED),
+                       ("graph", A.MODE_GRAPH)):
+        lv.set_variant(0 if name == "plain_direct" else 1)
         assert lv.step(cfg.dt, warmup, mode) == 0
         t0 = time.perf_counter()
         assert lv.step(cfg.dt, steps, mode) == 0
@@ -86,6 +88,9 @@ def batching(steps, warmup):
         out[name + "_ms_per_step"] = round(dt * 1e3 / steps, 3)
     out["graph_vs_plain"] = round(out["graph_mpoints_s"] / out["plain_mpoints_s"], 3)
     out["batched_vs_plain"] = round(out["batched_mpoints_s"] / out["plain_mpoints_s"], 3)
+    out["graph_vs_plain_direct"] = round(out["graph_mpoints_s"] / out["plain_direct_mpoints_s"], 3)
+    out["note"] = ("plain = one launch per patch of the LDS kernel (variant 1); plain_direct = one launch per patch "
+                   "of the direct one-thread-per-point kernel (variant 0)")
     out["tune_table_ms"] = [[t[0], t[1], t[2], round(t[3], 4)] for t in table]
     lv.close()
     return out
@@ -115,6 +120,23 @@ def adjoint_leg(steps, checkpoint=20):
                 fd_dJ_dm1=fd, fd_rel_err_m1=abs(fd - g[0]) / abs(g[0]))
 
 
+def batched_adjoint_leg(steps):
+    """The 16-patch differentiation workload (uneven patches, multiples of 8 in
+    [48,128], seed 2; masses 0.55 / 0.45 at x = +-4): the batched forward
+    (hipGraph mode) with the checkpoint spacing planned from the free HBM, the
+    level adjoint (exchange reversed) backwards; dJ/d(masses, positions)."""
+    import amssbssn.gradient as G
+    cfg = G.config4(steps=steps)
+    J, g, info = G.value_and_grad(cfg, mode=2)
+    return dict(workload=f"16 uneven patches (multiples of 8 in [48,128], seed 2) in [-24,24]^3, {steps} RK4 steps; "
+                         "theta = (m1, m2, x1, x2)",
+                interior_points=cfg.interior_pts, J=J, dJ_dtheta=[float(x) for x in g[:4]],
+                checkpoint=info["checkpoint"], forward_s=round(info["forward_s"], 4),
+                backward_s=round(info["backward_s"], 4),
+                adjoint_over_forward=round(info["backward_s"] / info["forward_s"], 3),
+                peak_device_GB=round(info["peak_device_bytes"] / 1e9, 2) if info["peak_device_bytes"] else None)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +150,8 @@ def main():
                     help="timed steps of each mode on the 37-patch level (0 skips the batching comparison)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the 64^3 gradient run (0 skips the adjoint/forward measurement)")
+    ap.add_argument("--batched-adjoint-steps", type=int, default=20,
+                    help="steps of the 16-patch batched gradient run (0 skips it; the full workload is 400)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -199,6 +223,8 @@ def main():
         out["batching"] = batching(args.batch_steps, 2)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
+    if args.batched_adjoint_steps > 0:
+        out["batched_adjoint"] = batched_adjoint_leg(args.batched_adjoint_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
         out["cpu_baseline"]["gpu_speedup"] = round(value / out["cpu_baseline"]["value"], 1)

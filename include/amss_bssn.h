@@ -22,6 +22,7 @@
  */
 #ifndef AMSS_BSSN_H
 #define AMSS_BSSN_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -145,6 +146,10 @@ double *amss_level_field(amss_level *L, int patch, int list, int field);
 /* Copy a whole grid function in (dir 0) or out (dir 1); host_or_dev is any
  * pointer hipMemcpy accepts.  Synchronous. */
 int amss_level_copy(amss_level *L, int patch, int list, int field, double *host_or_dev, int dir);
+/* Copy the 24 grid functions of a list of one patch from (dir 0) or to
+ * (dir 1) 24 device arrays, ordered on the level's stream (no
+ * synchronisation; order other streams after amss_level_stream). */
+int amss_level_copy_list(amss_level *L, int patch, int list, double *const *ptrs, int dir);
 /* Advance nsteps RK4 steps of size dT (bssn_class::Step order).
  * mode 0: one launch per patch and sub-step; mode 1: batched work queue over
  * all patches; mode 2: batched, replayed from a captured hipGraph.
@@ -186,12 +191,17 @@ int amss_level_get_tune(amss_level *L, int *choice);
 
 /* ---- discrete adjoint of one RK4 step (GAUGE 0, vacuum, frozen ghosts) ----
  * No counterpart in the reference: the reverse of bssn_class::Step
- * (bssn_class.C:3042-3163, 3401-3526) for one Block with ghost width 3 and
- * frozen ghost zones, for gradients of a scalar of the evolved state.
- * All pointers are 24 device arrays of the Block's full shape ex, in the
- * reference field order; calls are ordered on `stream` (a hipStream_t, NULL
- * for the default stream) and do not synchronise.  The object holds the
- * stage states and work arrays (about 520 grid functions).
+ * (bssn_class.C:3042-3163, 3401-3526) with ghost width 3 and frozen ghost
+ * zones, for gradients of a scalar of the evolved state; a level of several
+ * patches includes the reverse of the inter-patch exchange (Parallel::Sync).
+ * amss_adjoint_create: one Block of shape ex and spacing h;
+ * amss_level_adjoint_create: npatch patches as in amss_level_create.
+ * Array arguments are 24 device arrays per patch (patch-major: patch p,
+ * field v at index 24 p + v) of the patch's full shape, in the reference
+ * field order; calls are ordered on `stream` (a hipStream_t, NULL for the
+ * default stream) and do not synchronise.  The object holds the stage
+ * states and work arrays (about 520 grid functions per patch;
+ * amss_adjoint_bytes).
  *  forward: U -> Unext (one RK4 step, the adjoint's own evaluation of it);
  *  step:    lam = dJ/dU^{n+1} in, dJ/dU^n out, U = the state at t^n;
  *  rhs:     out = dJ/dU of sum_p lam(p) . rhs(U)(p) over the points with
@@ -199,6 +209,9 @@ int amss_level_get_tune(amss_level *L, int *choice);
  *           compute_rhs_bssn_ there). */
 typedef struct amss_adjoint amss_adjoint;
 amss_adjoint *amss_adjoint_create(const int *ex, const double *h, double eps, double chitiny, int gauge);
+amss_adjoint *amss_level_adjoint_create(int npatch, const int *shapes, const double *bboxes, double eps,
+                                        double chitiny, int gauge);
+size_t amss_adjoint_bytes(amss_adjoint *A);
 void amss_adjoint_destroy(amss_adjoint *A);
 int amss_adjoint_forward(amss_adjoint *A, double dT, const double *const *U, double *const *Unext, void *stream);
 int amss_adjoint_step(amss_adjoint *A, double dT, const double *const *U, double *const *lam, void *stream);

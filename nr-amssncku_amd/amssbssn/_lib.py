@@ -40,11 +40,14 @@ SIGNATURES = {
     "amss_level_timing": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_set_variant": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_variant": ([C.c_void_p], C.c_int),
+    "amss_level_copy_list": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int], C.c_int),
     "amss_level_autotune": ([C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "amss_level_set_tune": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
     "amss_level_get_tune": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "amss_adjoint_create": ([_ip, _dp, C.c_double, C.c_double, C.c_int], C.c_void_p),
     "amss_adjoint_destroy": ([C.c_void_p], None),
+    "amss_level_adjoint_create": ([C.c_int, _ip, _dp, C.c_double, C.c_double, C.c_int], C.c_void_p),
+    "amss_adjoint_bytes": ([C.c_void_p], C.c_size_t),
     "amss_adjoint_forward": ([C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amss_adjoint_step": ([C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amss_adjoint_rhs": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),

@@ -24,6 +24,14 @@ def _ptrs(t):
     return (C.c_void_p * NF)(*[base + v * step for v in range(NF)])
 
 
+def _level_ptrs(ts):
+    """npatch x 24 device pointers (patch-major) of a list of per-patch tensors."""
+    out = []
+    for t in ts:
+        out.extend(_ptrs(t))
+    return (C.c_void_p * len(out))(*out)
+
+
 def _stream():
     import torch
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -63,3 +71,39 @@ class StepAdjoint:
     def rhs_vjp(self, U, lam, out) -> None:
         """out <- dJ/dU of sum_p lam(p).rhs(U)(p) over the interior points."""
         check(lib().amss_adjoint_rhs(self.h, _ptrs(U), _ptrs(lam), _ptrs(out), _stream()), "amss_adjoint_rhs")
+
+
+class LevelAdjoint:
+    """Forward step and its adjoint for a level of patches (PatchGeom list,
+    ghost width 3, frozen ghosts, GAUGE 0), including the reverse of the
+    inter-patch exchange.  States are lists of per-patch [24, nz, ny, nx]
+    device tensors."""
+
+    def __init__(self, patches, eps: float = 0.02, chitiny: float = 1e-5, gauge: int = 0):
+        shapes = (C.c_int * (3 * len(patches)))(*[s for p in patches for s in p.shape])
+        bbs = (C.c_double * (6 * len(patches)))(*[b for p in patches for b in p.bbox])
+        self.h = lib().amss_level_adjoint_create(len(patches), shapes, bbs, eps, chitiny, gauge)
+        if not self.h:
+            raise AmssError("amss_level_adjoint_create: " + (lib().amss_last_error() or b"").decode())
+        self.npatch = len(patches)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().amss_adjoint_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bytes(self) -> int:
+        return int(lib().amss_adjoint_bytes(self.h))
+
+    def forward(self, dt: float, U, Unext) -> None:
+        check(lib().amss_adjoint_forward(self.h, dt, _level_ptrs(U), _level_ptrs(Unext), _stream()),
+              "amss_adjoint_forward")
+
+    def step(self, dt: float, U, lam) -> None:
+        check(lib().amss_adjoint_step(self.h, dt, _level_ptrs(U), _level_ptrs(lam), _stream()), "amss_adjoint_step")

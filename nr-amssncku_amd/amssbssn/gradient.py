@@ -1,35 +1,38 @@
 """Gradients of a scalar observable of the evolved BSSN state with respect to
 initial-data parameters, through the hand-written step adjoint.
 
-Parameters theta = (m1, m2, x1, x2, c[16^3]): the two puncture masses, their
-x positions, and the coefficients of a cubic B-spline field added to A_xx.
-The forward run evolves the level's fused RK4 step and keeps a checkpoint
-every `checkpoint` steps; the backward pass recomputes each segment from its
-checkpoint and runs the step adjoint (libamssbssn.so) backwards through it.
-EvolveObservable is the torch.autograd.Function; the initial-data map and the
-observable are small elementwise torch expressions on the device, so torch's
-autograd chains dJ/dU0 to dJ/dtheta.
+Parameters theta = (m1, m2, x1, x2, c[n^3]): the two puncture masses, their
+x positions, and (n > 0) the coefficients of a cubic B-spline field added to
+A_xx.  The forward run evolves the level's fused RK4 step (one patch, or a
+batched level of uneven patches) and keeps a checkpoint every `checkpoint`
+steps; the backward pass rebuilds each segment from its checkpoint and runs
+the level's step adjoint (libamssbssn.so) backwards through it, exchange
+included.  EvolveObservable is the torch.autograd.Function; the initial-data
+map and the observable are small elementwise torch expressions on the
+device, so torch's autograd chains dJ/dU0 to dJ/dtheta.
 """
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
+import math
 import time
 from dataclasses import dataclass
 
 import numpy as np
 
-from .grid import AXX, CHI, LAP, NF, PatchGeom, cube_patch, philox_uniform
+from .grid import AXX, CHI, LAP, NF, PatchGeom, cube_patch, philox_uniform, uneven_level
 from .level import LIST_STATE, Level
-from .adjoint import StepAdjoint
+from .adjoint import LevelAdjoint, _ptrs
 from ._lib import check, lib
 
 
 @dataclass
 class DiffConfig:
-    """One patch with frozen ghost zones, two punctures, A_ij noise, and a
-    spline perturbation of A_xx; the observable is the Gaussian-shell
-    integral of alpha * chi."""
-    patch: PatchGeom
+    """Patches with frozen outer ghost zones, two punctures, A_ij noise, an
+    optional spline perturbation of A_xx; the observable is the
+    Gaussian-shell integral of alpha * chi over the owned points."""
+    patches: list
     dt: float
     steps: int = 200
     checkpoint: int = 20
@@ -37,12 +40,12 @@ class DiffConfig:
     chitiny: float = 1e-5
     masses: tuple = (0.5, 0.5)
     xs: tuple = (3.0, -3.0)
-    spline_n: int = 16
+    spline_n: int = 16           # 0: no spline parameters
     spline_sigma: float = 1e-6
     spline_seed: int = 3
     a_noise: float = 1e-8
     noise_seed: int = 0
-    shift0: tuple = (0.0, 0.0, 0.0)  # constant initial shift (0 in the benchmark)
+    shift0: tuple = (0.0, 0.0, 0.0)  # constant initial shift (0 in the benchmark configurations)
     shell_r: float = 6.0
     shell_width: float = 1.0
 
@@ -50,14 +53,41 @@ class DiffConfig:
         """Initial parameter vector: masses, x positions, spline coefficients
         N(0, sigma^2) from numpy's default_rng(spline_seed)."""
         rng = np.random.default_rng(self.spline_seed)
-        c = self.spline_sigma * rng.standard_normal(self.spline_n ** 3)
-        return np.concatenate([np.array(self.masses + self.xs, dtype=np.float64), c])
+        c = self.spline_sigma * rng.standard_normal(self.spline_n ** 3) if self.spline_n else np.zeros(0)
+        return np.concatenate([np.array(tuple(self.masses) + tuple(self.xs), dtype=np.float64), c])
+
+    @property
+    def interior_pts(self):
+        return sum(p.interior_pts for p in self.patches)
 
 
 def config3(n: int = 64, steps: int = 200, checkpoint: int = 20) -> DiffConfig:
-    """One n^3 patch (+3 ghosts) over [-16,16]^3 (h = 32/n), config-1 data."""
+    """One n^3 patch (+3 ghosts) over [-16,16]^3 (h = 32/n), config-1 data,
+    16^3 spline coefficients on A_xx."""
     g = cube_patch(n, 16.0)
-    return DiffConfig(g, dt=0.25 * g.h[0], steps=steps, checkpoint=checkpoint)
+    return DiffConfig([g], dt=0.25 * g.h[0], steps=steps, checkpoint=checkpoint)
+
+
+def config4(steps: int = 400, checkpoint: int = 0) -> DiffConfig:
+    """16 uneven patches (multiples of 8 in [48,128], seed 2) packed in
+    [-24,24]^3; masses 0.55 / 0.45 at x = +-4; gradient w.r.t. masses and
+    positions.  checkpoint 0: chosen by plan_checkpoints at run time."""
+    patches, _ = uneven_level(16, 48, 128, seed=2, half_width=24.0)
+    return DiffConfig(patches, dt=0.25 * patches[0].h[0], steps=steps, checkpoint=checkpoint,
+                      masses=(0.55, 0.45), xs=(4.0, -4.0), spline_n=0)
+
+
+def plan_checkpoints(steps: int, state_bytes: int, free_bytes: int, reserve: float = 0.1) -> int:
+    """Checkpoint spacing k for a run of `steps`: the backward pass holds
+    ceil(steps/k) checkpoints plus the k states of the segment it rebuilds.
+    Each segment rebuild costs k - 1 forward steps, so the recomputation is
+    steps - ceil(steps/k) forward steps in total: the smallest k whose states
+    fit in the free memory (less a reserve) recomputes least."""
+    budget = int(free_bytes * (1.0 - reserve)) // max(state_bytes, 1)
+    for k in range(1, steps + 1):
+        if math.ceil(steps / k) + k <= budget:
+            return k
+    raise MemoryError(f"{steps} steps of {state_bytes} B states do not fit in {free_bytes} B")
 
 
 # ---------------------------------------------------------------------------
@@ -67,118 +97,132 @@ def bspline3(t):
     return np.where(a < 1, (4.0 - 6.0 * a ** 2 + 3.0 * a ** 3) / 6.0, np.where(a < 2, (2.0 - a) ** 3 / 6.0, 0.0))
 
 
-def spline_basis(coord, n):
+def spline_basis(coord, n, lo=None, hi=None):
     """[len(coord), n] matrix of n uniform cubic B-splines whose centres span
-    the coordinate range end to end."""
-    lo, hi = coord[0], coord[-1]
+    [lo, hi] (default: the coordinate range) end to end."""
+    lo = coord[0] if lo is None else lo
+    hi = coord[-1] if hi is None else hi
     d = (hi - lo) / (n - 1)
     centres = lo + d * np.arange(n)
     return bspline3((coord[:, None] - centres[None, :]) / d)
 
 
 class InitialData:
-    """theta -> U0 as torch expressions on one device (differentiable)."""
+    """theta -> [U0 of each patch] as torch expressions on one device."""
 
     def __init__(self, cfg: DiffConfig, device="cuda"):
         import torch
         self.cfg = cfg
-        g = cfg.patch
-        X, Y, Z = g.coords()
-        z, y, x = np.meshgrid(Z, Y, X, indexing="ij")
         t = lambda a: torch.tensor(a, dtype=torch.float64, device=device)  # noqa: E731
-        self.x, self.y, self.z = t(x), t(y), t(z)
-        self.Bx = t(spline_basis(X, cfg.spline_n))
-        self.By = t(spline_basis(Y, cfg.spline_n))
-        self.Bz = t(spline_basis(Z, cfg.spline_n))
-        base = np.zeros((NF,) + x.shape)
-        if cfg.a_noise:
-            for s in range(6):
-                u = philox_uniform(g.npts, AXX + s, cfg.noise_seed).reshape(x.shape)
-                base[AXX + s] = cfg.a_noise * (2.0 * u - 1.0)
-        self.base = t(base)
-        self.device = device
+        # the spline spans the level's coordinate range (ghost cells included)
+        lo = [min(p.coords()[a][0] for p in cfg.patches) for a in range(3)]
+        hi = [max(p.coords()[a][-1] for p in cfg.patches) for a in range(3)]
+        self.parts = []
+        for g in cfg.patches:
+            X, Y, Z = g.coords()
+            z, y, x = np.meshgrid(Z, Y, X, indexing="ij")
+            part = dict(x=t(x), y=t(y), z=t(z))
+            if cfg.spline_n:
+                part["B"] = [t(spline_basis(c, cfg.spline_n, lo[a], hi[a])) for a, c in enumerate((X, Y, Z))]
+            base = np.zeros((NF,) + x.shape)
+            if cfg.a_noise:
+                for s in range(6):
+                    u = philox_uniform(g.npts, AXX + s, cfg.noise_seed).reshape(x.shape)
+                    base[AXX + s] = cfg.a_noise * (2.0 * u - 1.0)
+            for a in range(3):
+                base[18 + a] += cfg.shift0[a]
+            part["base"] = t(base)
+            self.parts.append(part)
 
     def __call__(self, theta):
         import torch
         n = self.cfg.spline_n
         m1, m2, x1, x2 = theta[0], theta[1], theta[2], theta[3]
-        r2 = self.y ** 2 + self.z ** 2 + 0.01
-        psi = 1.0 + m1 / (2.0 * torch.sqrt((self.x - x1) ** 2 + r2)) + m2 / (2.0 * torch.sqrt((self.x - x2) ** 2 + r2))
-        coef = theta[4:].reshape(n, n, n)  # [c_z, c_y, c_x]
-        axx = torch.einsum("cba,kc,jb,ia->kji", coef, self.Bz, self.By, self.Bx)
-        fields = [self.base[v] for v in range(NF)]
-        fields[CHI] = psi ** -4 - 1.0
-        fields[LAP] = psi ** -2 - 1.0
-        fields[AXX] = fields[AXX] + axx
-        for a in range(3):
-            if self.cfg.shift0[a]:
-                fields[18 + a] = fields[18 + a] + self.cfg.shift0[a]
-        return torch.stack(fields).contiguous()
+        out = []
+        for P in self.parts:
+            r2 = P["y"] ** 2 + P["z"] ** 2 + 0.01
+            psi = (1.0 + m1 / (2.0 * torch.sqrt((P["x"] - x1) ** 2 + r2)) +
+                   m2 / (2.0 * torch.sqrt((P["x"] - x2) ** 2 + r2)))
+            fields = [P["base"][v] for v in range(NF)]
+            fields[CHI] = psi ** -4 - 1.0
+            fields[LAP] = psi ** -2 - 1.0
+            if n:
+                Bx, By, Bz = P["B"]
+                coef = theta[4:].reshape(n, n, n)  # [c_z, c_y, c_x]
+                fields[AXX] = fields[AXX] + torch.einsum("cba,kc,jb,ia->kji", coef, Bz, By, Bx)
+            out.append(torch.stack(fields).contiguous())
+        return out
 
 
 class Observable:
-    """J = sum over owned points of (alpha)(chi) w(r) dV, w = exp(-(r - r0)^2 / (2 sigma^2))."""
+    """J = sum over owned points of alpha chi w(r) dV, w = exp(-(r - r0)^2 / (2 sigma^2))."""
 
     def __init__(self, cfg: DiffConfig, device="cuda"):
         import torch
-        g = cfg.patch
-        X, Y, Z = g.coords()
-        z, y, x = np.meshgrid(Z, Y, X, indexing="ij")
-        r = np.sqrt(x ** 2 + y ** 2 + z ** 2)
-        w = np.exp(-(r - cfg.shell_r) ** 2 / (2.0 * cfg.shell_width ** 2)) * float(np.prod(g.h))
-        mask = np.zeros_like(w)
-        gw = g.gw
-        mask[gw:-gw, gw:-gw, gw:-gw] = 1.0
-        self.w = torch.tensor(w * mask, dtype=torch.float64, device=device)
+        self.w = []
+        for g in cfg.patches:
+            X, Y, Z = g.coords()
+            z, y, x = np.meshgrid(Z, Y, X, indexing="ij")
+            r = np.sqrt(x ** 2 + y ** 2 + z ** 2)
+            w = np.exp(-(r - cfg.shell_r) ** 2 / (2.0 * cfg.shell_width ** 2)) * float(np.prod(g.h))
+            mask = np.zeros_like(w)
+            gw = g.gw
+            mask[gw:-gw, gw:-gw, gw:-gw] = 1.0
+            self.w.append(torch.tensor(w * mask, dtype=torch.float64, device=device))
 
-    def value(self, U):
-        import torch
-        return torch.sum((U[LAP] + 1.0) * (U[CHI] + 1.0) * self.w)
+    def value(self, Us):
+        return sum(((U[LAP] + 1.0) * (U[CHI] + 1.0) * w).sum() for U, w in zip(Us, self.w))
 
-    def grad(self, U):
-        """dJ/dU (24 fields)."""
+    def grad(self, Us):
+        """dJ/dU of each patch."""
         import torch
-        lam = torch.zeros_like(U)
-        lam[CHI] = (U[LAP] + 1.0) * self.w
-        lam[LAP] = (U[CHI] + 1.0) * self.w
-        return lam
+        out = []
+        for U, w in zip(Us, self.w):
+            lam = torch.zeros_like(U)
+            lam[CHI] = (U[LAP] + 1.0) * w
+            lam[LAP] = (U[CHI] + 1.0) * w
+            out.append(lam)
+        return out
 
 
 class Evolution:
-    """The forward evolution (fused level step) with checkpoints, and the
-    backward pass through the step adjoint."""
+    """The forward evolution (the level's fused step) with checkpoints, and
+    the backward pass through the level's step adjoint."""
 
     def __init__(self, cfg: DiffConfig, mode: int = 0):
         self.cfg = cfg
-        self.level = Level([cfg.patch], gauge=0, eps=cfg.eps, chitiny=cfg.chitiny)
-        self.adj = StepAdjoint(cfg.patch.shape, cfg.patch.h, eps=cfg.eps, chitiny=cfg.chitiny)
+        self.level = Level(cfg.patches, gauge=0, eps=cfg.eps, chitiny=cfg.chitiny)
+        self.adj = LevelAdjoint(cfg.patches, eps=cfg.eps, chitiny=cfg.chitiny)
         self.mode = mode
+        if mode > 0:
+            self.level.autotune()  # outside the timed forward (the first batched step would run it)
         self.timing = {}
+        self.min_free = None
 
     def close(self):
         self.level.close()
         self.adj.close()
 
-    def _put(self, U):
-        n = U[0].numel()
-        for v in range(NF):
-            check(lib().amss_level_copy(self.level.h, 0, LIST_STATE, v, C.c_void_p(U.data_ptr() + 8 * v * n), 0),
-                  "amss_level_copy(in)")
+    def _note_memory(self):
+        import torch
+        free, total = torch.cuda.mem_get_info()
+        self.total = total
+        self.min_free = free if self.min_free is None else min(self.min_free, free)
 
-    def _get(self, U):
-        n = U[0].numel()
-        for v in range(NF):
-            check(lib().amss_level_copy(self.level.h, 0, LIST_STATE, v, C.c_void_p(U.data_ptr() + 8 * v * n), 1),
-                  "amss_level_copy(out)")
+    def _copy(self, Us, direction):
+        """State of every patch in (0) or out (1), ordered on the level's stream."""
+        for p, U in enumerate(Us):
+            check(lib().amss_level_copy_list(self.level.h, p, LIST_STATE, _ptrs(U), direction), "amss_level_copy_list")
 
-    def forward(self, U0):
-        """Evolve U0 for cfg.steps steps; returns (final state, checkpoints)."""
+    def forward(self, U0s):
+        """Evolve for cfg.steps steps; returns (final states, checkpoints)."""
         import torch
         cfg = self.cfg
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ckpts = [U0.detach().clone()]
-        self._put(ckpts[0])
+        ckpts = [[U.detach().clone() for U in U0s]]
+        torch.cuda.synchronize()  # the level's stream reads what torch's stream wrote
+        self._copy(ckpts[0], 0)
         done = 0
         while done < cfg.steps:
             k = min(cfg.checkpoint, cfg.steps - done)
@@ -186,52 +230,54 @@ class Evolution:
             if rc:
                 raise RuntimeError(f"forward evolution failed (code {rc})")
             done += k
-            U = torch.empty_like(U0)
-            self._get(U)
-            ckpts.append(U)
+            Us = [torch.empty_like(U) for U in U0s]
+            self._copy(Us, 1)
+            ckpts.append(Us)
         torch.cuda.synchronize()
         self.timing["forward_s"] = time.perf_counter() - t0
+        self._note_memory()
         return ckpts[-1], ckpts[:-1]
 
-    def backward(self, lam, ckpts):
-        """lam = dJ/dU(T) -> dJ/dU0, recomputing each segment from its checkpoint."""
+    def backward(self, lams, ckpts):
+        """dJ/dU(T) of each patch -> dJ/dU0, rebuilding each segment from its checkpoint."""
         import torch
         cfg = self.cfg
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lam = lam.clone().contiguous()
-        nseg = len(ckpts)
-        for s in range(nseg - 1, -1, -1):
+        lams = [l.clone().contiguous() for l in lams]
+        for s in range(len(ckpts) - 1, -1, -1):
             k = min(cfg.checkpoint, cfg.steps - s * cfg.checkpoint)
             states = [ckpts[s]]
             for _ in range(k - 1):
-                nxt = torch.empty_like(states[-1])
+                nxt = [torch.empty_like(U) for U in states[-1]]
                 self.adj.forward(cfg.dt, states[-1], nxt)
                 states.append(nxt)
+            if s == len(ckpts) - 1:
+                self._note_memory()
             for n in range(k - 1, -1, -1):
-                self.adj.step(cfg.dt, states[n], lam)
+                self.adj.step(cfg.dt, states[n], lams)
         torch.cuda.synchronize()
         self.timing["backward_s"] = time.perf_counter() - t0
-        return lam
+        return lams
 
 
 def _autograd_function():
     import torch
 
     class EvolveObservable(torch.autograd.Function):
-        """J(U0) = observable(evolve(U0)); backward through the step adjoint."""
+        """J(U0 of every patch) = observable(evolve(U0)); backward through the step adjoint."""
 
         @staticmethod
-        def forward(ctx, U0, evo, obs):
-            UT, ckpts = evo.forward(U0)
+        def forward(ctx, evo, obs, *U0s):
+            UT, ckpts = evo.forward(list(U0s))
             ctx.evo, ctx.ckpts, ctx.lamT = evo, ckpts, obs.grad(UT)
             return obs.value(UT)
 
         @staticmethod
         def backward(ctx, gJ):
-            lam0 = ctx.evo.backward(ctx.lamT * gJ, ctx.ckpts)
+            lam0 = ctx.evo.backward([l * gJ for l in ctx.lamT], ctx.ckpts)
             ctx.ckpts = None
-            return lam0, None, None
+            return (None, None) + tuple(lam0)
 
     return EvolveObservable
 
@@ -239,17 +285,31 @@ def _autograd_function():
 _EVOLVE = None
 
 
-def evolve_observable(U0, evo: Evolution, obs: Observable):
-    """J as a differentiable function of U0 (a [24, nz, ny, nx] device tensor)."""
+def evolve_observable(U0s, evo: Evolution, obs: Observable):
+    """J as a differentiable function of the initial states (device tensors)."""
     global _EVOLVE
     if _EVOLVE is None:
         _EVOLVE = _autograd_function()
-    return _EVOLVE.apply(U0, evo, obs)
+    return _EVOLVE.apply(evo, obs, *U0s)
+
+
+def _with_checkpoint(cfg: DiffConfig, device) -> DiffConfig:
+    if cfg.checkpoint > 0:
+        return cfg
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    state = sum(NF * 8 * p.npts for p in cfg.patches)
+    # the level (127 grid functions) and the adjoint work arrays (~520) are
+    # allocated next to the checkpoints
+    work = sum((127 + 520) * 8 * p.npts for p in cfg.patches)
+    return dataclasses.replace(cfg, checkpoint=plan_checkpoints(cfg.steps, state, free - work))
 
 
 def value_and_grad(cfg: DiffConfig, theta=None, device="cuda", mode: int = 0):
-    """(J, dJ/dtheta, timings) for the configuration's parameters."""
+    """(J, dJ/dtheta, info) for the configuration's parameters; info holds the
+    forward / backward wall times, the checkpoint spacing and peak device memory."""
     import torch
+    cfg = _with_checkpoint(cfg, device)
     init = InitialData(cfg, device)
     obs = Observable(cfg, device)
     evo = Evolution(cfg, mode)
@@ -258,7 +318,9 @@ def value_and_grad(cfg: DiffConfig, theta=None, device="cuda", mode: int = 0):
     try:
         J = evolve_observable(init(th), evo, obs)
         J.backward()
-        return float(J.detach()), th.grad.detach().cpu().numpy(), dict(evo.timing)
+        info = dict(evo.timing, checkpoint=cfg.checkpoint,
+                    peak_device_bytes=int(evo.total - evo.min_free) if evo.min_free is not None else None)
+        return float(J.detach()), th.grad.detach().cpu().numpy(), info
     finally:
         evo.close()
 
@@ -266,17 +328,17 @@ def value_and_grad(cfg: DiffConfig, theta=None, device="cuda", mode: int = 0):
 def value(cfg: DiffConfig, theta, device="cuda", mode: int = 0) -> float:
     """J alone (no tape), for finite differences."""
     import torch
+    cfg = dataclasses.replace(cfg, checkpoint=cfg.checkpoint or cfg.steps)
     init = InitialData(cfg, device)
     obs = Observable(cfg, device)
     evo = Evolution(cfg, mode)
     try:
         with torch.no_grad():
-            U0 = init(torch.tensor(theta, dtype=torch.float64, device=device))
-            UT, _ = evo.forward(U0)
+            UT, _ = evo.forward(init(torch.tensor(theta, dtype=torch.float64, device=device)))
             return float(obs.value(UT))
     finally:
         evo.close()
 
 
-__all__ = ["DiffConfig", "config3", "InitialData", "Observable", "Evolution", "evolve_observable",
-           "value_and_grad", "value", "spline_basis", "bspline3"]
+__all__ = ["DiffConfig", "config3", "config4", "plan_checkpoints", "InitialData", "Observable", "Evolution",
+           "evolve_observable", "value_and_grad", "value", "spline_basis", "bspline3", "PatchGeom"]

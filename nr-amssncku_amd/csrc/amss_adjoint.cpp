@@ -1,5 +1,10 @@
-// amss_adjoint.cpp -- the step adjoint's workspace and stage sequencing.
+// amss_adjoint.cpp -- work arrays, stage sequencing and the inter-patch
+// exchange of the level step adjoint.
 #include "amss_adjoint.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
 
 #include "../../include/amss_bssn.h"
 #include "bssn_adjoint.h"
@@ -8,20 +13,20 @@
 namespace amss {
 
 namespace {
-// work arrays (in fields of N doubles)
-constexpr int kY0 = 0;                            // enforced U^n (24)
-constexpr int kY = kY0 + NFIELD;                  // Y1..Y3, the inputs of stages 1-3 (3 x 24)
-constexpr int kZ = kY + 3 * NFIELD;               // z0..z3, stage outputs before bound/enforce (4 x 24)
-constexpr int kK = kZ + 4 * NFIELD;               // k of the current stage (24)
-constexpr int kAcc = kK + NFIELD;                 // RK4 accumulator (24)
-constexpr int kBars = kAcc + NFIELD;              // feature bars
-constexpr int kLY = kBars + kAdjFeatures;         // dJ/dY of the stage being reversed (24)
-constexpr int kLY0 = kLY + NFIELD;                // dJ/dY0 (24)
-constexpr int kLK = kLY0 + NFIELD;                // dJ/dk0..k3 (4 x 24)
+// work arrays of a Block (in grid functions of N doubles)
+constexpr int kY0 = 0;                     // enforced U^n (24)
+constexpr int kY = kY0 + NFIELD;           // Y1..Y3, the inputs of sub-steps 1-3 (3 x 24)
+constexpr int kZ = kY + 3 * NFIELD;        // z0..z3, sub-step outputs before bound/enforce (4 x 24)
+constexpr int kK = kZ + 4 * NFIELD;        // k of the current sub-step (24)
+constexpr int kAcc = kK + NFIELD;          // RK4 accumulator (24)
+constexpr int kBars = kAcc + NFIELD;       // feature adjoints (kAdjFeatures)
+constexpr int kLY = kBars + kAdjFeatures;  // U^{n+1} scratch (forward); dJ/dY of the sub-step being reversed
+constexpr int kLY0 = kLY + NFIELD;         // dJ/dY0 (24)
+constexpr int kLK = kLY0 + NFIELD;         // dJ/dk0..k3 (4 x 24)
 constexpr int kFields = kLK + 4 * NFIELD;
 }  // namespace
 
-AdjointStep::AdjointStep(const int ex[3], const Coef& C, double chitiny) : C_(C), tinny_(chitiny - 1.0) {
+AdjointBlock::AdjointBlock(const int ex[3], const Coef& C, double tinny) : C_(C), tinny_(tinny) {
   for (int a = 0; a < 3; ++a) {
     if (ex[a] < 7) {
       err_ = "every axis needs at least 7 points";
@@ -37,25 +42,24 @@ AdjointStep::AdjointStep(const int ex[3], const Coef& C, double chitiny) : C_(C)
   const hipError_t e = hipMalloc(&buf_, (size_t)kFields * N_ * sizeof(double));
   if (e != hipSuccess) {
     buf_ = nullptr;
-    err_ = std::string("hipMalloc of the adjoint workspace: ") + hipGetErrorString(e);
+    err_ = std::string("hipMalloc of the adjoint work arrays: ") + hipGetErrorString(e);
   }
 }
 
-AdjointStep::~AdjointStep() {
+AdjointBlock::~AdjointBlock() {
   if (buf_) (void)hipFree(buf_);
 }
 
-FieldsIn AdjointStep::fin(int first) const {
+FieldsIn AdjointBlock::fin(int first) const {
   FieldsIn f;
   for (int v = 0; v < NFIELD; ++v) f.p[v] = buf_ + (long)(first + v) * N_;
   return f;
 }
-FieldsOut AdjointStep::fout(int first) const {
+FieldsOut AdjointBlock::fout(int first) const {
   FieldsOut f;
   for (int v = 0; v < NFIELD; ++v) f.p[v] = buf_ + (long)(first + v) * N_;
   return f;
 }
-
 static FieldsIn as_in(const double* const* p) {
   FieldsIn f;
   for (int v = 0; v < NFIELD; ++v) f.p[v] = p[v];
@@ -67,50 +71,239 @@ static FieldsOut as_out(double* const* p) {
   return f;
 }
 
-int AdjointStep::forward(double dT, const double* const* U, double* const* Unext, hipStream_t s) {
-  const double c[4] = {0.5 * dT, 0.5 * dT, dT, dT / 6.0};
+PatchDesc AdjointBlock::desc(int which) const {
+  PatchDesc d;
+  d.slab = buf_ + (long)which * N_;
+  d.n = N_;
+  d.sy = box_.n[0];
+  d.sz = box_.n[0] * box_.n[1];
+  for (int a = 0; a < 3; ++a) d.shape[a] = box_.n[a];
+  return d;
+}
+double* AdjointBlock::stage_output(int st) const { return buf_ + (long)(st < 3 ? kY + st * NFIELD : kLY) * N_; }
+double* AdjointBlock::lY() const { return buf_ + (long)kLY * N_; }
+
+void AdjointBlock::fwd_open(const double* const* U, hipStream_t s) {
   adj_launch_copy_enforce(as_in(U), fout(kY0), N_, s);
-  for (int st = 0; st < 4; ++st) {
-    const FieldsIn in = st == 0 ? fin(kY0) : fin(kY + (st - 1) * NFIELD);
-    adj_launch_rhs_k(in, fout(kK), box_, C_, s);
-    const FieldsOut yn = st < 3 ? fout(kY + st * NFIELD) : (Unext ? as_out(Unext) : fout(kLY));
-    adj_launch_fwd_stage(fin(kY0), fin(kK), fout(kAcc), fout(kZ + st * NFIELD), yn, box_, st, c[st], tinny_, s);
-  }
-  if (hipGetLastError() != hipSuccess) {
-    err_ = "adjoint forward: kernel launch failed";
-    return AMSS_ERR_HIP;
-  }
-  return 0;
 }
 
-int AdjointStep::backward(double dT, const double* const* U, double* const* lam, hipStream_t s) {
-  if (int r = forward(dT, U, nullptr, s)) return r;
-  const double c[4] = {0.5 * dT, 0.5 * dT, dT, dT / 6.0};
+void AdjointBlock::fwd_stage(int st, double c, double* const* Unext, hipStream_t s) {
+  const FieldsIn in = st == 0 ? fin(kY0) : fin(kY + (st - 1) * NFIELD);
+  adj_launch_rhs_k(in, fout(kK), box_, C_, s);
+  const FieldsOut yn = st < 3 ? fout(kY + st * NFIELD) : (Unext ? as_out(Unext) : fout(kLY));
+  adj_launch_fwd_stage(fin(kY0), fin(kK), fout(kAcc), fout(kZ + st * NFIELD), yn, box_, st, c, tinny_, s);
+}
+
+void AdjointBlock::rev_final(const double* const* lam, double c, hipStream_t s) {
   const FieldsOut lk[4] = {fout(kLK), fout(kLK + NFIELD), fout(kLK + 2 * NFIELD), fout(kLK + 3 * NFIELD)};
-  adj_launch_final(as_in(lam), fin(kZ + 3 * NFIELD), fin(kY0), fout(kLY0), lk, box_, c[3], tinny_, s);
-  double* bars = buf_ + (long)kBars * N_;
-  for (int st = 3; st >= 1; --st) {
-    const FieldsIn Ys = fin(kY + (st - 1) * NFIELD);
-    const FieldsIn lks = fin(kLK + st * NFIELD);
-    adj_launch_point(Ys, lks, bars, box_, C_, s);
-    adj_launch_gather(Ys, lks, bars, fout(kLY), box_, C_, false, s);
-    adj_launch_stage(fin(kLY), fin(kZ + (st - 1) * NFIELD), fin(kY0), fout(kLY0), lk[st - 1], box_, c[st - 1],
-                     tinny_, s);
-  }
-  adj_launch_point(fin(kY0), fin(kLK), bars, box_, C_, s);
-  adj_launch_gather(fin(kY0), fin(kLK), bars, fout(kLY0), box_, C_, true, s);
-  adj_launch_open(as_in(U), fin(kLY0), as_out(lam), N_, s);
-  if (hipGetLastError() != hipSuccess) {
-    err_ = "adjoint backward: kernel launch failed";
-    return AMSS_ERR_HIP;
-  }
-  return 0;
+  adj_launch_final(as_in(lam), fin(kZ + 3 * NFIELD), fin(kY0), fout(kLY0), lk, box_, c, tinny_, s);
 }
 
-int AdjointStep::rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s) {
+void AdjointBlock::rev_rhs(int st, hipStream_t s) {
+  double* bars = buf_ + (long)kBars * N_;
+  const FieldsIn Ys = st == 0 ? fin(kY0) : fin(kY + (st - 1) * NFIELD);
+  const FieldsIn lks = fin(kLK + st * NFIELD);
+  adj_launch_point(Ys, lks, bars, box_, C_, s);
+  if (st == 0)
+    adj_launch_gather(Ys, lks, bars, fout(kLY0), box_, C_, true, s);
+  else
+    adj_launch_gather(Ys, lks, bars, fout(kLY), box_, C_, false, s);
+}
+
+void AdjointBlock::rev_stage(int st, double c, hipStream_t s) {
+  adj_launch_stage(fin(kLY), fin(kZ + (st - 1) * NFIELD), fin(kY0), fout(kLY0), fout(kLK + (st - 1) * NFIELD),
+                   box_, c, tinny_, s);
+}
+
+void AdjointBlock::rev_open(const double* const* U, double* const* lam, hipStream_t s) {
+  adj_launch_open(as_in(U), fin(kLY0), as_out(lam), N_, s);
+}
+
+void AdjointBlock::rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s) {
   double* bars = buf_ + (long)kBars * N_;
   adj_launch_point(as_in(U), as_in(lam), bars, box_, C_, s);
   adj_launch_gather(as_in(U), as_in(lam), bars, as_out(out), box_, C_, false, s);
+}
+
+// ---------------------------------------------------------------------------
+static Coef coef_of(const double h[3], double eps) {
+  Coef C;
+  for (int a = 0; a < 3; ++a) {
+    C.h[a] = h[a];
+    C.d1[a] = 1.0 / 12.0 / h[a];
+    C.d1lo[a] = 1.0 / 2.0 / h[a];
+    C.d2[a] = (1.0 / 12.0) / (h[a] * h[a]);
+    C.d2lo[a] = 1.0 / (h[a] * h[a]);
+    C.ko[a] = (eps > 0 ? eps : 0.0) / 64.0 / h[a];
+  }
+  const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+  for (int q = 0; q < 3; ++q) {
+    C.dm[q] = (1.0 / 144.0) / (h[pa[q]] * h[pb[q]]);
+    C.dmlo[q] = 0.25 / (h[pa[q]] * h[pb[q]]);
+  }
+  return C;
+}
+
+LevelAdjoint::LevelAdjoint(int npatch, const int* shapes, const double* bboxes, double eps, double chitiny,
+                           const double* hgiven) {
+  const int gw = 3;
+  std::vector<std::array<double, 3>> h(npatch);
+  for (int p = 0; p < npatch; ++p)
+    for (int a = 0; a < 3; ++a) {
+      h[p][a] = hgiven ? hgiven[a] : (bboxes[6 * p + 3 + a] - bboxes[6 * p + a]) / shapes[3 * p + a];
+      if (!(h[p][a] > 0.0)) {
+        err_ = "bad patch box";
+        return;
+      }
+      if (std::fabs(h[p][a] - h[0][a]) > 1e-12 * h[0][a]) {
+        err_ = "all patches of a level must share one grid spacing";
+        return;
+      }
+    }
+  for (int p = 0; p < npatch; ++p) {
+    auto* b = new AdjointBlock(shapes + 3 * p, coef_of(h[p].data(), eps), chitiny - 1.0);
+    blocks_.push_back(b);
+    if (!b->ok()) {
+      err_ = b->err();
+      return;
+    }
+  }
+  // exchange boxes: the ghost points of patch A inside the interior of patch B
+  double gmin[3] = {1e300, 1e300, 1e300};
+  for (int p = 0; p < npatch; ++p)
+    for (int a = 0; a < 3; ++a) gmin[a] = std::min(gmin[a], bboxes[6 * p + a]);
+  std::vector<std::array<int, 3>> org(npatch);
+  for (int p = 0; p < npatch; ++p)
+    for (int a = 0; a < 3; ++a) org[p][a] = (int)std::lround((bboxes[6 * p + a] - gmin[a]) / h[0][a]);
+  for (int A = 0; A < npatch; ++A)
+    for (int B = 0; B < npatch; ++B) {
+      if (A == B) continue;
+      SegRef r;
+      bool empty = false;
+      for (int a = 0; a < 3; ++a) {
+        const int alo = org[A][a], ahi = org[A][a] + shapes[3 * A + a];
+        const int blo = org[B][a] + gw, bhi = org[B][a] + shapes[3 * B + a] - gw;
+        const int lo = std::max(alo, blo), hi = std::min(ahi, bhi);
+        if (hi <= lo) {
+          empty = true;
+          break;
+        }
+        r.dlo[a] = lo - org[A][a];
+        r.slo[a] = lo - org[B][a];
+        r.ext[a] = hi - lo;
+      }
+      if (empty) continue;
+      r.src = B;
+      r.dst = A;
+      hseg_.push_back(r);
+      seg_max_ = std::max(seg_max_, (long)r.ext[0] * r.ext[1] * r.ext[2]);
+    }
+  if (hseg_.size() > 65535) {
+    err_ = "too many exchange boxes";
+    return;
+  }
+  auto upload = [&](const void* src, size_t bytes, void** dst) -> bool {
+    if (hipMalloc(dst, bytes) != hipSuccess || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      err_ = "device tables of the level adjoint";
+      return false;
+    }
+    return true;
+  };
+  if (!hseg_.empty() && !upload(hseg_.data(), hseg_.size() * sizeof(SegRef), (void**)&dseg_)) return;
+  // PatchDesc tables of the exchanged runs: Y1, Y2, Y3 and the scratch / dJ/dY run
+  for (int which : {kY, kY + NFIELD, kY + 2 * NFIELD, kLY}) {
+    std::vector<PatchDesc> t;
+    for (AdjointBlock* b : blocks_) t.push_back(b->desc(which));
+    PatchDesc* d = nullptr;
+    if (!upload(t.data(), t.size() * sizeof(PatchDesc), (void**)&d)) return;
+    dtab_.push_back(d);
+  }
+}
+
+LevelAdjoint::~LevelAdjoint() {
+  (void)hipDeviceSynchronize();
+  for (AdjointBlock* b : blocks_) delete b;
+  if (dseg_) (void)hipFree(dseg_);
+  for (PatchDesc* d : dtab_) (void)hipFree(d);
+}
+
+size_t LevelAdjoint::bytes() const {
+  size_t n = 0;
+  for (AdjointBlock* b : blocks_) n += (size_t)kFields * b->npts() * sizeof(double);
+  return n;
+}
+
+void LevelAdjoint::exchange(int table, hipStream_t s) {
+  if (!hseg_.empty()) launch_seg_boxes(dtab_[table], dseg_, (int)hseg_.size(), (int)seg_max_, 0, s);
+}
+void LevelAdjoint::exchange_adjoint(int table, hipStream_t s) {
+  if (!hseg_.empty()) launch_seg_boxes_adjoint(dtab_[table], dseg_, (int)hseg_.size(), (int)seg_max_, s);
+}
+
+int LevelAdjoint::forward(double dT, const double* const* U, double* const* Unext, hipStream_t s) {
+  const double c[4] = {0.5 * dT, 0.5 * dT, dT, dT / 6.0};
+  const int np = npatch();
+  for (int p = 0; p < np; ++p) blocks_[p]->fwd_open(U + p * NFIELD, s);
+  for (int st = 0; st < 4; ++st) {
+    for (int p = 0; p < np; ++p) blocks_[p]->fwd_stage(st, c[st], nullptr, s);
+    exchange(st, s);  // tables 0..2 = Y1..Y3, 3 = the U^{n+1} scratch
+  }
+  if (Unext)
+    for (int p = 0; p < np; ++p) {
+      const long N = blocks_[p]->npts();
+      for (int v = 0; v < NFIELD; ++v)
+        if (hipMemcpyAsync(Unext[p * NFIELD + v], blocks_[p]->stage_output(3) + v * N, N * sizeof(double),
+                           hipMemcpyDeviceToDevice, s) != hipSuccess) {
+          err_ = "copy of the step's output";
+          return AMSS_ERR_HIP;
+        }
+    }
+  if (hipGetLastError() != hipSuccess) {
+    err_ = "level adjoint forward: kernel launch failed";
+    return AMSS_ERR_HIP;
+  }
+  return 0;
+}
+
+int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam, hipStream_t s) {
+  if (int r = forward(dT, U, nullptr, s)) return r;
+  const double c[4] = {0.5 * dT, 0.5 * dT, dT, dT / 6.0};
+  const int np = npatch();
+  // dJ/dU^{n+1} into each Block's dJ/dY run, then the final exchange reversed
+  for (int p = 0; p < np; ++p) {
+    const long N = blocks_[p]->npts();
+    for (int v = 0; v < NFIELD; ++v)
+      if (hipMemcpyAsync(blocks_[p]->lY() + v * N, lam[p * NFIELD + v], N * sizeof(double), hipMemcpyDeviceToDevice,
+                         s) != hipSuccess) {
+        err_ = "copy of dJ/dU";
+        return AMSS_ERR_HIP;
+      }
+  }
+  exchange_adjoint(3, s);
+  for (int p = 0; p < np; ++p) {
+    double* ly = blocks_[p]->lY();
+    const long N = blocks_[p]->npts();
+    const double* ptr[NFIELD];
+    for (int v = 0; v < NFIELD; ++v) ptr[v] = ly + v * N;
+    blocks_[p]->rev_final(ptr, c[3], s);
+  }
+  for (int st = 3; st >= 1; --st) {
+    for (int p = 0; p < np; ++p) blocks_[p]->rev_rhs(st, s);  // dJ/dY_st into the dJ/dY run
+    exchange_adjoint(3, s);  // Y_st was exchanged after sub-step st-1
+    for (int p = 0; p < np; ++p) blocks_[p]->rev_stage(st, c[st - 1], s);
+  }
+  for (int p = 0; p < np; ++p) blocks_[p]->rev_rhs(0, s);
+  for (int p = 0; p < np; ++p) blocks_[p]->rev_open(U + p * NFIELD, lam + p * NFIELD, s);
+  if (hipGetLastError() != hipSuccess) {
+    err_ = "level adjoint backward: kernel launch failed";
+    return AMSS_ERR_HIP;
+  }
+  return 0;
+}
+
+int LevelAdjoint::rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s) {
+  for (int p = 0; p < npatch(); ++p)
+    blocks_[p]->rhs_vjp(U + p * NFIELD, lam + p * NFIELD, out + p * NFIELD, s);
   if (hipGetLastError() != hipSuccess) {
     err_ = "adjoint rhs: kernel launch failed";
     return AMSS_ERR_HIP;

@@ -440,6 +440,21 @@ int amss_level_copy(amss_level* L, int patch, int list, int field, double* ptr, 
   return 0;
 }
 
+int amss_level_copy_list(amss_level* L, int patch, int list, double* const* ptrs, int dir) {
+  if (!L || !ptrs) return fail(AMSS_ERR_ARG, "amss_level_copy_list: null argument");
+  if (list < AMSS_LIST_STATE || list > AMSS_LIST_RHS) return fail(AMSS_ERR_ARG, "amss_level_copy_list: bad list");
+  const size_t n = (size_t)L->L->npts(patch);
+  for (int v = 0; v < 24; ++v) {
+    double* f = L->L->field(patch, list, v);
+    if (!f || !ptrs[v]) return fail(AMSS_ERR_ARG, "amss_level_copy_list: no such grid function");
+    if (dir == 0)
+      HIPCHK(hipMemcpyAsync(f, ptrs[v], n * sizeof(double), hipMemcpyDeviceToDevice, L->L->stream()));
+    else
+      HIPCHK(hipMemcpyAsync(ptrs[v], f, n * sizeof(double), hipMemcpyDeviceToDevice, L->L->stream()));
+  }
+  return 0;
+}
+
 int amss_level_step(amss_level* L, double dT, int nsteps, int mode, int want_ham) {
   if (!L) return fail(AMSS_ERR_ARG, "amss_level_step: null level");
   const int r = L->L->step(dT, nsteps, mode, want_ham != 0);
@@ -502,7 +517,7 @@ int amss_level_kernel_ms(amss_level* L, double* ms4, long* n4) {
 
 // ---- step adjoint ----------------------------------------------------------
 struct amss_adjoint {
-  AdjointStep* A;
+  LevelAdjoint* A;
 };
 
 amss_adjoint* amss_adjoint_create(const int* ex, const double* h, double eps, double chitiny, int gauge) {
@@ -519,7 +534,27 @@ amss_adjoint* amss_adjoint_create(const int* ex, const double* h, double eps, do
       fail(AMSS_ERR_ARG, "amss_adjoint_create: grid spacing must be positive");
       return nullptr;
     }
-  auto* A = new AdjointStep(ex, make_coef(h, eps > 0 ? eps : 0.0), chitiny);
+  const double bb[6] = {0.0, 0.0, 0.0, ex[0] * h[0], ex[1] * h[1], ex[2] * h[2]};
+  auto* A = new LevelAdjoint(1, ex, bb, eps, chitiny, h);
+  if (!A->ok()) {
+    fail(AMSS_ERR_ARG, A->err());
+    delete A;
+    return nullptr;
+  }
+  return new amss_adjoint{A};
+}
+
+amss_adjoint* amss_level_adjoint_create(int npatch, const int* shapes, const double* bboxes, double eps,
+                                        double chitiny, int gauge) {
+  if (npatch < 1 || !shapes || !bboxes) {
+    fail(AMSS_ERR_ARG, "amss_level_adjoint_create: bad arguments");
+    return nullptr;
+  }
+  if (gauge != 0) {
+    fail(AMSS_ERR_UNSUPPORTED, "amss_level_adjoint_create: the adjoint covers GAUGE 0 only");
+    return nullptr;
+  }
+  auto* A = new LevelAdjoint(npatch, shapes, bboxes, eps, chitiny);
   if (!A->ok()) {
     fail(AMSS_ERR_ARG, A->err());
     delete A;
@@ -530,7 +565,6 @@ amss_adjoint* amss_adjoint_create(const int* ex, const double* h, double eps, do
 
 void amss_adjoint_destroy(amss_adjoint* A) {
   if (!A) return;
-  (void)hipDeviceSynchronize();
   delete A->A;
   delete A;
 }
@@ -553,5 +587,7 @@ int amss_adjoint_rhs(amss_adjoint* A, const double* const* U, const double* cons
   const int r = A->A->rhs_vjp(U, lam, out, (hipStream_t)stream);
   return r ? fail(r, A->A->err()) : 0;
 }
+
+size_t amss_adjoint_bytes(amss_adjoint* A) { return A ? A->A->bytes() : 0; }
 
 }  // extern "C"

@@ -83,6 +83,10 @@ void launch_fill_boxes(const PatchDesc* P, const FillBox* B, int nbox, int maxpt
                        double tinny, int* flag, hipStream_t s);
 // Inter-patch ghost exchange of list slot `slot`.
 void launch_seg_boxes(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, int slot, hipStream_t s);
+// Reverse of launch_seg_boxes (slot 0 of P): each ghost-box value's adjoint is
+// added to the source point it was copied from (atomically: a source point
+// can feed the ghosts of several patches) and the ghost value's adjoint zeroed.
+void launch_seg_boxes_adjoint(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, hipStream_t s);
 // enforce_ga on every point of list slot `slot` of every patch.
 void launch_enforce_patches(const PatchDesc* P, int npatch, long maxn, int slot, hipStream_t s);
 

@@ -70,6 +70,26 @@ __global__ void __launch_bounds__(256) seg_boxes_kernel(const PatchDesc* __restr
   }
 }
 
+__global__ void __launch_bounds__(256) seg_boxes_adjoint_kernel(const PatchDesc* __restrict__ P,
+                                                                const SegRef* __restrict__ S) {
+  const SegRef g = S[blockIdx.y];
+  const PatchDesc a = P[g.src], b = P[g.dst];
+  const long npts = (long)g.ext[0] * g.ext[1] * g.ext[2];
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < npts; t += gridDim.x * 256L) {
+    const int x = (int)(t % g.ext[0]), y = (int)((t / g.ext[0]) % g.ext[1]);
+    const int z = (int)(t / ((long)g.ext[0] * g.ext[1]));
+    const long ps = (g.slo[0] + x) + (long)(g.slo[1] + y) * a.sy + (long)(g.slo[2] + z) * a.sz;
+    const long pd = (g.dlo[0] + x) + (long)(g.dlo[1] + y) * b.sy + (long)(g.dlo[2] + z) * b.sz;
+#pragma unroll
+    for (int v = 0; v < NFIELD; ++v) {
+      double* d = b.slab + v * b.n + pd;
+      const double l = *d;
+      *d = 0.0;
+      atomicAdd(a.slab + v * a.n + ps, l);
+    }
+  }
+}
+
 // enforce_ga (enforce_algebra.f90:72-127) on every point of every patch.
 __global__ void __launch_bounds__(256) enforce_patches_kernel(const PatchDesc* __restrict__ P, int slot) {
   const PatchDesc d = P[blockIdx.y];
@@ -113,6 +133,10 @@ void launch_fill_boxes(const PatchDesc* P, const FillBox* B, int nbox, int maxpt
 
 void launch_seg_boxes(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, int slot, hipStream_t s) {
   if (nseg > 0) seg_boxes_kernel<<<dim3(xblocks(maxpts), nseg), 256, 0, s>>>(P, S, slot);
+}
+
+void launch_seg_boxes_adjoint(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, hipStream_t s) {
+  if (nseg > 0) seg_boxes_adjoint_kernel<<<dim3(xblocks(maxpts), nseg), 256, 0, s>>>(P, S);
 }
 
 void launch_enforce_patches(const PatchDesc* P, int npatch, long maxn, int slot, hipStream_t s) {

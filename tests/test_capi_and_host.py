@@ -86,3 +86,15 @@ def test_philox_noise_amplitude_and_determinism():
         assert np.array_equal(a[v], b[v])
         assert 0.9e-8 < np.max(np.abs(a[v])) < 1e-8
     assert not np.array_equal(a[8], a[9])
+
+
+def test_checkpoint_plan_fits_memory_and_minimises_recompute():
+    import amssbssn.gradient as G
+    # 400 steps of 2.5 GB states in 288 GB: 0.9 * 288 / 2.5 = 103 states fit;
+    # k = 4 needs 100 + 4 > 103, k = 5 needs 80 + 5
+    assert G.plan_checkpoints(400, 2_500_000_000, 288_000_000_000) == 5
+    # everything fits: a checkpoint every step, nothing recomputed
+    assert G.plan_checkpoints(200, 66_000_000, 280_000_000_000) == 1
+    import pytest
+    with pytest.raises(MemoryError):
+        G.plan_checkpoints(400, 2_500_000_000, 10_000_000_000)

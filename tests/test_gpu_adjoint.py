@@ -149,3 +149,65 @@ def test_step_adjoint_matches_torch_autograd_of_the_step():
         worst = max(worst, e)
         assert e <= 1e-11, (v, e)
     print(f"step adjoint vs torch autograd: worst {worst:.2e}")
+
+
+def test_level_adjoint_split_equals_whole():
+    """A 20^3 cube and the same cube as two 10 x 20 x 20 patches: the split
+    forward step reproduces the whole interior, and for J = <w, U^{n+2}>
+    the split adjoint (exchange reversed) gives the same directional
+    derivative sum_p <dJ/dU0_p, e|p> as the whole cube's <dJ/dU0, e>."""
+    import torch
+    import amssbssn as A
+    from amssbssn.adjoint import LevelAdjoint
+    n, gw = 20, 3
+    whole = A.cube_patch(n, 8.0)
+    h = whole.h[0]
+    lo = -8.0 - gw * h
+    left = A.PatchGeom((n // 2 + 2 * gw, n + 2 * gw, n + 2 * gw),
+                       (lo, lo, lo, 0.0 + gw * h, 8.0 + gw * h, 8.0 + gw * h), gw)
+    right = A.PatchGeom((n // 2 + 2 * gw, n + 2 * gw, n + 2 * gw),
+                        (0.0 - gw * h, lo, lo, 8.0 + gw * h, 8.0 + gw * h, 8.0 + gw * h), gw)
+    cfg = A.Config("whole", [whole], dt=0.25 * h, steps=2, punctures=[(0.5, 1.0, 0.5, 0.0), (0.5, -1.5, 0.0, 0.3)],
+                   a_noise=1e-6)
+    rng = np.random.default_rng(12)
+    U = _dev(cfg.initial_state(0))
+    U = U + 1e-3 * torch.tensor(rng.standard_normal(U.shape), device="cuda")
+    w = torch.zeros_like(U)
+    w[:, gw:-gw, gw:-gw, gw:-gw] = torch.tensor(rng.standard_normal((24, n, n, n)), device="cuda")
+    e = torch.tensor(rng.standard_normal(U.shape), device="cuda")
+    cut = [lambda t: t[..., : n // 2 + 2 * gw].contiguous(), lambda t: t[..., n // 2:].contiguous()]
+    Lw = LevelAdjoint([whole], eps=cfg.eps)
+    Ls = LevelAdjoint([left, right], eps=cfg.eps)
+    # forward: two steps each
+    tw = [[U]]
+    ts = [[c(U) for c in cut]]
+    for _ in range(2):
+        nw = [torch.empty_like(U)]
+        Lw.forward(cfg.dt, tw[-1], nw)
+        tw.append(nw)
+        ns = [torch.empty_like(x) for x in ts[-1]]
+        Ls.forward(cfg.dt, ts[-1], ns)
+        ts.append(ns)
+    W = tw[-1][0][:, gw:-gw, gw:-gw, gw:-gw]
+    a = ts[-1][0][:, gw:-gw, gw:-gw, gw:-gw]
+    b = ts[-1][1][:, gw:-gw, gw:-gw, gw:-gw]
+    m = float(W.abs().max())
+    assert float((W[..., : n // 2] - a).abs().max()) <= 1e-13 * m
+    assert float((W[..., n // 2:] - b).abs().max()) <= 1e-13 * m
+    # adjoint
+    lw = [w.clone()]
+    wl = torch.zeros_like(ts[0][0])
+    wr = torch.zeros_like(ts[0][1])
+    wl[:, gw:-gw, gw:-gw, gw:-gw] = w[:, gw:-gw, gw:-gw, gw:gw + n // 2]
+    wr[:, gw:-gw, gw:-gw, gw:-gw] = w[:, gw:-gw, gw:-gw, gw + n // 2:-gw]
+    ls = [wl, wr]
+    for k in (1, 0):
+        Lw.step(cfg.dt, tw[k], lw)
+        Ls.step(cfg.dt, ts[k], ls)
+    torch.cuda.synchronize()
+    dw = float(torch.sum(lw[0] * e))
+    ds = sum(float(torch.sum(l * c(e))) for l, c in zip(ls, cut))
+    Lw.close()
+    Ls.close()
+    print(f"whole {dw:.12e} split {ds:.12e} rel {abs(ds - dw) / abs(dw):.2e}")
+    assert abs(ds - dw) <= 1e-11 * abs(dw)
