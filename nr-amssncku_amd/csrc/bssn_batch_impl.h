@@ -9,15 +9,15 @@ namespace amss {
 // Persistent workgroups stride over the tile list; each tile runs the same
 // LDS-staged point computation as rk4_lds_kernel, with the fields of its
 // patch addressed through the slab and the role slots.  The LDS buffers need
-// no extra barrier between tiles: the next tile's prologue fills buffer 0,
-// last read by group 22, which every wave finished before the barrier that
-// opened group 23.
+// no extra barrier between tiles: the next tile's prologue fills pair 0, last
+// read by groups 20 and 21, which every wave finished before the barrier that
+// opened groups 22 and 23.
 template <int GAUGE, int STAGE, bool CO, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(2, 2)))
 rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T, int t0, int ntiles, Roles m,
                  Coef C, Rk4Param R, int remap, int* flag) {
   using Tl = LdsTile<TX, TY, TZ>;
-  __shared__ __attribute__((aligned(16))) double tiles[2 * Tl::NPAD];
+  __shared__ __attribute__((aligned(16))) double tiles[4 * Tl::NPAD];
   lds_double* b0 = (lds_double*)tiles;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     // keep the compiler from hoisting per-thread LDS addresses out of the
@@ -33,13 +33,13 @@ rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T,
     const SlabFields in{d.slab + m.in * ln, d.n}, u0{d.slab + m.u0 * ln, d.n};
     const SlabFields acc{d.slab + m.acc * ln, d.n}, out{d.slab + m.out * ln, d.n};
     const SlabAux aux{d.slab + kResidualBase * d.n, d.n};
-    if (bad_input(in, p)) *flag = 1;
-    LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + Tl::NPAD, p, ltv,
+    LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + 2 * Tl::NPAD, p, ltv,
                                      (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz, d.sy, d.sz, (int)d.n, tid};
     E.prologue();
     Rk4SinkT<STAGE, SlabFields, SlabFields, SlabAux> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
     bssn_rhs_point<GAUGE, CO, false, false>(E, S);
     S.finish();
+    if (E.bad) *flag = 1;
   }
 }
 

@@ -85,7 +85,7 @@ template <int GAUGE, int STAGE, bool CO>
 AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
                                  Coef C, Rk4Param R, int* flag) {
   using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
-  __shared__ __attribute__((aligned(16))) double tiles[2 * Tl::NPAD];
+  __shared__ __attribute__((aligned(16))) double tiles[4 * Tl::NPAD];
   const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
   const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
   const int b = xcd_spread(blockIdx.x, gridDim.x);
@@ -94,10 +94,9 @@ AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, Fields
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   // the launch covers whole tiles only (remainder slabs go to the direct kernel)
   const long p = i + j * box.sy + k * box.sz;
-  if (bad_input(in, p)) *flag = 1;
   const int N = box.n[0] * box.n[1] * box.n[2];
   lds_double* t0 = (lds_double*)tiles;
-  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + Tl::NPAD, p,
+  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + 2 * Tl::NPAD, p,
                                   (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
                                   (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N,
                                   (int)threadIdx.x};
@@ -105,6 +104,7 @@ AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, Fields
   Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
   bssn_rhs_point<GAUGE, CO, false, false>(E, S);
   S.finish();
+  if (E.bad) *flag = 1;
 }
 
 template <int GAUGE, bool CO>

@@ -47,7 +47,7 @@ struct LdsEnv {
   const FI& F;
   const FI& M;
   const Coef& C;
-  lds_double* buf0;   // two LDS tile buffers
+  lds_double* buf0;   // two pairs of LDS tile buffers (2 x NPAD doubles each)
   lds_double* buf1;
   long p;         // global index of this thread's point (clamped into the array)
   int lt;         // LDS index of this thread's point
@@ -56,6 +56,7 @@ struct LdsEnv {
   int gmax;       // number of points of one field
   int tid;        // thread index in the workgroup
   int g = 0;      // next group to open
+  bool bad = false;  // a NaN among the centre values of the first 21 fields (bssn_rhs.f90:110-113)
 
   // Stream the halo tile of field v into buffer b: every lane issues ROUNDS
   // 16-byte direct global->LDS loads.  The buffer is padded to whole rounds,
@@ -77,15 +78,31 @@ struct LdsEnv {
                                        16, 0, 0);
     }
   }
-  __device__ __forceinline__ lds_double* cur() const { return (g & 1) ? buf0 : buf1; }  // g already advanced
+  // Groups are staged two at a time: groups 2q and 2q+1 share pair q & 1,
+  // so a tile needs one barrier per two field groups.
+  __device__ __forceinline__ lds_double* slot(int grp) const {
+    return (((grp >> 1) & 1) ? buf1 : buf0) + (grp & 1) * T::NPAD;
+  }
+  __device__ __forceinline__ lds_double* cur() const { return slot(g - 1); }  // g already advanced
 
-  __device__ __forceinline__ void prologue() { issue(group_field(0), buf0); }
+  __device__ __forceinline__ void prologue() {
+    issue(group_field(0), slot(0));
+    issue(group_field(1), slot(1));
+  }
   __device__ __forceinline__ void begin(int v) {
-    (void)v;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA of group g (and everything else) landed
-    __syncthreads();                // ... for every wave; and group g-1's readers are done
-    if (g + 1 < NFIELD) issue(group_field(g + 1), ((g + 1) & 1) ? buf1 : buf0);
+    if ((g & 1) == 0) {
+      __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA of groups g, g+1 (and everything else) landed
+      __syncthreads();                // ... for every wave; and the readers of groups g-2, g-1 are done
+      if (g + 2 < NFIELD) {
+        issue(group_field(g + 2), slot(g + 2));
+        issue(group_field(g + 3), slot(g + 3));
+      }
+    }
     ++g;
+    if (v < 21) {
+      const double x = cur()[lt];
+      bad |= !(x == x);
+    }
   }
   __device__ __forceinline__ void end() const {}
 
