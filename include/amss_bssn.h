@@ -175,16 +175,16 @@ int amss_level_variant(amss_level *L);
 int amss_level_timing(amss_level *L, int enable);
 int amss_level_kernel_ms(amss_level *L, double *ms4, long *launches4);
 /* Batched step (modes 1, 2): the interior of every patch is cut into tiles of
- * one of AMSS_NUM_TILE_SHAPES shapes (0: 8x8x4, 1: 4x8x8, 2: 8x4x8 points);
+ * one of AMSS_NUM_TILE_SHAPES shapes (0: 8x8x4, 1: 4x8x8, 2: 8x4x8, 3: 8x8x8 points);
  * one launch per sub-step walks the tile list of all patches with `grid`
  * persistent workgroups (0: one workgroup per tile), in list order or
  * (remap 1) with each XCD given a contiguous run of tiles.  autotune times
- * every (shape, grid in {0, 2, 4} x CUs, remap) candidate on sub-step 0 of
+ * every (shape, grid in {0, 1, 2} x the resident workgroups, remap) candidate on sub-step 0 of
  * the current state (writes only the RHS and stage lists), keeps the fastest
  * and returns it in choice[3] (shape, grid, remap); ms (nms entries, may be
  * NULL) receives each candidate's time in that loop order.  The first
  * batched step autotunes by itself unless set_tune was called. */
-#define AMSS_NUM_TILE_SHAPES 3
+#define AMSS_NUM_TILE_SHAPES 4
 int amss_level_autotune(amss_level *L, double *ms, int nms, int *choice);
 int amss_level_set_tune(amss_level *L, int shape, int grid, int remap);
 int amss_level_get_tune(amss_level *L, int *choice);

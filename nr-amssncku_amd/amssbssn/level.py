@@ -94,17 +94,18 @@ class Level:
     def variant(self) -> int:
         return lib().amss_level_variant(self.h)
 
-    TILE_SHAPES = ((8, 8, 4), (4, 8, 8), (8, 4, 8))
+    TILE_SHAPES = ((8, 8, 4), (4, 8, 8), (8, 4, 8), (8, 8, 8))
 
     def autotune(self):
         """Time every batched-kernel candidate (tile shape x persistent grid x
         tile order) on the current state, keep the fastest.  Returns
         (choice dict, list of (shape, grid, remap, ms))."""
-        n = 3 * 3 * 2
+        ns = len(self.TILE_SHAPES)
+        n = ns * 3 * 2
         ms = (C.c_double * n)()
         ch = (C.c_int * 3)()
         check(lib().amss_level_autotune(self.h, ms, n, ch), "amss_level_autotune")
-        cands = [(s, g, r) for s in range(3) for g in range(3) for r in range(2)]
+        cands = [(s, g, r) for s in range(ns) for g in range(3) for r in range(2)]
         return self._tune_dict(ch), [(s, g, r, float(t)) for (s, g, r), t in zip(cands, ms)]
 
     def set_tune(self, shape: int, grid: int, remap: int) -> None:

@@ -597,8 +597,8 @@ int BssnLevel::set_tune(const BatchTune& t) {
   return 0;
 }
 
-// Candidates: 3 tile shapes x {one workgroup per tile, persistent with 1x and
-// 2x the resident workgroups (2 per CU at 2 waves per SIMD)} x {list order,
+// Candidates: the tile shapes x {one workgroup per tile, persistent with 1x
+// and 2x the resident workgroups (2 waves per SIMD)} x {list order,
 // XCD-contiguous order}.  Each is timed (best of 2 after a warm-up) on
 // sub-step 0 of the current state, which writes only the RHS and stage lists
 // that the next step overwrites anyway.  ms (may be NULL) receives the times
@@ -621,7 +621,9 @@ int BssnLevel::autotune(double* ms, int nms) {
   for (int sh = 0; sh < kNumTileShapes; ++sh)
     for (int gm = 0; gm < 3; ++gm)
       for (int rm = 0; rm < 2; ++rm, ++c) {
-        const BatchTune t{sh, gm * 2 * ncu_, rm};
+        // resident workgroups per CU at 2 waves per SIMD: 512 threads / tile size
+        const int per_cu = 512 / (tile_dim(sh, 0) * tile_dim(sh, 1) * tile_dim(sh, 2));
+        const BatchTune t{sh, gm * per_cu * ncu_, rm};
         float tbest = 1e30f;
         for (int rep = 0; rep < 3; ++rep) {
           RCHK(hipEventRecord(e0, stream_));

@@ -43,10 +43,10 @@ struct SegRef {
 };
 
 // Tile shapes of the batched kernel (points per tile along x, y, z).
-constexpr int kNumTileShapes = 3;
+constexpr int kNumTileShapes = 4;
 __host__ __device__ constexpr int tile_dim(int shape, int a) {
-  // 0: 8x8x4, 1: 4x8x8, 2: 8x4x8 -- 256 threads each
-  return shape == 0 ? (a == 2 ? 4 : 8) : shape == 1 ? (a == 0 ? 4 : 8) : (a == 1 ? 4 : 8);
+  // 0: 8x8x4, 1: 4x8x8, 2: 8x4x8 (256 threads); 3: 8x8x8 (512 threads)
+  return shape == 0 ? (a == 2 ? 4 : 8) : shape == 1 ? (a == 0 ? 4 : 8) : shape == 2 ? (a == 1 ? 4 : 8) : 8;
 }
 
 // Workgroup t runs on XCD t % 8 (round-robin dispatch over the 8 XCDs of an
@@ -74,9 +74,11 @@ void launch_rk4_tiles(int gauge, const PatchDesc* P, const TileRef* T, int t0, i
 AMSS_DECL_TILES(0, 0)
 AMSS_DECL_TILES(0, 1)
 AMSS_DECL_TILES(0, 2)
+AMSS_DECL_TILES(0, 3)
 AMSS_DECL_TILES(2, 0)
 AMSS_DECL_TILES(2, 1)
 AMSS_DECL_TILES(2, 2)
+AMSS_DECL_TILES(2, 3)
 #undef AMSS_DECL_TILES
 // Frozen / copied boundary points of every patch (boxes of points of u0 -> out).
 void launch_fill_boxes(const PatchDesc* P, const FillBox* B, int nbox, int maxpts, const Roles& m, int stage,

@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(256) fill_boxes_kernel(const PatchDesc* __rest
   const FillBox b = B[blockIdx.y];
   const PatchDesc d = P[b.patch];
   const long ln = (long)NFIELD * d.n;
-  const SlabFields u0{d.slab + m.u0 * ln, d.n}, out{d.slab + m.out * ln, d.n};
+  const SlabFields u0{(gdouble*)d.slab + m.u0 * ln, d.n}, out{(gdouble*)d.slab + m.out * ln, d.n};
   const long npts = (long)b.ext[0] * b.ext[1] * b.ext[2];
   for (long t = blockIdx.x * 256L + threadIdx.x; t < npts; t += gridDim.x * 256L) {
     const int i = b.lo[0] + (int)(t % b.ext[0]);
@@ -57,8 +57,8 @@ __global__ void __launch_bounds__(256) seg_boxes_kernel(const PatchDesc* __restr
                                                         const SegRef* __restrict__ S, int slot) {
   const SegRef g = S[blockIdx.y];
   const PatchDesc a = P[g.src], b = P[g.dst];
-  const double* src = a.slab + (long)slot * NFIELD * a.n;
-  double* dst = b.slab + (long)slot * NFIELD * b.n;
+  const gdouble* src = (gdouble*)a.slab + (long)slot * NFIELD * a.n;
+  gdouble* dst = (gdouble*)b.slab + (long)slot * NFIELD * b.n;
   const long npts = (long)g.ext[0] * g.ext[1] * g.ext[2];
   for (long t = blockIdx.x * 256L + threadIdx.x; t < npts; t += gridDim.x * 256L) {
     const int x = (int)(t % g.ext[0]), y = (int)((t / g.ext[0]) % g.ext[1]);
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) seg_boxes_adjoint_kernel(const PatchDesc*
 // enforce_ga (enforce_algebra.f90:72-127) on every point of every patch.
 __global__ void __launch_bounds__(256) enforce_patches_kernel(const PatchDesc* __restrict__ P, int slot) {
   const PatchDesc d = P[blockIdx.y];
-  const SlabFields f{d.slab + (long)slot * NFIELD * d.n, d.n};
+  const SlabFields f{(gdouble*)d.slab + (long)slot * NFIELD * d.n, d.n};
   for (long p = blockIdx.x * 256L + threadIdx.x; p < d.n; p += gridDim.x * 256L) {
     double ga[12];
 #pragma unroll
@@ -119,9 +119,9 @@ void launch_rk4_tiles(int gauge, const PatchDesc* P, const TileRef* T, int t0, i
     launch_rk4_tiles_g##G##_s##S(P, T, t0, ntiles, m, C, R, want_res, tune, flag, s); \
     break;
   if (gauge == 2) {
-    switch (tune.shape) { AMSS_CASE(2, 0) AMSS_CASE(2, 1) AMSS_CASE(2, 2) }
+    switch (tune.shape) { AMSS_CASE(2, 0) AMSS_CASE(2, 1) AMSS_CASE(2, 2) AMSS_CASE(2, 3) }
   } else {
-    switch (tune.shape) { AMSS_CASE(0, 0) AMSS_CASE(0, 1) AMSS_CASE(0, 2) }
+    switch (tune.shape) { AMSS_CASE(0, 0) AMSS_CASE(0, 1) AMSS_CASE(0, 2) AMSS_CASE(0, 3) }
   }
 #undef AMSS_CASE
 }

@@ -6,40 +6,74 @@
 
 namespace amss {
 
-// Persistent workgroups stride over the tile list; each tile runs the same
-// LDS-staged point computation as rk4_lds_kernel, with the fields of its
-// patch addressed through the slab and the role slots.  The LDS buffers need
-// no extra barrier between tiles: the next tile's prologue fills pair 0, last
-// read by groups 20 and 21, which every wave finished before the barrier that
-// opened groups 22 and 23.
+// One tile of the list: the same LDS-staged point computation as
+// rk4_lds_kernel, with the fields of the tile's patch addressed through the
+// slab and the role slots.
 template <int GAUGE, int STAGE, bool CO, int TX, int TY, int TZ>
+__device__ __forceinline__ void tile_body(const PatchDesc* __restrict__ P, const TileRef& r, const Roles& m,
+                                          const Coef& C, const Rk4Param& R, int* flag, lds_double* b0, int tid) {
+  using Tl = LdsTile<TX, TY, TZ>;
+  const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+  const int ltv = (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3));
+  const PatchDesc d = P[r.patch];
+  const long p = (r.i0 + tx) + (long)(r.j0 + ty) * d.sy + (long)(r.k0 + tz) * d.sz;
+  const long ln = (long)NFIELD * d.n;
+  gdouble* slab = (gdouble*)d.slab;
+  const SlabFields in{slab + m.in * ln, d.n}, u0{slab + m.u0 * ln, d.n};
+  const SlabFields acc{slab + m.acc * ln, d.n}, out{slab + m.out * ln, d.n};
+  const SlabAux aux{slab + kResidualBase * d.n, d.n};
+  LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + 2 * Tl::NPAD, p, ltv,
+                                   (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz, d.sy, d.sz, (int)d.n, tid};
+  E.prologue();
+  Rk4SinkT<STAGE, SlabFields, SlabFields, SlabAux> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  S.finish();
+  if (E.bad) *flag = 1;
+}
+
+// PERSIST: a grid of resident workgroups strides over the tile list.  The
+// LDS buffers need no extra barrier between tiles: the next tile's prologue
+// fills pair 0, last read by groups 20 and 21, which every wave finished
+// before the barrier that opened groups 22 and 23.  Otherwise one workgroup
+// per tile (no loop: a loop makes the compiler keep more values across
+// iterations and spill more).
+template <int GAUGE, int STAGE, bool CO, int TX, int TY, int TZ, bool PERSIST>
 __global__ void __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(2, 2)))
 rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T, int t0, int ntiles, Roles m,
                  Coef C, Rk4Param R, int remap, int* flag) {
   using Tl = LdsTile<TX, TY, TZ>;
   __shared__ __attribute__((aligned(16))) double tiles[4 * Tl::NPAD];
   lds_double* b0 = (lds_double*)tiles;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    // keep the compiler from hoisting per-thread LDS addresses out of the
-    // loop (hundreds of loop-invariant addresses would occupy VGPRs)
-    int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
-    const int ltv = (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3));
+  if (PERSIST) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      // keep the compiler from hoisting per-thread LDS addresses out of the
+      // loop (hundreds of loop-invariant addresses would occupy VGPRs)
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      const TileRef r = T[t0 + (remap ? xcd_spread(t, ntiles) : t)];
+      tile_body<GAUGE, STAGE, CO, TX, TY, TZ>(P, r, m, C, R, flag, b0, tid);
+    }
+  } else {
+    const int t = blockIdx.x;
     const TileRef r = T[t0 + (remap ? xcd_spread(t, ntiles) : t)];
-    const PatchDesc d = P[r.patch];
-    const long p = (r.i0 + tx) + (long)(r.j0 + ty) * d.sy + (long)(r.k0 + tz) * d.sz;
-    const long ln = (long)NFIELD * d.n;
-    const SlabFields in{d.slab + m.in * ln, d.n}, u0{d.slab + m.u0 * ln, d.n};
-    const SlabFields acc{d.slab + m.acc * ln, d.n}, out{d.slab + m.out * ln, d.n};
-    const SlabAux aux{d.slab + kResidualBase * d.n, d.n};
-    LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + 2 * Tl::NPAD, p, ltv,
-                                     (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz, d.sy, d.sz, (int)d.n, tid};
-    E.prologue();
-    Rk4SinkT<STAGE, SlabFields, SlabFields, SlabAux> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
-    bssn_rhs_point<GAUGE, CO, false, false>(E, S);
-    S.finish();
-    if (E.bad) *flag = 1;
+    tile_body<GAUGE, STAGE, CO, TX, TY, TZ>(P, r, m, C, R, flag, b0, (int)threadIdx.x);
+  }
+}
+
+template <int G, int TX, int TY, int TZ, bool PERSIST>
+void launch_tiles_p(unsigned g, const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
+                    const Coef& C, const Rk4Param& R, bool want_res, int remap, int* flag, hipStream_t s) {
+  const int nt = TX * TY * TZ;
+  switch (R.stage) {
+    case 0:
+      if (want_res)
+        rk4_tiles_kernel<G, 0, true, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag);
+      else
+        rk4_tiles_kernel<G, 0, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag);
+      break;
+    case 1: rk4_tiles_kernel<G, 1, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag); break;
+    case 2: rk4_tiles_kernel<G, 2, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag); break;
+    default: rk4_tiles_kernel<G, 3, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag); break;
   }
 }
 
@@ -48,19 +82,10 @@ void launch_rk4_tiles_shape(const PatchDesc* P, const TileRef* T, int t0, int nt
                             const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune, int* flag,
                             hipStream_t s) {
   if (ntiles <= 0) return;
-  const unsigned g = (unsigned)(tune.grid > 0 && tune.grid < ntiles ? tune.grid : ntiles);
-  const int nt = TX * TY * TZ;
-  switch (R.stage) {
-    case 0:
-      if (want_res)
-        rk4_tiles_kernel<G, 0, true, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag);
-      else
-        rk4_tiles_kernel<G, 0, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag);
-      break;
-    case 1: rk4_tiles_kernel<G, 1, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag); break;
-    case 2: rk4_tiles_kernel<G, 2, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag); break;
-    default: rk4_tiles_kernel<G, 3, false, TX, TY, TZ><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, tune.remap, flag); break;
-  }
+  if (tune.grid > 0 && tune.grid < ntiles)
+    launch_tiles_p<G, TX, TY, TZ, true>((unsigned)tune.grid, P, T, t0, ntiles, m, C, R, want_res, tune.remap, flag, s);
+  else
+    launch_tiles_p<G, TX, TY, TZ, false>((unsigned)ntiles, P, T, t0, ntiles, m, C, R, want_res, tune.remap, flag, s);
 }
 
 // One translation unit per gauge and tile shape (they compile in parallel).

@@ -46,12 +46,17 @@ struct FieldsOut {
   double* __restrict__ p[NFIELD];
   __device__ __forceinline__ double* ptr(int v) const { return p[v]; }
 };
+// A double in global memory.  Pointers read from device tables are generic
+// to the compiler, which then emits flat loads; flat operations count against
+// the LDS wait counter too, so every LDS wait would also wait for them.
+typedef __attribute__((address_space(1))) double gdouble;
+
 // The 24 fields of one time level inside a Block's slab: field v starts at
 // base + v * n (a Block keeps its lists as consecutive 24-field runs).
 struct SlabFields {
-  double* base;
+  gdouble* base;
   long n;
-  __device__ __forceinline__ double* ptr(int v) const { return base + v * n; }
+  __device__ __forceinline__ gdouble* ptr(int v) const { return base + v * n; }
 };
 
 // Finite-difference coefficients of one patch (host-computed from

@@ -235,9 +235,9 @@ struct AuxOut {
 };
 // The 7 residual fields inside a Block's slab: residual a at base + a * n.
 struct SlabAux {
-  double* base;
+  gdouble* base;
   long n;
-  __device__ __forceinline__ double* resp(int a) const { return base + a * n; }
+  __device__ __forceinline__ gdouble* resp(int a) const { return base + a * n; }
 };
 
 // Plain RHS sink: k -> rhs arrays.

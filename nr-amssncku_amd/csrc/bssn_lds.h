@@ -64,7 +64,7 @@ struct LdsEnv {
   // address) and the loop has no bounds branches -- branches here split the
   // point kernel into many blocks and wreck its register allocation.
   __device__ __forceinline__ void issue(int v, lds_double* b) const {
-    const double* base = F.ptr(v);
+    const auto base = F.ptr(v);
     const int wave = tid >> 6;
 #pragma unroll
     for (int r = 0; r < T::ROUNDS; ++r) {
@@ -73,7 +73,7 @@ struct LdsEnv {
       const int x = e % T::LX, y = (e / T::LX) % T::LY, z = e / (T::LX * T::LY);
       int gi = gbase + x + y * sy + z * sz;  // < 2^29 points per field
       gi = gi < 0 ? 0 : (gi > gmax - 2 ? gmax - 2 : gi);
-      __builtin_amdgcn_global_load_lds((const void*)(base + gi),
+      __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(base + gi),
                                        (__attribute__((address_space(3))) void*)(b + 2 * (r * T::NT + wave * 64)),
                                        16, 0, 0);
     }

@@ -51,7 +51,7 @@ def test_batched_matches_per_patch_uneven_level(mode):
     print(f"mode {mode}: worst {worst:.2e}, ham {h1:.6e}")
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("grid,remap", [(0, 0), (0, 1), (24, 1), (24, 0)])
 def test_every_tuning_candidate_matches_per_patch(shape, grid, remap):
     import amssbssn as A
@@ -132,8 +132,8 @@ def test_autotune_picks_a_valid_candidate():
     cfg = _uneven(npatch=4, seed=13)
     lv = A.Level.from_config(cfg)
     choice, table = lv.autotune()
-    assert len(table) == 18 and all(t[3] > 0 for t in table)
-    assert choice["shape"] in (0, 1, 2) and choice["grid"] >= 0
+    assert len(table) == 24 and all(t[3] > 0 for t in table)
+    assert choice["shape"] in (0, 1, 2, 3) and choice["grid"] >= 0
     best = min(t[3] for t in table)
     assert any(t[3] == best and t[0] == choice["shape"] and t[2] == int(choice["xcd_order"]) for t in table)
     assert lv.get_tune() == choice
