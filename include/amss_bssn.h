@@ -212,6 +212,15 @@ amss_adjoint *amss_adjoint_create(const int *ex, const double *h, double eps, do
 amss_adjoint *amss_level_adjoint_create(int npatch, const int *shapes, const double *bboxes, double eps,
                                         double chitiny, int gauge);
 size_t amss_adjoint_bytes(amss_adjoint *A);
+/* A tape of `slots` steps' stage data (192 grid functions per patch each):
+ * forward_slot keeps the step's stage data in slot `slot`, and step_slot
+ * then reverses that step from it without re-evaluating it (U must be the
+ * state that forward_slot stepped). */
+int amss_adjoint_reserve(amss_adjoint *A, int slots);
+int amss_adjoint_forward_slot(amss_adjoint *A, double dT, const double *const *U, double *const *Unext, int slot,
+                              void *stream);
+int amss_adjoint_step_slot(amss_adjoint *A, double dT, const double *const *U, double *const *lam, int slot,
+                           void *stream);
 void amss_adjoint_destroy(amss_adjoint *A);
 int amss_adjoint_forward(amss_adjoint *A, double dT, const double *const *U, double *const *Unext, void *stream);
 int amss_adjoint_step(amss_adjoint *A, double dT, const double *const *U, double *const *lam, void *stream);

@@ -48,6 +48,9 @@ SIGNATURES = {
     "amss_adjoint_destroy": ([C.c_void_p], None),
     "amss_level_adjoint_create": ([C.c_int, _ip, _dp, C.c_double, C.c_double, C.c_int], C.c_void_p),
     "amss_adjoint_bytes": ([C.c_void_p], C.c_size_t),
+    "amss_adjoint_reserve": ([C.c_void_p, C.c_int], C.c_int),
+    "amss_adjoint_forward_slot": ([C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "amss_adjoint_step_slot": ([C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "amss_adjoint_forward": ([C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amss_adjoint_step": ([C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "amss_adjoint_rhs": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),

@@ -101,9 +101,17 @@ class LevelAdjoint:
     def bytes(self) -> int:
         return int(lib().amss_adjoint_bytes(self.h))
 
-    def forward(self, dt: float, U, Unext) -> None:
-        check(lib().amss_adjoint_forward(self.h, dt, _level_ptrs(U), _level_ptrs(Unext), _stream()),
-              "amss_adjoint_forward")
+    def reserve(self, slots: int) -> None:
+        """A tape of `slots` steps' stage data (see forward / step)."""
+        check(lib().amss_adjoint_reserve(self.h, slots), "amss_adjoint_reserve")
 
-    def step(self, dt: float, U, lam) -> None:
-        check(lib().amss_adjoint_step(self.h, dt, _level_ptrs(U), _level_ptrs(lam), _stream()), "amss_adjoint_step")
+    def forward(self, dt: float, U, Unext, slot: int = -1) -> None:
+        """Unext <- one step of U; slot >= 0 keeps the step's stage data in that tape slot."""
+        check(lib().amss_adjoint_forward_slot(self.h, dt, _level_ptrs(U), _level_ptrs(Unext), slot, _stream()),
+              "amss_adjoint_forward_slot")
+
+    def step(self, dt: float, U, lam, slot: int = -1) -> None:
+        """lam: dJ/dU^{n+1} in, dJ/dU^n out; slot >= 0 reuses that slot's stage data
+        (which forward(U, ..., slot) must have written), -1 re-evaluates the step."""
+        check(lib().amss_adjoint_step_slot(self.h, dt, _level_ptrs(U), _level_ptrs(lam), slot, _stream()),
+              "amss_adjoint_step_slot")

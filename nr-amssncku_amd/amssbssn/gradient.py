@@ -77,15 +77,17 @@ def config4(steps: int = 400, checkpoint: int = 0) -> DiffConfig:
                       masses=(0.55, 0.45), xs=(4.0, -4.0), spline_n=0)
 
 
-def plan_checkpoints(steps: int, state_bytes: int, free_bytes: int, reserve: float = 0.1) -> int:
+def plan_checkpoints(steps: int, state_bytes: int, free_bytes: int, reserve: float = 0.1,
+                     segment_factor: float = 1.0) -> int:
     """Checkpoint spacing k for a run of `steps`: the backward pass holds
-    ceil(steps/k) checkpoints plus the k states of the segment it rebuilds.
+    ceil(steps/k) checkpoints (state_bytes each) plus what it keeps for the k
+    steps of the segment it rebuilds (segment_factor x state_bytes each).
     Each segment rebuild costs k - 1 forward steps, so the recomputation is
-    steps - ceil(steps/k) forward steps in total: the smallest k whose states
-    fit in the free memory (less a reserve) recomputes least."""
-    budget = int(free_bytes * (1.0 - reserve)) // max(state_bytes, 1)
+    steps - ceil(steps/k) forward steps in total: the smallest k that fits in
+    the free memory (less a reserve) recomputes least."""
+    budget = free_bytes * (1.0 - reserve) / max(state_bytes, 1)
     for k in range(1, steps + 1):
-        if math.ceil(steps / k) + k <= budget:
+        if math.ceil(steps / k) + segment_factor * k <= budget:
             return k
     raise MemoryError(f"{steps} steps of {state_bytes} B states do not fit in {free_bytes} B")
 
@@ -189,10 +191,15 @@ class Evolution:
     """The forward evolution (the level's fused step) with checkpoints, and
     the backward pass through the level's step adjoint."""
 
-    def __init__(self, cfg: DiffConfig, mode: int = 0):
+    def __init__(self, cfg: DiffConfig, mode: int = 0, tape: bool = True):
         self.cfg = cfg
         self.level = Level(cfg.patches, gauge=0, eps=cfg.eps, chitiny=cfg.chitiny)
         self.adj = LevelAdjoint(cfg.patches, eps=cfg.eps, chitiny=cfg.chitiny)
+        # with a tape, each segment's rebuild keeps every step's stage data and
+        # the step adjoints reuse it instead of re-evaluating the step
+        self.tape = tape
+        if tape:
+            self.adj.reserve(cfg.checkpoint)
         self.mode = mode
         if mode > 0:
             self.level.autotune()  # outside the timed forward (the first batched step would run it)
@@ -248,14 +255,14 @@ class Evolution:
         for s in range(len(ckpts) - 1, -1, -1):
             k = min(cfg.checkpoint, cfg.steps - s * cfg.checkpoint)
             states = [ckpts[s]]
-            for _ in range(k - 1):
+            for n in range(k if self.tape else k - 1):
                 nxt = [torch.empty_like(U) for U in states[-1]]
-                self.adj.forward(cfg.dt, states[-1], nxt)
+                self.adj.forward(cfg.dt, states[-1], nxt, n if self.tape else -1)
                 states.append(nxt)
             if s == len(ckpts) - 1:
                 self._note_memory()
             for n in range(k - 1, -1, -1):
-                self.adj.step(cfg.dt, states[n], lams)
+                self.adj.step(cfg.dt, states[n], lams, n if self.tape else -1)
         torch.cuda.synchronize()
         self.timing["backward_s"] = time.perf_counter() - t0
         return lams
@@ -293,32 +300,43 @@ def evolve_observable(U0s, evo: Evolution, obs: Observable):
     return _EVOLVE.apply(evo, obs, *U0s)
 
 
-def _with_checkpoint(cfg: DiffConfig, device) -> DiffConfig:
-    if cfg.checkpoint > 0:
-        return cfg
+TAPE_FIELDS = 192  # stage data of one step (Y0..Y3, z0..z3), grid functions per patch
+
+
+def _plan(cfg: DiffConfig, device):
+    """(config with its checkpoint spacing, whether the segment tape fits)."""
     import torch
     free, _ = torch.cuda.mem_get_info()
     state = sum(NF * 8 * p.npts for p in cfg.patches)
-    # the level (127 grid functions) and the adjoint work arrays (~520) are
-    # allocated next to the checkpoints
+    # the level (127 grid functions) and the adjoint work arrays (~520) sit
+    # next to the checkpoints
     work = sum((127 + 520) * 8 * p.npts for p in cfg.patches)
-    return dataclasses.replace(cfg, checkpoint=plan_checkpoints(cfg.steps, state, free - work))
+    factor = 1.0 + TAPE_FIELDS / NF  # a segment step keeps its state and its stage data
+    if cfg.checkpoint > 0:
+        k = cfg.checkpoint
+        fits = math.ceil(cfg.steps / k) + factor * k <= 0.9 * (free - work) / state
+        return cfg, fits
+    try:
+        return dataclasses.replace(cfg, checkpoint=plan_checkpoints(cfg.steps, state, free - work,
+                                                                    segment_factor=factor)), True
+    except MemoryError:
+        return dataclasses.replace(cfg, checkpoint=plan_checkpoints(cfg.steps, state, free - work)), False
 
 
 def value_and_grad(cfg: DiffConfig, theta=None, device="cuda", mode: int = 0):
     """(J, dJ/dtheta, info) for the configuration's parameters; info holds the
     forward / backward wall times, the checkpoint spacing and peak device memory."""
     import torch
-    cfg = _with_checkpoint(cfg, device)
+    cfg, tape = _plan(cfg, device)
     init = InitialData(cfg, device)
     obs = Observable(cfg, device)
-    evo = Evolution(cfg, mode)
+    evo = Evolution(cfg, mode, tape=tape)
     th = torch.tensor(cfg.theta0() if theta is None else theta, dtype=torch.float64, device=device,
                       requires_grad=True)
     try:
         J = evolve_observable(init(th), evo, obs)
         J.backward()
-        info = dict(evo.timing, checkpoint=cfg.checkpoint,
+        info = dict(evo.timing, checkpoint=cfg.checkpoint, tape=tape,
                     peak_device_bytes=int(evo.total - evo.min_free) if evo.min_free is not None else None)
         return float(J.detach()), th.grad.detach().cpu().numpy(), info
     finally:
@@ -331,7 +349,7 @@ def value(cfg: DiffConfig, theta, device="cuda", mode: int = 0) -> float:
     cfg = dataclasses.replace(cfg, checkpoint=cfg.checkpoint or cfg.steps)
     init = InitialData(cfg, device)
     obs = Observable(cfg, device)
-    evo = Evolution(cfg, mode)
+    evo = Evolution(cfg, mode, tape=False)
     try:
         with torch.no_grad():
             UT, _ = evo.forward(init(torch.tensor(theta, dtype=torch.float64, device=device)))

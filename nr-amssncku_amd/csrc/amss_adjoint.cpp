@@ -17,6 +17,7 @@ namespace {
 constexpr int kY0 = 0;                     // enforced U^n (24)
 constexpr int kY = kY0 + NFIELD;           // Y1..Y3, the inputs of sub-steps 1-3 (3 x 24)
 constexpr int kZ = kY + 3 * NFIELD;        // z0..z3, sub-step outputs before bound/enforce (4 x 24)
+constexpr int kStage = kZ + 4 * NFIELD;    // grid functions of one step's stage data
 constexpr int kK = kZ + 4 * NFIELD;        // k of the current sub-step (24)
 constexpr int kAcc = kK + NFIELD;          // RK4 accumulator (24)
 constexpr int kBars = kAcc + NFIELD;       // feature adjoints (kAdjFeatures)
@@ -48,16 +49,39 @@ AdjointBlock::AdjointBlock(const int ex[3], const Coef& C, double tinny) : C_(C)
 
 AdjointBlock::~AdjointBlock() {
   if (buf_) (void)hipFree(buf_);
+  if (tape_) (void)hipFree(tape_);
 }
 
+int AdjointBlock::reserve(int slots) {
+  if (slots <= slots_) return 0;
+  if (tape_) (void)hipFree(tape_);
+  tape_ = nullptr;
+  slots_ = 0;
+  const hipError_t e = hipMalloc(&tape_, (size_t)slots * kStage * N_ * sizeof(double));
+  if (e != hipSuccess) {
+    tape_ = nullptr;
+    err_ = std::string("hipMalloc of the adjoint tape: ") + hipGetErrorString(e);
+    return AMSS_ERR_HIP;
+  }
+  slots_ = slots;
+  return 0;
+}
+
+// grid function `first` (stage data from the current slot)
+double* AdjointBlock::base_of(int first, int slot) const {
+  if (first < kStage && slot >= 0) return tape_ + ((long)slot * kStage + first) * N_;
+  return buf_ + (long)first * N_;
+}
 FieldsIn AdjointBlock::fin(int first) const {
   FieldsIn f;
-  for (int v = 0; v < NFIELD; ++v) f.p[v] = buf_ + (long)(first + v) * N_;
+  const double* b = base_of(first, slot_);
+  for (int v = 0; v < NFIELD; ++v) f.p[v] = b + (long)v * N_;
   return f;
 }
 FieldsOut AdjointBlock::fout(int first) const {
   FieldsOut f;
-  for (int v = 0; v < NFIELD; ++v) f.p[v] = buf_ + (long)(first + v) * N_;
+  double* b = base_of(first, slot_);
+  for (int v = 0; v < NFIELD; ++v) f.p[v] = b + (long)v * N_;
   return f;
 }
 static FieldsIn as_in(const double* const* p) {
@@ -71,16 +95,16 @@ static FieldsOut as_out(double* const* p) {
   return f;
 }
 
-PatchDesc AdjointBlock::desc(int which) const {
+PatchDesc AdjointBlock::desc(int which, int slot) const {
   PatchDesc d;
-  d.slab = buf_ + (long)which * N_;
+  d.slab = base_of(which, slot);
   d.n = N_;
   d.sy = box_.n[0];
   d.sz = box_.n[0] * box_.n[1];
   for (int a = 0; a < 3; ++a) d.shape[a] = box_.n[a];
   return d;
 }
-double* AdjointBlock::stage_output(int st) const { return buf_ + (long)(st < 3 ? kY + st * NFIELD : kLY) * N_; }
+double* AdjointBlock::stage_output(int st) const { return base_of(st < 3 ? kY + st * NFIELD : kLY, slot_); }
 double* AdjointBlock::lY() const { return buf_ + (long)kLY * N_; }
 
 void AdjointBlock::fwd_open(const double* const* U, hipStream_t s) {
@@ -213,11 +237,41 @@ LevelAdjoint::LevelAdjoint(int npatch, const int* shapes, const double* bboxes, 
   // PatchDesc tables of the exchanged runs: Y1, Y2, Y3 and the scratch / dJ/dY run
   for (int which : {kY, kY + NFIELD, kY + 2 * NFIELD, kLY}) {
     std::vector<PatchDesc> t;
-    for (AdjointBlock* b : blocks_) t.push_back(b->desc(which));
+    for (AdjointBlock* b : blocks_) t.push_back(b->desc(which, -1));
     PatchDesc* d = nullptr;
     if (!upload(t.data(), t.size() * sizeof(PatchDesc), (void**)&d)) return;
     dtab_.push_back(d);
   }
+}
+
+int LevelAdjoint::reserve(int slots) {
+  const int have = ((int)dtab_.size() - 4) / 3;
+  if (slots <= have) return 0;
+  for (AdjointBlock* b : blocks_)
+    if (int r = b->reserve(slots)) {
+      err_ = b->err();
+      return r;
+    }
+  for (size_t i = 4; i < dtab_.size(); ++i) (void)hipFree(dtab_[i]);
+  dtab_.resize(4);
+  for (int sl = 0; sl < slots; ++sl)
+    for (int which : {kY, kY + NFIELD, kY + 2 * NFIELD}) {
+      std::vector<PatchDesc> t;
+      for (AdjointBlock* b : blocks_) t.push_back(b->desc(which, sl));
+      PatchDesc* d = nullptr;
+      if (hipMalloc(&d, t.size() * sizeof(PatchDesc)) != hipSuccess ||
+          hipMemcpy(d, t.data(), t.size() * sizeof(PatchDesc), hipMemcpyHostToDevice) != hipSuccess) {
+        err_ = "device tables of the adjoint tape";
+        return AMSS_ERR_HIP;
+      }
+      dtab_.push_back(d);
+    }
+  return 0;
+}
+
+PatchDesc* LevelAdjoint::table(int which, int slot) {
+  if (which == 3 || slot < 0) return dtab_[which];
+  return dtab_[4 + 3 * slot + which];
 }
 
 LevelAdjoint::~LevelAdjoint() {
@@ -229,24 +283,30 @@ LevelAdjoint::~LevelAdjoint() {
 
 size_t LevelAdjoint::bytes() const {
   size_t n = 0;
-  for (AdjointBlock* b : blocks_) n += (size_t)kFields * b->npts() * sizeof(double);
+  const int slots = ((int)dtab_.size() - 4) / 3;
+  for (AdjointBlock* b : blocks_) n += (size_t)(kFields + (size_t)slots * kStage) * b->npts() * sizeof(double);
   return n;
 }
 
-void LevelAdjoint::exchange(int table, hipStream_t s) {
-  if (!hseg_.empty()) launch_seg_boxes(dtab_[table], dseg_, (int)hseg_.size(), (int)seg_max_, 0, s);
+void LevelAdjoint::exchange(int tab, int slot, hipStream_t s) {
+  if (!hseg_.empty()) launch_seg_boxes(table(tab, slot), dseg_, (int)hseg_.size(), (int)seg_max_, 0, s);
 }
-void LevelAdjoint::exchange_adjoint(int table, hipStream_t s) {
-  if (!hseg_.empty()) launch_seg_boxes_adjoint(dtab_[table], dseg_, (int)hseg_.size(), (int)seg_max_, s);
+void LevelAdjoint::exchange_adjoint(hipStream_t s) {
+  if (!hseg_.empty()) launch_seg_boxes_adjoint(dtab_[3], dseg_, (int)hseg_.size(), (int)seg_max_, s);
 }
 
-int LevelAdjoint::forward(double dT, const double* const* U, double* const* Unext, hipStream_t s) {
+int LevelAdjoint::forward(double dT, const double* const* U, double* const* Unext, hipStream_t s, int slot) {
+  if (slot >= ((int)dtab_.size() - 4) / 3) {
+    err_ = "tape slot not reserved";
+    return AMSS_ERR_ARG;
+  }
   const double c[4] = {0.5 * dT, 0.5 * dT, dT, dT / 6.0};
   const int np = npatch();
+  for (AdjointBlock* b : blocks_) b->use_slot(slot);
   for (int p = 0; p < np; ++p) blocks_[p]->fwd_open(U + p * NFIELD, s);
   for (int st = 0; st < 4; ++st) {
     for (int p = 0; p < np; ++p) blocks_[p]->fwd_stage(st, c[st], nullptr, s);
-    exchange(st, s);  // tables 0..2 = Y1..Y3, 3 = the U^{n+1} scratch
+    exchange(st, slot, s);  // tables 0..2 = Y1..Y3, 3 = the U^{n+1} scratch
   }
   if (Unext)
     for (int p = 0; p < np; ++p) {
@@ -265,10 +325,16 @@ int LevelAdjoint::forward(double dT, const double* const* U, double* const* Unex
   return 0;
 }
 
-int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam, hipStream_t s) {
-  if (int r = forward(dT, U, nullptr, s)) return r;
+int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam, hipStream_t s, int slot) {
+  if (slot < 0) {
+    if (int r = forward(dT, U, nullptr, s)) return r;
+  } else if (slot >= ((int)dtab_.size() - 4) / 3) {
+    err_ = "tape slot not reserved";
+    return AMSS_ERR_ARG;
+  }
   const double c[4] = {0.5 * dT, 0.5 * dT, dT, dT / 6.0};
   const int np = npatch();
+  for (AdjointBlock* b : blocks_) b->use_slot(slot);
   // dJ/dU^{n+1} into each Block's dJ/dY run, then the final exchange reversed
   for (int p = 0; p < np; ++p) {
     const long N = blocks_[p]->npts();
@@ -279,7 +345,7 @@ int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam
         return AMSS_ERR_HIP;
       }
   }
-  exchange_adjoint(3, s);
+  exchange_adjoint(s);
   for (int p = 0; p < np; ++p) {
     double* ly = blocks_[p]->lY();
     const long N = blocks_[p]->npts();
@@ -289,7 +355,7 @@ int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam
   }
   for (int st = 3; st >= 1; --st) {
     for (int p = 0; p < np; ++p) blocks_[p]->rev_rhs(st, s);  // dJ/dY_st into the dJ/dY run
-    exchange_adjoint(3, s);  // Y_st was exchanged after sub-step st-1
+    exchange_adjoint(s);  // Y_st was exchanged after sub-step st-1
     for (int p = 0; p < np; ++p) blocks_[p]->rev_stage(st, c[st - 1], s);
   }
   for (int p = 0; p < np; ++p) blocks_[p]->rev_rhs(0, s);
@@ -302,6 +368,7 @@ int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam
 }
 
 int LevelAdjoint::rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s) {
+  for (AdjointBlock* b : blocks_) b->use_slot(-1);
   for (int p = 0; p < npatch(); ++p)
     blocks_[p]->rhs_vjp(U + p * NFIELD, lam + p * NFIELD, out + p * NFIELD, s);
   if (hipGetLastError() != hipSuccess) {

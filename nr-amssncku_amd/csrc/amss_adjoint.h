@@ -26,6 +26,10 @@ class AdjointBlock {
   long npts() const { return N_; }
   const Box& box() const { return box_; }
 
+  // Stage data (Y0..Y3, z0..z3: 192 grid functions) live in the work array
+  // (slot -1) or in slot s of a tape reserved for a segment of steps.
+  int reserve(int slots);
+  void use_slot(int slot) { slot_ = slot; }
   // forward: Y0 = E(U); stage s: k = RHS(Y_s), z_s, Y_{s+1} (s = 3: into Unext
   // or a scratch array)
   void fwd_open(const double* const* U, hipStream_t s);
@@ -37,7 +41,7 @@ class AdjointBlock {
   void rev_open(const double* const* U, double* const* lam, hipStream_t s);
   void rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s);
   // a 24-field run of the work slab as a PatchDesc "slot 0" (for the exchange kernels)
-  PatchDesc desc(int which) const;
+  PatchDesc desc(int which, int slot) const;
   double* stage_output(int st) const;  // Y_{st+1} (st = 0..2) or the scratch U^{n+1} (st = 3)
   double* lY() const;
 
@@ -49,6 +53,10 @@ class AdjointBlock {
   double tinny_;
   long N_ = 0;
   double* buf_ = nullptr;
+  double* tape_ = nullptr;
+  int slots_ = 0;
+  int slot_ = -1;
+  double* base_of(int first, int slot) const;
   std::string err_;
 };
 
@@ -62,21 +70,26 @@ class LevelAdjoint {
   bool ok() const { return err_.empty(); }
   const std::string& err() const { return err_; }
   int npatch() const { return (int)blocks_.size(); }
-  // U, Unext, lam: npatch x 24 device pointers (patch-major)
-  int forward(double dT, const double* const* U, double* const* Unext, hipStream_t s);
-  int backward(double dT, const double* const* U, double* const* lam, hipStream_t s);
+  // U, Unext, lam: npatch x 24 device pointers (patch-major).  slot >= 0:
+  // keep / use the stage data of this step in that tape slot (backward then
+  // skips re-evaluating the step); -1: the work arrays (backward re-evaluates).
+  int reserve(int slots);
+  int forward(double dT, const double* const* U, double* const* Unext, hipStream_t s, int slot = -1);
+  int backward(double dT, const double* const* U, double* const* lam, hipStream_t s, int slot = -1);
   int rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s);
   size_t bytes() const;
 
  private:
   // table: 0..2 = Y1..Y3, 3 = the U^{n+1} scratch / dJ/dY run
-  void exchange(int table, hipStream_t s);          // copy source interiors into ghost boxes
-  void exchange_adjoint(int table, hipStream_t s);  // add ghost-box adjoints to the sources
+  void exchange(int table, int slot, hipStream_t s);  // copy source interiors into ghost boxes
+  void exchange_adjoint(hipStream_t s);                // add the dJ/dY run's ghost-box adjoints to the sources
+  PatchDesc* table(int which, int slot);
   std::vector<AdjointBlock*> blocks_;
   std::vector<SegRef> hseg_;
   SegRef* dseg_ = nullptr;
   long seg_max_ = 0;
-  std::vector<PatchDesc*> dtab_;  // device PatchDesc tables of the exchanged runs (Y1, Y2, Y3, scratch)
+  std::vector<PatchDesc*> dtab_;  // device PatchDesc tables: Y1, Y2, Y3 of the work arrays, the dJ/dY run,
+                                  // then Y1..Y3 of each tape slot
   std::string err_;
 };
 

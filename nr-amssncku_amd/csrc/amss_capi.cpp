@@ -590,4 +590,24 @@ int amss_adjoint_rhs(amss_adjoint* A, const double* const* U, const double* cons
 
 size_t amss_adjoint_bytes(amss_adjoint* A) { return A ? A->A->bytes() : 0; }
 
+int amss_adjoint_reserve(amss_adjoint* A, int slots) {
+  if (!A || slots < 0) return fail(AMSS_ERR_ARG, "amss_adjoint_reserve: bad argument");
+  const int r = A->A->reserve(slots);
+  return r ? fail(r, A->A->err()) : 0;
+}
+
+int amss_adjoint_forward_slot(amss_adjoint* A, double dT, const double* const* U, double* const* Unext, int slot,
+                              void* stream) {
+  if (!A || !U) return fail(AMSS_ERR_ARG, "amss_adjoint_forward_slot: null argument");
+  const int r = A->A->forward(dT, U, Unext, (hipStream_t)stream, slot);
+  return r ? fail(r, A->A->err()) : 0;
+}
+
+int amss_adjoint_step_slot(amss_adjoint* A, double dT, const double* const* U, double* const* lam, int slot,
+                           void* stream) {
+  if (!A || !U || !lam) return fail(AMSS_ERR_ARG, "amss_adjoint_step_slot: null argument");
+  const int r = A->A->backward(dT, U, lam, (hipStream_t)stream, slot);
+  return r ? fail(r, A->A->err()) : 0;
+}
+
 }  // extern "C"

@@ -109,12 +109,14 @@ __global__ void __launch_bounds__(256) adj_gather_kernel(FieldsIn in, FieldsIn l
       }
     }
   }
-  // advection: sum over output points p = q - o e_a
+  // advection and KO: both read lam at the output points p = q - o e_a
+  // (the KO stencil is symmetric, so its transpose uses the same points)
   const double* L = lam.p[v];
   for (int a = 0; a < 3; ++a) {
     const double* beta = in.p[BETAX + a];
     const double up[7] = {0.0, 0.0, -3.0, -10.0, 18.0, -6.0, 1.0};  // offsets -3..3 from p
-    double s = 0.0;
+    const double kw[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+    double s = 0.0, t = 0.0;
     for (int o = -3; o <= 3; ++o) {
       int di, dj, dk;
       sh(a, -o, di, dj, dk);
@@ -123,19 +125,9 @@ __global__ void __launch_bounds__(256) adj_gather_kernel(FieldsIn in, FieldsIn l
       const double b = beta[pp], l = L[pp];
       // up weight at offset o; the downwind stencil is its mirror
       s += up[o + 3] * fmax(b, 0.0) * l - up[3 - o] * fmin(b, 0.0) * l;
+      t += kw[o + 3] * l;
     }
-    acc += C.d1[a] * s;
-  }
-  // Kreiss-Oliger (symmetric)
-  for (int a = 0; a < 3; ++a) {
-    const double kw[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
-    double s = 0.0;
-    for (int o = -3; o <= 3; ++o) {
-      int di, dj, dk;
-      sh(a, o, di, dj, dk);
-      s += kw[o + 3] * rd(L, di, dj, dk);
-    }
-    acc += C.ko[a] * s;
+    acc += C.d1[a] * s + C.ko[a] * t;
   }
   if (accumulate)
     out.p[v][q] += acc;

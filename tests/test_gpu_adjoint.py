@@ -211,3 +211,28 @@ def test_level_adjoint_split_equals_whole():
     Ls.close()
     print(f"whole {dw:.12e} split {ds:.12e} rel {abs(ds - dw) / abs(dw):.2e}")
     assert abs(ds - dw) <= 1e-11 * abs(dw)
+
+
+def test_tape_slots_give_the_same_adjoint():
+    """Reversing steps from tape slots written by the forward pass equals
+    reversing them with the step re-evaluated (same kernels, same inputs)."""
+    import torch
+    from amssbssn.adjoint import LevelAdjoint
+    cfg, p, st = _perturbed_puncture(n=12, shift=(0.01, 0.0, -0.01))
+    rng = np.random.default_rng(8)
+    w = _dev([rng.standard_normal(st[0].shape) for _ in range(24)])
+    L = LevelAdjoint([p], eps=cfg.eps)
+    L.reserve(3)
+    traj = [_dev(st)]
+    for n in range(3):
+        nxt = torch.empty_like(traj[-1])
+        L.forward(cfg.dt, [traj[-1]], [nxt], n)
+        traj.append(nxt)
+    a, b = w.clone(), w.clone()
+    for n in (2, 1, 0):
+        L.step(cfg.dt, [traj[n]], [a], n)
+    for n in (2, 1, 0):
+        L.step(cfg.dt, [traj[n]], [b], -1)
+    torch.cuda.synchronize()
+    L.close()
+    assert torch.equal(a, b)
