@@ -385,6 +385,9 @@ void BssnLevel::build_batch_tables() {
       box.sz = (long)b.shape[0] * b.shape[1];
       int core[3];
       for (int a = 0; a < 3; ++a) core[a] = 3 + (box.hi[a] - 3) / T[a] * T[a];
+      // tiles need an even row pitch (16-byte aligned direct loads); a patch
+      // with an odd one is all remainder (direct kernel)
+      if (b.shape[0] % 2 != 0) core[0] = core[1] = core[2] = 3;
       if (core[0] > 3 && core[1] > 3 && core[2] > 3) {
         if (b.shape[0] > 65535 || b.shape[1] > 65535 || b.shape[2] > 65535 || pi > 65535) {
           err_ = "patch too large for the tile list";

@@ -12,7 +12,7 @@
 // together in one "field group" and immediately reduced to the handful of
 // numbers the algebra needs (e.g. a metric component's Hessian only enters
 // through gu^kl d_k d_l g_ij), so only reduced quantities cross groups.
-// AMSS_FENCE() between groups stops the compiler from hoisting the loads of
+// AMSS_FENCE() between groups (where Env::kFence) stops the compiler from hoisting the loads of
 // later groups (and from keeping loaded neighbours alive across groups).
 //
 // Env provides the stencils of the stored input fields at this point:
@@ -27,13 +27,16 @@ namespace amss {
 
 constexpr double kPi = 3.141592653589793;  // acos(-1), bssn_rhs.f90:140
 
-#ifndef AMSS_FENCE
-#define AMSS_FENCE()                     \
-  do {                                   \
-    asm volatile("" ::: "memory");       \
-    __builtin_amdgcn_sched_barrier(0);   \
-  } while (0)
-#endif
+// Group fence: the environment decides (Env::kFence) whether the compiler
+// may move work across field groups.
+template <class Env>
+__device__ __forceinline__ void amss_fence() {
+  if constexpr (Env::kFence) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+#define AMSS_FENCE() amss_fence<Env>()
 
 template <int GAUGE, bool CO, bool GAMRIC, bool MATTER, class Env, class Sink>
 __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {

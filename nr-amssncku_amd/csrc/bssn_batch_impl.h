@@ -22,7 +22,7 @@ __device__ __forceinline__ void tile_body(const PatchDesc* __restrict__ P, const
   const SlabFields in{slab + m.in * ln, d.n}, u0{slab + m.u0 * ln, d.n};
   const SlabFields acc{slab + m.acc * ln, d.n}, out{slab + m.out * ln, d.n};
   const SlabAux aux{slab + kResidualBase * d.n, d.n};
-  LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + 2 * Tl::NPAD, p, ltv,
+  LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + (kLdsBuffers / 2) * Tl::NPAD, p, ltv,
                                    (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz, d.sy, d.sz, (int)d.n, tid};
   E.prologue();
   Rk4SinkT<STAGE, SlabFields, SlabFields, SlabAux> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu
 rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T, int t0, int ntiles, Roles m,
                  Coef C, Rk4Param R, int remap, int* flag) {
   using Tl = LdsTile<TX, TY, TZ>;
-  __shared__ __attribute__((aligned(16))) double tiles[4 * Tl::NPAD];
+  __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
   lds_double* b0 = (lds_double*)tiles;
   if (PERSIST) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {

@@ -15,6 +15,7 @@ namespace amss {
 // stride would cost a 64-bit per-lane address pair.
 // Requires 3 + 3 sy + 3 sz <= p and field size < 4 GiB.
 struct InteriorEnv {
+  static constexpr bool kFence = true;
   const FieldsIn& F;
   const FieldsIn& M;
   const Coef& C;
@@ -95,6 +96,7 @@ struct InteriorEnv {
 // per direction, KO needs +-3 room on every axis, and the halo index -1
 // mirrors index 0 with the field's parity (symmetry_bd, fmisc.f90:877-901).
 struct EdgeEnv {
+  static constexpr bool kFence = true;
   const FieldsIn& F;
   const FieldsIn& M;
   const Coef& C;

@@ -81,11 +81,15 @@ AMSS_POINT_KERNEL rk4_interior_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, F
 // tiles in XCD-contiguous order: consecutive tiles, whose halos overlap, run
 // behind the same L2.
 constexpr int kLdsTX = 8, kLdsTY = 8, kLdsTZ = 4;
+#ifndef AMSS_LDS_WAVES
+#define AMSS_LDS_WAVES 2
+#endif
 template <int GAUGE, int STAGE, bool CO>
-AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
+__global__ void __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(AMSS_LDS_WAVES, AMSS_LDS_WAVES)))
+rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
                                  Coef C, Rk4Param R, int* flag) {
   using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
-  __shared__ __attribute__((aligned(16))) double tiles[4 * Tl::NPAD];
+  __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
   const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
   const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
   const int b = xcd_spread(blockIdx.x, gridDim.x);
@@ -96,7 +100,7 @@ AMSS_POINT_KERNEL rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, Fields
   const long p = i + j * box.sy + k * box.sz;
   const int N = box.n[0] * box.n[1] * box.n[2];
   lds_double* t0 = (lds_double*)tiles;
-  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + 2 * Tl::NPAD, p,
+  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p,
                                   (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
                                   (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N,
                                   (int)threadIdx.x};
@@ -144,7 +148,9 @@ template <int G>
 void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                       const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
                       bool edge, hipStream_t s, bool interior) {
-  if (interior && !box_empty(box) && R.variant == 1) {
+  // The LDS kernel's 16-byte direct loads start at even x of a row, which is
+  // 16-byte aligned only when the row pitch is even; odd pitches go direct.
+  if (interior && !box_empty(box) && R.variant == 1 && (box.sy % 2) == 0) {
     // LDS kernel on the largest tile-aligned sub-box, the direct kernel on the rest
     Box core = box;
     const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
@@ -172,7 +178,7 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
       if (!box_empty(rem)) launch_rk4_gauge<G>(in, u0, acc, out, aux, rem, C, Rd, want_res, flag, false, s, true);
     }
   } else if (interior && !box_empty(box)) {
-    const dim3 gr = interior_grid(box);
+    const dim3 gr = interior_grid(box);  // also variant 1 on an odd row pitch
     switch (R.stage) {
       case 0:
         if (want_res) rk4_interior_kernel<G, 0, true><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);

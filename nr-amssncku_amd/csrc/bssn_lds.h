@@ -39,8 +39,19 @@ struct LdsTile {
   static_assert(NT % 64 == 0, "whole waves");
 };
 
+#ifndef AMSS_LDS_FENCE
+#define AMSS_LDS_FENCE 1
+#endif
+#ifndef AMSS_LDS_PAIRS
+#define AMSS_LDS_PAIRS 1
+#endif
+// LDS tile buffers per workgroup: two pairs (two field groups per barrier)
+// or two single buffers.
+constexpr int kLdsBuffers = AMSS_LDS_PAIRS ? 4 : 2;
+
 template <int TX, int TY, int TZ, class FI = FieldsIn>
 struct LdsEnv {
+  static constexpr bool kFence = AMSS_LDS_FENCE;
   using T = LdsTile<TX, TY, TZ>;
   static constexpr long SY = T::LX, SZ = (long)T::LX * T::LY;
 
@@ -81,16 +92,21 @@ struct LdsEnv {
   // Groups are staged two at a time: groups 2q and 2q+1 share pair q & 1,
   // so a tile needs one barrier per two field groups.
   __device__ __forceinline__ lds_double* slot(int grp) const {
+    if (!AMSS_LDS_PAIRS) return (grp & 1) ? buf1 : buf0;
     return (((grp >> 1) & 1) ? buf1 : buf0) + (grp & 1) * T::NPAD;
   }
   __device__ __forceinline__ lds_double* cur() const { return slot(g - 1); }  // g already advanced
 
   __device__ __forceinline__ void prologue() {
     issue(group_field(0), slot(0));
-    issue(group_field(1), slot(1));
+    if (AMSS_LDS_PAIRS) issue(group_field(1), slot(1));
   }
   __device__ __forceinline__ void begin(int v) {
-    if ((g & 1) == 0) {
+    if (!AMSS_LDS_PAIRS) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (g + 1 < NFIELD) issue(group_field(g + 1), slot(g + 1));
+    } else if ((g & 1) == 0) {
       __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA of groups g, g+1 (and everything else) landed
       __syncthreads();                // ... for every wave; and the readers of groups g-2, g-1 are done
       if (g + 2 < NFIELD) {

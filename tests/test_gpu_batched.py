@@ -152,3 +152,20 @@ def test_batched_nan_input_is_reported(mode):
     lv.set_state(1, st)
     assert lv.step(cfg.dt, 1, mode) == 1
     lv.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batched_odd_row_pitch_matches_per_patch(mode):
+    """Patches whose row pitch is odd (15 and 23 points): all remainder, direct kernel."""
+    import amssbssn as A
+    n, gw = 17, 3
+    h = 16.0 / n
+    lo = -8.0 - gw * h
+    left = A.PatchGeom((9 + 2 * gw, n + 2 * gw, n + 2 * gw), (lo, lo, lo, -8.0 + (9 + gw) * h, 8.0 + gw * h, 8.0 + gw * h), gw)
+    right = A.PatchGeom((8 + 2 * gw, n + 2 * gw, n + 2 * gw),
+                        (-8.0 + (9 - gw) * h, lo, lo, 8.0 + gw * h, 8.0 + gw * h, 8.0 + gw * h), gw)
+    cfg = A.Config("odd split", [left, right], dt=0.25 * h, steps=2,
+                   punctures=[(0.5, 1.0, 0.5, 0.0), (0.5, -1.5, 0.0, 0.3)], a_noise=1e-6)
+    ref, _ = _run(cfg, A.MODE_PLAIN, 2)
+    got, _ = _run(cfg, mode, 2)
+    _compare(got, ref, 1e-12)

@@ -283,3 +283,12 @@ def test_c1_128cubed_100_steps_vs_reference_vectors(variant):
     worst = _vs_gold(lv.get_state(0), g, 1e-9)
     lv.close()
     print(f"C1 128^3, 100 steps vs reference: worst {worst:.2e}")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_odd_row_pitch_matches_oracle(orc, variant):
+    """A 17^3 interior (23 points per axis with ghosts): an odd row pitch, which
+    the LDS path hands to the direct kernel (its 16-byte loads need an even one)."""
+    import amssbssn as A
+    cfg = A.config1(n=17)
+    _level_vs_oracle(orc, cfg, 0, 0, 2, 1e-12, variant)
