@@ -148,12 +148,13 @@ class BssnLevel {
   std::vector<TileRef> htiles_[kNumTileShapes];
   TileRef* dtiles_[kNumTileShapes] = {};
   std::vector<std::pair<int, Box>> rem_[kNumTileShapes];
-  std::vector<FillBox> hfill_;
-  FillBox* dfill_ = nullptr;
-  long fill_max_ = 0;
-  std::vector<SegRef> hseg_;
-  SegRef* dseg_ = nullptr;
-  long seg_max_ = 0;
+  // ghost boxes (fill boxes, then exchange boxes) and their work list: fill
+  // entries [0, nfill_work_), exchange entries after them
+  std::vector<GhostBox> hgbox_;
+  GhostBox* dgbox_ = nullptr;
+  GhostWork* dgwork_ = nullptr;
+  int nfill_work_ = 0, nseg_work_ = 0;
+  bool ghost_fused_ = true;  // no exchange source overlaps a fill box: one launch
   long maxn_ = 0;
   int ncu_ = 256;
   BatchTune tune_{0, 0, 1};

@@ -19,6 +19,82 @@ static const char* kNames[NFIELD] = {"phi",  "trK",  "gxx", "gxy",  "gxz", "gyy"
 static const char* kListSfx[5] = {"0", "o", "", "1", "_rhs"};
 static const char* kConsNames[7] = {"Cons_Ham", "Cons_Px", "Cons_Py", "Cons_Pz", "Cons_Gx", "Cons_Gy", "Cons_Gz"};
 
+namespace {
+
+// Half-open box of array indices.
+struct IBox {
+  int lo[3], hi[3];
+  bool intersects(const IBox& o) const {
+    for (int a = 0; a < 3; ++a)
+      if (std::max(lo[a], o.lo[a]) >= std::min(hi[a], o.hi[a])) return false;
+    return true;
+  }
+};
+
+// The points of the disjoint boxes `keep` that no box of `cut` contains, as
+// disjoint boxes: the cell grid of all box faces, marked, then merged greedily
+// along x, y and z.
+void subtract_boxes(const std::vector<IBox>& keep, const std::vector<IBox>& cut, std::vector<IBox>& out) {
+  out.clear();
+  if (keep.empty()) return;
+  std::vector<int> c[3];
+  for (int a = 0; a < 3; ++a) {
+    for (const IBox& b : keep) {
+      c[a].push_back(b.lo[a]);
+      c[a].push_back(b.hi[a]);
+    }
+    for (const IBox& b : cut) {
+      c[a].push_back(b.lo[a]);
+      c[a].push_back(b.hi[a]);
+    }
+    std::sort(c[a].begin(), c[a].end());
+    c[a].erase(std::unique(c[a].begin(), c[a].end()), c[a].end());
+  }
+  const int n0 = (int)c[0].size() - 1, n1 = (int)c[1].size() - 1, n2 = (int)c[2].size() - 1;
+  auto index = [&](const std::vector<int>& v, int x) {
+    return (int)(std::lower_bound(v.begin(), v.end(), x) - v.begin());
+  };
+  std::vector<char> on((size_t)n0 * n1 * n2, 0);
+  auto at = [&](int i, int j, int k) -> char& { return on[i + (size_t)n0 * (j + (size_t)n1 * k)]; };
+  auto mark = [&](const IBox& b, char val) {
+    const int i0 = index(c[0], b.lo[0]), i1 = index(c[0], b.hi[0]);
+    const int j0 = index(c[1], b.lo[1]), j1 = index(c[1], b.hi[1]);
+    const int k0 = index(c[2], b.lo[2]), k1 = index(c[2], b.hi[2]);
+    for (int k = k0; k < k1; ++k)
+      for (int j = j0; j < j1; ++j)
+        for (int i = i0; i < i1; ++i) at(i, j, k) = val;
+  };
+  for (const IBox& b : keep) mark(b, 1);
+  for (const IBox& b : cut) mark(b, 0);
+  for (int k = 0; k < n2; ++k)
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n0; ++i) {
+        if (!at(i, j, k)) continue;
+        int i1 = i + 1;
+        while (i1 < n0 && at(i1, j, k)) ++i1;
+        auto row = [&](int jj, int kk) {
+          for (int x = i; x < i1; ++x)
+            if (!at(x, jj, kk)) return false;
+          return true;
+        };
+        int j1 = j + 1;
+        while (j1 < n1 && row(j1, k)) ++j1;
+        int k1 = k + 1;
+        while (k1 < n2) {
+          bool ok = true;
+          for (int y = j; y < j1 && ok; ++y) ok = row(y, k1);
+          if (!ok) break;
+          ++k1;
+        }
+        for (int z = k; z < k1; ++z)
+          for (int y = j; y < j1; ++y)
+            for (int x = i; x < i1; ++x) at(x, y, z) = 0;
+        out.push_back(IBox{{c[0][i], c[1][j], c[2][k]}, {c[0][i1], c[1][j1], c[2][k1]}});
+      }
+}
+
+}  // namespace
+
 void Block::swapList(MyList<var>* a, MyList<var>* b) {
   while (a && b) {
     std::swap(fgfs[a->data->sgfn], fgfs[b->data->sgfn]);
@@ -165,8 +241,8 @@ BssnLevel::~BssnLevel() {
   if (dpd_) (void)hipFree(dpd_);
   for (TileRef* t : dtiles_)
     if (t) (void)hipFree(t);
-  if (dfill_) (void)hipFree(dfill_);
-  if (dseg_) (void)hipFree(dseg_);
+  if (dgbox_) (void)hipFree(dgbox_);
+  if (dgwork_) (void)hipFree(dgwork_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -416,9 +492,9 @@ void BssnLevel::build_batch_tables() {
     }
   }
   // boundary boxes: the points outside [w.lo, n - w.hi) of each patch, as at
-  // most 6 disjoint slabs (z faces whole, then y faces, then x faces)
-  hfill_.clear();
-  fill_max_ = 0;
+  // most 6 disjoint slabs (z faces whole, then y faces, then x faces), less
+  // the points an exchange box overwrites
+  std::vector<std::vector<IBox>> fill(np), cover(np);
   if (bc_ != AMSS_BC_NONE) {
     for (int pi = 0; pi < np; ++pi) {
       const Block& b = *blocks_[pi];
@@ -435,53 +511,79 @@ void BssnLevel::build_batch_tables() {
       int lo[3] = {0, 0, 0}, hi[3] = {b.shape[0], b.shape[1], b.shape[2]};
       for (int a = 2; a >= 0; --a) {
         for (int side = 0; side < 2; ++side) {
-          FillBox f;
-          f.patch = pi;
+          IBox f;
           for (int c = 0; c < 3; ++c) {
             f.lo[c] = lo[c];
-            f.ext[c] = hi[c] - lo[c];
+            f.hi[c] = hi[c];
           }
-          if (side == 0) {
-            f.ext[a] = wl[a];
-          } else {
-            f.lo[a] = b.shape[a] - wh[a];
-            f.ext[a] = wh[a];
-          }
-          if (f.ext[0] > 0 && f.ext[1] > 0 && f.ext[2] > 0) {
-            hfill_.push_back(f);
-            fill_max_ = std::max(fill_max_, (long)f.ext[0] * f.ext[1] * f.ext[2]);
-          }
+          if (side == 0) f.hi[a] = wl[a];
+          else f.lo[a] = b.shape[a] - wh[a];
+          if (f.hi[0] > f.lo[0] && f.hi[1] > f.lo[1] && f.hi[2] > f.lo[2]) fill[pi].push_back(f);
         }
         lo[a] = wl[a];
         hi[a] = b.shape[a] - wh[a];
       }
     }
   }
-  hseg_.clear();
-  seg_max_ = 0;
+  hgbox_.clear();
+  std::vector<GhostBox> segbox;
+  ghost_fused_ = true;
   for (const GhostSeg& g : segs_) {
-    SegRef r;
+    GhostBox r;
+    r.kind = 1;
     r.src = g.src;
     r.dst = g.dst;
+    IBox d, src;
     for (int a = 0; a < 3; ++a) {
       r.slo[a] = g.slo[a];
       r.dlo[a] = g.dlo[a];
       r.ext[a] = g.ext[a];
+      d.lo[a] = g.dlo[a];
+      d.hi[a] = g.dlo[a] + g.ext[a];
+      src.lo[a] = g.slo[a];
+      src.hi[a] = g.slo[a] + g.ext[a];
     }
-    hseg_.push_back(r);
-    seg_max_ = std::max(seg_max_, (long)g.ext[0] * g.ext[1] * g.ext[2]);
+    cover[g.dst].push_back(d);
+    for (const IBox& f : fill[g.src])
+      if (f.intersects(src)) ghost_fused_ = false;  // the copy must read the filled values
+    segbox.push_back(r);
   }
-  if (hfill_.size() > 65535 || hseg_.size() > 65535 || blocks_.size() > 65535) {
-    err_ = "too many boundary or exchange boxes";
+  for (int pi = 0; pi < np; ++pi) {
+    std::vector<IBox> left;
+    subtract_boxes(fill[pi], cover[pi], left);
+    for (const IBox& f : left) {
+      GhostBox r{};
+      r.kind = 0;
+      r.src = r.dst = pi;
+      for (int a = 0; a < 3; ++a) {
+        r.dlo[a] = f.lo[a];
+        r.ext[a] = f.hi[a] - f.lo[a];
+      }
+      hgbox_.push_back(r);
+    }
+  }
+  const int nfillbox = (int)hgbox_.size();
+  hgbox_.insert(hgbox_.end(), segbox.begin(), segbox.end());
+  std::vector<GhostWork> work;
+  for (int bi = 0; bi < (int)hgbox_.size(); ++bi) {
+    const GhostBox& r = hgbox_[bi];
+    const long npts = (long)r.ext[0] * r.ext[1] * r.ext[2];  // < 2^29: inside one Block
+    for (long t = 0; t < npts; t += 256) work.push_back(GhostWork{bi, (int)t});
+    if (bi == nfillbox - 1) nfill_work_ = (int)work.size();
+  }
+  if (nfillbox == 0) nfill_work_ = 0;
+  nseg_work_ = (int)work.size() - nfill_work_;
+  if (work.size() > (size_t)INT32_MAX) {
+    err_ = "too many ghost points";
     return;
   }
-  if (!hfill_.empty()) {
-    LCHK(hipMalloc(&dfill_, hfill_.size() * sizeof(FillBox)));
-    LCHK(hipMemcpy(dfill_, hfill_.data(), hfill_.size() * sizeof(FillBox), hipMemcpyHostToDevice));
+  if (!hgbox_.empty()) {
+    LCHK(hipMalloc(&dgbox_, hgbox_.size() * sizeof(GhostBox)));
+    LCHK(hipMemcpy(dgbox_, hgbox_.data(), hgbox_.size() * sizeof(GhostBox), hipMemcpyHostToDevice));
   }
-  if (!hseg_.empty()) {
-    LCHK(hipMalloc(&dseg_, hseg_.size() * sizeof(SegRef)));
-    LCHK(hipMemcpy(dseg_, hseg_.data(), hseg_.size() * sizeof(SegRef), hipMemcpyHostToDevice));
+  if (!work.empty()) {
+    LCHK(hipMalloc(&dgwork_, work.size() * sizeof(GhostWork)));
+    LCHK(hipMemcpy(dgwork_, work.data(), work.size() * sizeof(GhostWork), hipMemcpyHostToDevice));
   }
 }
 
@@ -531,9 +633,18 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
                    coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false);
       }
     }
-    launch_fill_boxes(dpd_, dfill_, (int)hfill_.size(), (int)std::min(fill_max_, 1L << 30), m, it, tinny, dflag_,
-                      stream_);
-    launch_seg_boxes(dpd_, dseg_, (int)hseg_.size(), (int)std::min(seg_max_, 1L << 30), m.out, stream_);
+    // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap after
+    // sub-step 1), whose fill points already hold the same values when
+    // nothing else writes ghost points (frozen ghosts, no edge kernel).  The
+    // state is the same in all sub-steps, so sub-step 0 tests it for NaN at
+    // the exchange points as well and the later sub-steps need not.
+    const bool fill = !(it == 2 && bc_ == AMSS_BC_FROZEN_GHOSTS && gw_ >= 3);
+    if (ghost_fused_ && fill) {
+      launch_ghost_boxes(dpd_, dgbox_, dgwork_, nfill_work_ + nseg_work_, m, it, tinny, it == 0, dflag_, stream_);
+    } else {
+      if (fill) launch_ghost_boxes(dpd_, dgbox_, dgwork_, nfill_work_, m, it, tinny, false, dflag_, stream_);
+      launch_ghost_boxes(dpd_, dgbox_, dgwork_ + nfill_work_, nseg_work_, m, it, tinny, it == 0, dflag_, stream_);
+    }
     if (it > 0 && it < 3)
       for (Block* b : blocks_) b->swapList(SynchList_pre, SynchList_cor);
   }

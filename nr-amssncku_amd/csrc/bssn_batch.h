@@ -42,6 +42,20 @@ struct SegRef {
   int slo[3], dlo[3], ext[3];
 };
 
+// Ghost work of the batched step, as disjoint boxes of destination points:
+// kind 0 sets points of patch dst from its state u0 (the frozen or copied
+// boundary: chi's lower bound, enforce_ga for sub-steps 0-2), kind 1 copies
+// patch src's box at slo of the sub-step output into patch dst's box at dlo
+// (the exchange).  Fill boxes exclude the points an exchange box overwrites.
+struct GhostBox {
+  int kind, src, dst;
+  int slo[3], dlo[3], ext[3];
+};
+// Points [start, start + 256) of box `box` (one workgroup each).
+struct GhostWork {
+  int box, start;
+};
+
 // Tile shapes of the batched kernel (points per tile along x, y, z).
 constexpr int kNumTileShapes = 4;
 __host__ __device__ constexpr int tile_dim(int shape, int a) {
@@ -80,9 +94,11 @@ AMSS_DECL_TILES(2, 1)
 AMSS_DECL_TILES(2, 2)
 AMSS_DECL_TILES(2, 3)
 #undef AMSS_DECL_TILES
-// Frozen / copied boundary points of every patch (boxes of points of u0 -> out).
-void launch_fill_boxes(const PatchDesc* P, const FillBox* B, int nbox, int maxpts, const Roles& m, int stage,
-                       double tinny, int* flag, hipStream_t s);
+// Boundary fill and exchange of a sub-step over the work list W (nwork
+// entries); check_u0 also tests the state's values at exchange points for
+// NaN (the fill tests them at its own points).
+void launch_ghost_boxes(const PatchDesc* P, const GhostBox* B, const GhostWork* W, int nwork, const Roles& m,
+                        int stage, double tinny, bool check_u0, int* flag, hipStream_t s);
 // Inter-patch ghost exchange of list slot `slot`.
 void launch_seg_boxes(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, int slot, hipStream_t s);
 // Reverse of launch_seg_boxes (slot 0 of P): each ghost-box value's adjoint is

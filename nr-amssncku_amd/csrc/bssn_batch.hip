@@ -1,5 +1,5 @@
-// bssn_batch.hip -- box-table kernels of the batched level step (ghost fill,
-// inter-patch exchange, enforce_ga over all patches) and the dispatch of the
+// bssn_batch.hip -- box-table kernels of the batched level step (ghost fill
+// and inter-patch exchange, enforce_ga over all patches) and the dispatch of the
 // tile-list kernel by gauge and tile shape.
 #include "bssn_batch.h"
 
@@ -7,48 +7,55 @@ namespace amss {
 
 namespace {
 
-template <class FI>
-__device__ __forceinline__ bool bad_point(const FI& F, long p) {
-  // the reference's NaN test covers the first 21 fields (bssn_rhs.f90:110-113)
-  bool b = false;
-#pragma unroll
-  for (int v = 0; v < 21; ++v) {
-    const double x = F.ptr(v)[p];
-    b |= !(x == x);
-  }
-  return b;
-}
-
-// Boundary points after a sub-step: the state's values (frozen ghost zones or
-// the one-layer copy of sommerfeld_rout precor = 1), with chi bounded below
-// and, for sub-steps 0-2, enforce_ga applied -- the same per-point work as the
-// per-patch ghost fill.  blockIdx.y selects the box.
-__global__ void __launch_bounds__(256) fill_boxes_kernel(const PatchDesc* __restrict__ P,
-                                                         const FillBox* __restrict__ B, Roles m, int stage,
-                                                         double tinny, int* flag) {
-  const FillBox b = B[blockIdx.y];
-  const PatchDesc d = P[b.patch];
+// Boundary fill and exchange of one sub-step; blockIdx.x is one work entry
+// (256 points of one box, so the box kind is uniform per workgroup).  Every
+// field is loaded before any is stored, so a wave keeps all its loads in
+// flight.  Fill points take the state's values with chi bounded below and,
+// for sub-steps 0-2, enforce_ga -- the per-point work of the per-patch ghost
+// fill; exchange points take the source patch's sub-step output.
+__global__ void __launch_bounds__(256) ghost_boxes_kernel(const PatchDesc* __restrict__ P,
+                                                          const GhostBox* __restrict__ B,
+                                                          const GhostWork* __restrict__ W, Roles m, int stage,
+                                                          double tinny, int check_u0, int* flag) {
+  const GhostWork w = W[blockIdx.x];
+  const GhostBox b = B[w.box];
+  const unsigned t = (unsigned)w.start + threadIdx.x;
+  const unsigned e01 = (unsigned)b.ext[0] * b.ext[1];
+  if (t >= e01 * b.ext[2]) return;
+  const int x = (int)(t % b.ext[0]), y = (int)((t / b.ext[0]) % b.ext[1]), z = (int)(t / e01);
+  const PatchDesc d = P[b.dst];
+  const long pd = (b.dlo[0] + x) + (long)(b.dlo[1] + y) * d.sy + (long)(b.dlo[2] + z) * d.sz;
   const long ln = (long)NFIELD * d.n;
-  const SlabFields u0{(gdouble*)d.slab + m.u0 * ln, d.n}, out{(gdouble*)d.slab + m.out * ln, d.n};
-  const long npts = (long)b.ext[0] * b.ext[1] * b.ext[2];
-  for (long t = blockIdx.x * 256L + threadIdx.x; t < npts; t += gridDim.x * 256L) {
-    const int i = b.lo[0] + (int)(t % b.ext[0]);
-    const int j = b.lo[1] + (int)((t / b.ext[0]) % b.ext[1]);
-    const int k = b.lo[2] + (int)(t / ((long)b.ext[0] * b.ext[1]));
-    const long p = i + (long)j * d.sy + (long)k * d.sz;
-    if (bad_point(u0, p)) *flag = 1;
-    double ga[12];
+  const gdouble* u0 = (const gdouble*)d.slab + m.u0 * ln;
+  double u[NFIELD];
+  if (b.kind == 0) {
 #pragma unroll
-    for (int s = 0; s < 12; ++s) ga[s] = u0.ptr(DXX + s)[p];
-    if (stage < 3) enforce_point(ga, ga + 6);
+    for (int v = 0; v < NFIELD; ++v) u[v] = u0[v * d.n + pd];
+    bool bad = false;
 #pragma unroll
-    for (int s = 0; s < 12; ++s) out.ptr(DXX + s)[p] = ga[s];
-    const double c = u0.ptr(CHI)[p];
-    out.ptr(CHI)[p] = c < tinny ? tinny : c;
-    out.ptr(TRK)[p] = u0.ptr(TRK)[p];
+    for (int v = 0; v < 21; ++v) bad |= !(u[v] == u[v]);  // bssn_rhs.f90:110-113
+    if (bad) *flag = 1;
+    if (stage < 3) enforce_point(u + DXX, u + AXX);
+    u[CHI] = u[CHI] < tinny ? tinny : u[CHI];
+  } else {
+    const PatchDesc a = P[b.src];
+    const long ps = (b.slo[0] + x) + (long)(b.slo[1] + y) * a.sy + (long)(b.slo[2] + z) * a.sz;
+    const gdouble* src = (const gdouble*)a.slab + m.out * (long)NFIELD * a.n;
 #pragma unroll
-    for (int v = GAMX; v < NFIELD; ++v) out.ptr(v)[p] = u0.ptr(v)[p];
+    for (int v = 0; v < NFIELD; ++v) u[v] = src[v * a.n + ps];
+    if (check_u0) {
+      bool bad = false;
+#pragma unroll
+      for (int v = 0; v < 21; ++v) {
+        const double c = u0[v * d.n + pd];
+        bad |= !(c == c);
+      }
+      if (bad) *flag = 1;
+    }
   }
+  gdouble* out = (gdouble*)d.slab + m.out * ln;
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) out[v * d.n + pd] = u[v];
 }
 
 // Ghost exchange between patches: all 24 fields of list slot `slot`,
@@ -65,8 +72,11 @@ __global__ void __launch_bounds__(256) seg_boxes_kernel(const PatchDesc* __restr
     const int z = (int)(t / ((long)g.ext[0] * g.ext[1]));
     const long ps = (g.slo[0] + x) + (long)(g.slo[1] + y) * a.sy + (long)(g.slo[2] + z) * a.sz;
     const long pd = (g.dlo[0] + x) + (long)(g.dlo[1] + y) * b.sy + (long)(g.dlo[2] + z) * b.sz;
+    double u[NFIELD];
 #pragma unroll
-    for (int v = 0; v < NFIELD; ++v) dst[v * b.n + pd] = src[v * a.n + ps];
+    for (int v = 0; v < NFIELD; ++v) u[v] = src[v * a.n + ps];
+#pragma unroll
+    for (int v = 0; v < NFIELD; ++v) dst[v * b.n + pd] = u[v];
   }
 }
 
@@ -126,9 +136,9 @@ void launch_rk4_tiles(int gauge, const PatchDesc* P, const TileRef* T, int t0, i
 #undef AMSS_CASE
 }
 
-void launch_fill_boxes(const PatchDesc* P, const FillBox* B, int nbox, int maxpts, const Roles& m, int stage,
-                       double tinny, int* flag, hipStream_t s) {
-  if (nbox > 0) fill_boxes_kernel<<<dim3(xblocks(maxpts), nbox), 256, 0, s>>>(P, B, m, stage, tinny, flag);
+void launch_ghost_boxes(const PatchDesc* P, const GhostBox* B, const GhostWork* W, int nwork, const Roles& m,
+                        int stage, double tinny, bool check_u0, int* flag, hipStream_t s) {
+  if (nwork > 0) ghost_boxes_kernel<<<nwork, 256, 0, s>>>(P, B, W, m, stage, tinny, check_u0 ? 1 : 0, flag);
 }
 
 void launch_seg_boxes(const PatchDesc* P, const SegRef* S, int nseg, int maxpts, int slot, hipStream_t s) {

@@ -5,6 +5,8 @@
 //  This file holds the small pointwise kernels and the dispatch:
 //  * ghost_fill : the boundary fill of a stage output; seg_copy: ghost exchange.
 //  * enforce, lowerbound, rk4_update, copy_box, l2 partial sums.
+#include <algorithm>
+
 #include "bssn_kernels_impl.h"
 
 namespace amss {
@@ -15,29 +17,51 @@ namespace amss {
 // next RHS applies to its input -- exactly the sequence rk4 -> copy ->
 // lowerboundset -> enforce_ga the reference runs on such points
 // (bssn_class.C:3119-3156, 3415-3525).
-__global__ void ghost_fill_kernel(FieldsIn u0, FieldsOut out, Box box, Faces w, int stage, double tinny, int* flag) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long N = (long)box.n[0] * box.n[1] * box.n[2];
-  if (t >= N) return;
-  const int i = t % box.n[0];
-  const int j = (t / box.n[0]) % box.n[1];
-  const int k = t / ((long)box.n[0] * box.n[1]);
-  if (i >= w.lo[0] && i < box.n[0] - w.hi[0] && j >= w.lo[1] && j < box.n[1] - w.hi[1] && k >= w.lo[2] &&
-      k < box.n[2] - w.hi[2])
-    return;
-  const long p = i + j * box.sy + k * box.sz;
-  if (bad_input(u0, p)) *flag = 1;
-  double ga[12];
+// The shell outside [w.lo, n - w.hi) is cut into at most 6 disjoint slabs
+// (both z faces whole, then the y faces of what is left, then the x faces);
+// blockIdx.y picks the slab.  Every field is loaded before any is stored, so a
+// wave has all its loads in flight at once.
+__device__ __forceinline__ bool face_slab(const Box& box, const Faces& w, int f, int lo[3], int ext[3]) {
+  int l[3] = {0, 0, 0}, h[3] = {box.n[0], box.n[1], box.n[2]};
+  for (int a = 2; a >= 0; --a) {
+    for (int side = 0; side < 2; ++side, --f) {
+      if (f != 0) continue;
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = l[c];
+        ext[c] = h[c] - l[c];
+      }
+      lo[a] = side == 0 ? 0 : box.n[a] - w.hi[a];
+      ext[a] = side == 0 ? w.lo[a] : w.hi[a];
+      return ext[0] > 0 && ext[1] > 0 && ext[2] > 0;
+    }
+    l[a] = w.lo[a];
+    h[a] = box.n[a] - w.hi[a];
+  }
+  return false;
+}
+
+__global__ void __launch_bounds__(256) ghost_fill_kernel(FieldsIn u0, FieldsOut out, Box box, Faces w, int stage,
+                                                         double tinny, int* flag) {
+  int lo[3], ext[3];
+  if (!face_slab(box, w, blockIdx.y, lo, ext)) return;
+  const unsigned npts = (unsigned)ext[0] * ext[1] * ext[2];
+  for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < npts; t += gridDim.x * 256u) {
+    const int i = lo[0] + (int)(t % ext[0]);
+    const int j = lo[1] + (int)((t / ext[0]) % ext[1]);
+    const int k = lo[2] + (int)(t / ((unsigned)ext[0] * ext[1]));
+    const long p = i + j * box.sy + k * box.sz;
+    double u[NFIELD];
 #pragma unroll
-  for (int s = 0; s < 12; ++s) ga[s] = u0.p[DXX + s][p];
-  if (stage < 3) enforce_point(ga, ga + 6);
+    for (int v = 0; v < NFIELD; ++v) u[v] = u0.p[v][p];
+    bool bad = false;
 #pragma unroll
-  for (int s = 0; s < 12; ++s) out.p[DXX + s][p] = ga[s];
-  const double c = u0.p[CHI][p];
-  out.p[CHI][p] = c < tinny ? tinny : c;
-  out.p[TRK][p] = u0.p[TRK][p];
+    for (int v = 0; v < 21; ++v) bad |= !(u[v] == u[v]);  // bssn_rhs.f90:110-113
+    if (bad) *flag = 1;
+    if (stage < 3) enforce_point(u + DXX, u + AXX);
+    u[CHI] = u[CHI] < tinny ? tinny : u[CHI];
 #pragma unroll
-  for (int v = GAMX; v < NFIELD; ++v) out.p[v][p] = u0.p[v][p];
+    for (int v = 0; v < NFIELD; ++v) out.p[v][p] = u[v];
+  }
 }
 
 // Ghost exchange between patches of one level: copy all 24 fields of a box
@@ -50,8 +74,11 @@ __global__ void seg_copy_kernel(FieldsIn src, FieldsOut dst, SegDesc d) {
   const int a = t % d.ext[0], b = (t / d.ext[0]) % d.ext[1], c = t / ((long)d.ext[0] * d.ext[1]);
   const long ps = (d.slo[0] + a) + (d.slo[1] + b) * d.ssy + (d.slo[2] + c) * d.ssz;
   const long pd = (d.dlo[0] + a) + (d.dlo[1] + b) * d.dsy + (d.dlo[2] + c) * d.dsz;
+  double u[NFIELD];
 #pragma unroll
-  for (int v = 0; v < NFIELD; ++v) dst.p[v][pd] = src.p[v][ps];
+  for (int v = 0; v < NFIELD; ++v) u[v] = src.p[v][ps];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) dst.p[v][pd] = u[v];
 }
 
 __global__ void enforce_kernel(FieldsOut f, long n) {
@@ -140,8 +167,13 @@ void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, co
 
 void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,
                        double tinny, int* flag, hipStream_t s) {
-  const long N = (long)box.n[0] * box.n[1] * box.n[2];
-  ghost_fill_kernel<<<blocks_for(N, 256), 256, 0, s>>>(u0, out, box, w, stage, tinny, flag);
+  // slabs are grid-strided; size the grid for the largest possible one (the
+  // kernel's 32-bit indexing holds: a Block has fewer than 2^29 points)
+  int wmax = 0;
+  for (int a = 0; a < 3; ++a) wmax = std::max(wmax, std::max(w.lo[a], w.hi[a]));
+  const long face = std::max({(long)box.n[0] * box.n[1], (long)box.n[0] * box.n[2], (long)box.n[1] * box.n[2]});
+  const unsigned gx = (unsigned)std::min(std::max(blocks_for(face * wmax, 256), 1u), 2048u);
+  if (wmax > 0) ghost_fill_kernel<<<dim3(gx, 6), 256, 0, s>>>(u0, out, box, w, stage, tinny, flag);
 }
 
 void launch_seg_copy(const FieldsIn& src, const FieldsOut& dst, const SegDesc& d, hipStream_t s) {

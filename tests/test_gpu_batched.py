@@ -154,6 +154,27 @@ def test_batched_nan_input_is_reported(mode):
     lv.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("x", [1, 12])
+def test_nan_in_a_ghost_point_is_reported(mode, x):
+    """A NaN in the state at a ghost point: x = 1 is a boundary ghost of the
+    left patch (filled), x = 12 one the exchange overwrites with the right
+    patch's values.  The reference tests every point of an RHS input."""
+    import amssbssn as A
+    n, gw = 16, 3
+    h = 16.0 / n
+    lo = -8.0 - gw * h
+    left = A.PatchGeom((8 + 2 * gw, n + 2 * gw, n + 2 * gw), (lo, lo, lo, gw * h, 8.0 + gw * h, 8.0 + gw * h), gw)
+    right = A.PatchGeom((8 + 2 * gw, n + 2 * gw, n + 2 * gw), (-gw * h, lo, lo, 8.0 + gw * h, 8.0 + gw * h, 8.0 + gw * h), gw)
+    cfg = A.Config("split", [left, right], dt=0.25 * h, steps=1, punctures=[(0.5, 1.0, 0.5, 0.0)], a_noise=1e-6)
+    lv = A.Level.from_config(cfg)
+    st = cfg.initial_state(0)
+    st[5][10, 10, x] = np.nan
+    lv.set_state(0, st)
+    assert lv.step(cfg.dt, 1, mode) == 1
+    lv.close()
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_batched_odd_row_pitch_matches_per_patch(mode):
     """Patches whose row pitch is odd (15 and 23 points): all remainder, direct kernel."""
