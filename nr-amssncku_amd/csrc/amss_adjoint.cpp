@@ -14,17 +14,20 @@ namespace amss {
 
 namespace {
 // work arrays of a Block (in grid functions of N doubles)
-constexpr int kY0 = 0;                     // enforced U^n (24)
-constexpr int kY = kY0 + NFIELD;           // Y1..Y3, the inputs of sub-steps 1-3 (3 x 24)
-constexpr int kZ = kY + 3 * NFIELD;        // z0..z3, sub-step outputs before bound/enforce (4 x 24)
-constexpr int kStage = kZ + 4 * NFIELD;    // grid functions of one step's stage data
-constexpr int kK = kZ + 4 * NFIELD;        // k of the current sub-step (24)
-constexpr int kAcc = kK + NFIELD;          // RK4 accumulator (24)
-constexpr int kBars = kAcc + NFIELD;       // feature adjoints (kAdjFeatures)
-constexpr int kLY = kBars + kAdjFeatures;  // U^{n+1} scratch (forward); dJ/dY of the sub-step being reversed
-constexpr int kLY0 = kLY + NFIELD;         // dJ/dY0 (24)
-constexpr int kLK = kLY0 + NFIELD;         // dJ/dk0..k3 (4 x 24)
-constexpr int kFields = kLK + 4 * NFIELD;
+constexpr int kY0 = 0;                   // enforced U^n (24)
+constexpr int kY = kY0 + NFIELD;         // Y1..Y3, the inputs of sub-steps 1-3 (3 x 24)
+constexpr int kZ = kY + 3 * NFIELD;      // z0..z3, sub-step outputs before bound/enforce (4 x 24)
+constexpr int kStage = kZ + 4 * NFIELD;  // grid functions of one step's stage data
+constexpr int kK = kZ + 4 * NFIELD;      // k of the current sub-step (24)
+constexpr int kAcc = kK + NFIELD;        // RK4 accumulator (24)
+constexpr int kLY = kAcc + NFIELD;       // U^{n+1} scratch (forward); dJ/dY of the sub-step being reversed
+constexpr int kLY0 = kLY + NFIELD;       // dJ/dY0 (24)
+constexpr int kFields = kLY0 + NFIELD;
+// padded arrays (3 zero planes before and after each, never written outside
+// the interior, so the gather reads them at any stencil offset unchecked)
+constexpr int kBars = 0;                   // feature adjoints (kAdjFeatures)
+constexpr int kLK = kBars + kAdjFeatures;  // dJ/dk0..k3 (4 x 24)
+constexpr int kPadFields = kLK + 4 * NFIELD;
 }  // namespace
 
 AdjointBlock::AdjointBlock(const int ex[3], const Coef& C, double tinny) : C_(C), tinny_(tinny) {
@@ -40,15 +43,26 @@ AdjointBlock::AdjointBlock(const int ex[3], const Coef& C, double tinny) : C_(C)
   box_.sy = ex[0];
   box_.sz = (long)ex[0] * ex[1];
   N_ = (long)ex[0] * ex[1] * ex[2];
-  const hipError_t e = hipMalloc(&buf_, (size_t)kFields * N_ * sizeof(double));
+  hipError_t e = hipMalloc(&buf_, (size_t)kFields * N_ * sizeof(double));
   if (e != hipSuccess) {
     buf_ = nullptr;
+    err_ = std::string("hipMalloc of the adjoint work arrays: ") + hipGetErrorString(e);
+    return;
+  }
+  pad_ = 3 * box_.sz;
+  NP_ = N_ + 2 * pad_;
+  e = hipMalloc(&padbuf_, (size_t)kPadFields * NP_ * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(padbuf_, 0, (size_t)kPadFields * NP_ * sizeof(double));
+  if (e != hipSuccess) {
+    if (padbuf_) (void)hipFree(padbuf_);
+    padbuf_ = nullptr;
     err_ = std::string("hipMalloc of the adjoint work arrays: ") + hipGetErrorString(e);
   }
 }
 
 AdjointBlock::~AdjointBlock() {
   if (buf_) (void)hipFree(buf_);
+  if (padbuf_) (void)hipFree(padbuf_);
   if (tape_) (void)hipFree(tape_);
 }
 
@@ -71,6 +85,17 @@ int AdjointBlock::reserve(int slots) {
 double* AdjointBlock::base_of(int first, int slot) const {
   if (first < kStage && slot >= 0) return tape_ + ((long)slot * kStage + first) * N_;
   return buf_ + (long)first * N_;
+}
+double* AdjointBlock::padded(int first) const { return padbuf_ + (long)first * NP_ + pad_; }
+FieldsIn AdjointBlock::pin(int first) const {
+  FieldsIn f;
+  for (int v = 0; v < NFIELD; ++v) f.p[v] = padded(first + v);
+  return f;
+}
+FieldsOut AdjointBlock::pout(int first) const {
+  FieldsOut f;
+  for (int v = 0; v < NFIELD; ++v) f.p[v] = padded(first + v);
+  return f;
 }
 FieldsIn AdjointBlock::fin(int first) const {
   FieldsIn f;
@@ -119,23 +144,22 @@ void AdjointBlock::fwd_stage(int st, double c, double* const* Unext, hipStream_t
 }
 
 void AdjointBlock::rev_final(const double* const* lam, double c, hipStream_t s) {
-  const FieldsOut lk[4] = {fout(kLK), fout(kLK + NFIELD), fout(kLK + 2 * NFIELD), fout(kLK + 3 * NFIELD)};
+  const FieldsOut lk[4] = {pout(kLK), pout(kLK + NFIELD), pout(kLK + 2 * NFIELD), pout(kLK + 3 * NFIELD)};
   adj_launch_final(as_in(lam), fin(kZ + 3 * NFIELD), fin(kY0), fout(kLY0), lk, box_, c, tinny_, s);
 }
 
 void AdjointBlock::rev_rhs(int st, hipStream_t s) {
-  double* bars = buf_ + (long)kBars * N_;
   const FieldsIn Ys = st == 0 ? fin(kY0) : fin(kY + (st - 1) * NFIELD);
-  const FieldsIn lks = fin(kLK + st * NFIELD);
-  adj_launch_point(Ys, lks, bars, box_, C_, s);
+  const FieldsIn lks = pin(kLK + st * NFIELD);
+  adj_launch_point(Ys, lks, padded(kBars), NP_, box_, C_, s);
   if (st == 0)
-    adj_launch_gather(Ys, lks, bars, fout(kLY0), box_, C_, true, s);
+    adj_launch_gather(Ys, lks, true, padded(kBars), NP_, fout(kLY0), box_, C_, true, s);
   else
-    adj_launch_gather(Ys, lks, bars, fout(kLY), box_, C_, false, s);
+    adj_launch_gather(Ys, lks, true, padded(kBars), NP_, fout(kLY), box_, C_, false, s);
 }
 
 void AdjointBlock::rev_stage(int st, double c, hipStream_t s) {
-  adj_launch_stage(fin(kLY), fin(kZ + (st - 1) * NFIELD), fin(kY0), fout(kLY0), fout(kLK + (st - 1) * NFIELD),
+  adj_launch_stage(fin(kLY), fin(kZ + (st - 1) * NFIELD), fin(kY0), fout(kLY0), pout(kLK + (st - 1) * NFIELD),
                    box_, c, tinny_, s);
 }
 
@@ -144,9 +168,8 @@ void AdjointBlock::rev_open(const double* const* U, double* const* lam, hipStrea
 }
 
 void AdjointBlock::rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s) {
-  double* bars = buf_ + (long)kBars * N_;
-  adj_launch_point(as_in(U), as_in(lam), bars, box_, C_, s);
-  adj_launch_gather(as_in(U), as_in(lam), bars, as_out(out), box_, C_, false, s);
+  adj_launch_point(as_in(U), as_in(lam), padded(kBars), NP_, box_, C_, s);
+  adj_launch_gather(as_in(U), as_in(lam), false, padded(kBars), NP_, as_out(out), box_, C_, false, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -284,7 +307,9 @@ LevelAdjoint::~LevelAdjoint() {
 size_t LevelAdjoint::bytes() const {
   size_t n = 0;
   const int slots = ((int)dtab_.size() - 4) / 3;
-  for (AdjointBlock* b : blocks_) n += (size_t)(kFields + (size_t)slots * kStage) * b->npts() * sizeof(double);
+  for (AdjointBlock* b : blocks_)
+    n += ((size_t)(kFields + (size_t)slots * kStage) * b->npts() + (size_t)kPadFields * b->padded_npts()) *
+         sizeof(double);
   return n;
 }
 

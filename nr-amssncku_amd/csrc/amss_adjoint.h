@@ -24,6 +24,7 @@ class AdjointBlock {
   bool ok() const { return err_.empty(); }
   const std::string& err() const { return err_; }
   long npts() const { return N_; }
+  long padded_npts() const { return NP_; }
   const Box& box() const { return box_; }
 
   // Stage data (Y0..Y3, z0..z3: 192 grid functions) live in the work array
@@ -48,11 +49,16 @@ class AdjointBlock {
  private:
   FieldsIn fin(int first) const;
   FieldsOut fout(int first) const;
+  double* padded(int first) const;  // element 0 of padded array `first`
+  FieldsIn pin(int first) const;
+  FieldsOut pout(int first) const;
   Box box_;
   Coef C_;
   double tinny_;
   long N_ = 0;
   double* buf_ = nullptr;
+  double* padbuf_ = nullptr;  // padded arrays: NP_ = N_ + 2 pad_ doubles each, pad_ = 3 planes
+  long pad_ = 0, NP_ = 0;
   double* tape_ = nullptr;
   int slots_ = 0;
   int slot_ = -1;

@@ -36,93 +36,103 @@ __global__ void __launch_bounds__(kRhsThreads) rhs_k_kernel(FieldsIn in, FieldsO
   bssn_rhs_point<0, false, false, false>(E, S);
 }
 
-__global__ void __launch_bounds__(256) adj_point_kernel(FieldsIn in, FieldsIn lam, double* bars, Box box, Coef C) {
+__global__ void __launch_bounds__(256) adj_point_kernel(FieldsIn in, FieldsIn lam, double* bars, long stride, Box box,
+                                                        Coef C) {
   int i, j, k;
   if (!interior_index(box, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   InteriorEnv E{in, in, C, p, box.sy, box.sz};
   double l[NFIELD];
+#pragma unroll
   for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
-  const AdjSink O{bars, (long)box.n[0] * box.n[1] * box.n[2], p};
+  const AdjSink O{bars, stride, p};
   bssn_rhs_adjoint_point(E, l, O);
 }
 
+// Loads of one array at q + o for uniform offsets o in [-3 sz, 3 sz]: a
+// buffer resource based 3 planes before element 0, the thread's q in the
+// vector offset and o in the scalar one (one instruction per load).
+struct Taps {
+  __amdgpu_buffer_rsrc_t r;
+  unsigned vo;
+  int bias;
+  __device__ __forceinline__ Taps(const double* f, long q, int sz3)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)(f - sz3), (short)0, (int)0xFFFFFFFF, 0x00020000)),
+        vo((unsigned)(q * 8)),
+        bias(sz3) {}
+  __device__ __forceinline__ double operator()(int o) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, (unsigned)((o + bias) * 8), 0));
+  }
+};
+
 // dJ/dU(q) = value bar + sum over the output points p whose stencils read q
-// of (stencil weight) x (feature bar at p); bars and lam exist at interior
-// points only.  blockIdx.y = field.
+// of (stencil weight) x (feature bar at p).  The bars are zero outside the
+// interior and readable 3 planes beyond the array, so they are read at every
+// offset unchecked; lam likewise when LAMPAD (else lam, and beta with it, is
+// read only at interior points).  blockIdx.y = field; a workgroup is 256
+// consecutive points (one thread per point per field beat both one thread per
+// point for all fields and 16 x 4 x 4 bricks).
+template <bool LAMPAD>
 __global__ void __launch_bounds__(256) adj_gather_kernel(FieldsIn in, FieldsIn lam, const double* __restrict__ bars,
-                                                         FieldsOut out, Box box, Coef C, int accumulate) {
+                                                         long bs, FieldsOut out, Box box, Coef C, int accumulate) {
   const long N = (long)box.n[0] * box.n[1] * box.n[2];
   const long q = blockIdx.x * 256L + threadIdx.x;
   if (q >= N) return;
   const int v = blockIdx.y;
   const int i = (int)(q % box.n[0]), j = (int)((q / box.n[0]) % box.n[1]), k = (int)(q / ((long)box.n[0] * box.n[1]));
-  auto rd = [&](const double* f, int di, int dj, int dk) -> double {
-    return inside(box, i + di, j + dj, k + dk) ? f[q + di + dj * box.sy + dk * box.sz] : 0.0;
-  };
-  auto sh = [](int a, int o, int& di, int& dj, int& dk) {
-    di = a == 0 ? o : 0;
-    dj = a == 1 ? o : 0;
-    dk = a == 2 ? o : 0;
-  };
-  double acc = inside(box, i, j, k) ? bars[v * N + q] : 0.0;
+  const int sy = (int)box.sy, sz = (int)box.sz, sz3 = 3 * sz;
+  const int st[3] = {1, sy, sz};
+  double acc = bars[v * bs + q];
   const int gs = adj_grad_slot(v);
   if (gs >= 0) {
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const double* b = bars + adj_grad_index(gs, a) * N;
-      int d[4][3];
-      const int os[4] = {2, 1, -1, -2};
-      for (int t = 0; t < 4; ++t) sh(a, os[t], d[t][0], d[t][1], d[t][2]);
-      acc += C.d1[a] * ((rd(b, d[0][0], d[0][1], d[0][2]) - rd(b, d[3][0], d[3][1], d[3][2])) +
-                        8.0 * (rd(b, d[2][0], d[2][1], d[2][2]) - rd(b, d[1][0], d[1][1], d[1][2])));
+      const Taps b(bars + adj_grad_index(gs, a) * bs, q, sz3);
+      const int s = st[a];
+      acc += C.d1[a] * ((b(2 * s) - b(-2 * s)) + 8.0 * (b(-s) - b(s)));
     }
   }
   const int hs = adj_hess_slot(v);
   if (hs >= 0) {
-    const int ta[6] = {0, 0, 0, 1, 1, 2}, tb[6] = {0, 1, 2, 1, 2, 2};
-    for (int t = 0; t < 6; ++t) {
-      const double* b = bars + adj_hess_index(hs, t) * N;
-      const int a = ta[t], c = tb[t];
-      if (a == c) {
-        int d1[3], d2[3], d3[3], d4[3];
-        sh(a, -1, d1[0], d1[1], d1[2]);
-        sh(a, 1, d2[0], d2[1], d2[2]);
-        sh(a, -2, d3[0], d3[1], d3[2]);
-        sh(a, 2, d4[0], d4[1], d4[2]);
-        acc += C.d2[a] * (16.0 * (rd(b, d1[0], d1[1], d1[2]) + rd(b, d2[0], d2[1], d2[2])) -
-                          (rd(b, d3[0], d3[1], d3[2]) + rd(b, d4[0], d4[1], d4[2])) - 30.0 * rd(b, 0, 0, 0));
-      } else {
-        const double dm = t == 1 ? C.dm[0] : (t == 2 ? C.dm[1] : C.dm[2]);
-        const int os[4] = {-2, -1, 1, 2};
-        const double w[4] = {1.0, -8.0, 8.0, -1.0};  // d1 weights at offsets -2, -1, 1, 2
-        double s = 0.0;
-        for (int x = 0; x < 4; ++x)
-          for (int y = 0; y < 4; ++y) {
-            int di = 0, dj = 0, dk = 0;
-            if (a == 0) di += os[x];
-            if (a == 1) dj += os[x];
-            if (c == 1) dj += os[y];
-            if (c == 2) dk += os[y];
-            s += w[x] * w[y] * rd(b, di, dj, dk);
-          }
-        acc += dm * s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {  // diagonal second derivatives
+      const Taps b(bars + adj_hess_index(hs, a == 0 ? 0 : (a == 1 ? 3 : 5)) * bs, q, sz3);
+      const int s = st[a];
+      acc += C.d2[a] * (16.0 * (b(-s) + b(s)) - (b(-2 * s) + b(2 * s)) - 30.0 * b(0));
+    }
+    const int ta[3] = {0, 0, 1}, tb[3] = {1, 2, 2}, tt[3] = {1, 2, 4};
+    constexpr int os[4] = {-2, -1, 1, 2};
+    constexpr double w[4] = {1.0, -8.0, 8.0, -1.0};  // d1 weights at offsets -2, -1, 1, 2
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {  // mixed: the 4 x 4 cross stencil is its own transpose
+      const Taps b(bars + adj_hess_index(hs, tt[m]) * bs, q, sz3);
+      const int sa = st[ta[m]], sb = st[tb[m]];
+      double s = 0.0;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        double r = 0.0;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) r += w[y] * b(os[x] * sa + os[y] * sb);
+        s += w[x] * r;
       }
+      acc += C.dm[m] * s;
     }
   }
   // advection and KO: both read lam at the output points p = q - o e_a
   // (the KO stencil is symmetric, so its transpose uses the same points)
-  const double* L = lam.p[v];
+  constexpr double up[7] = {0.0, 0.0, -3.0, -10.0, 18.0, -6.0, 1.0};  // offsets -3..3 from p
+  constexpr double kw[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double* beta = in.p[BETAX + a];
-    const double up[7] = {0.0, 0.0, -3.0, -10.0, 18.0, -6.0, 1.0};  // offsets -3..3 from p
-    const double kw[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+    const Taps L(lam.p[v], q, sz3), B(in.p[BETAX + a], q, sz3);
     double s = 0.0, t = 0.0;
+#pragma unroll
     for (int o = -3; o <= 3; ++o) {
-      int di, dj, dk;
-      sh(a, -o, di, dj, dk);
-      if (!inside(box, i + di, j + dj, k + dk)) continue;
-      const long pp = q + di + dj * box.sy + dk * box.sz;
-      const double b = beta[pp], l = L[pp];
+      if (!LAMPAD) {
+        const int pi = i - (a == 0 ? o : 0), pj = j - (a == 1 ? o : 0), pk = k - (a == 2 ? o : 0);
+        if (!inside(box, pi, pj, pk)) continue;
+      }
+      const double b = B(-o * st[a]), l = L(-o * st[a]);
       // up weight at offset o; the downwind stencil is its mirror
       s += up[o + 3] * fmax(b, 0.0) * l - up[3 - o] * fmin(b, 0.0) * l;
       t += kw[o + 3] * l;
@@ -290,14 +300,18 @@ unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, const Coef& C, hipStream_t s) {
   rhs_k_kernel<<<interior_grid(box), kRhsThreads, 0, s>>>(in, out, box, C);
 }
-void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, const Box& box, const Coef& C,
-                      hipStream_t s) {
-  adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, box, C);
+void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, long stride, const Box& box,
+                      const Coef& C, hipStream_t s) {
+  adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, stride, box, C);
 }
-void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, const double* bars, const FieldsOut& out,
-                       const Box& box, const Coef& C, bool accumulate, hipStream_t s) {
+void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, bool lam_padded, const double* bars, long stride,
+                       const FieldsOut& out, const Box& box, const Coef& C, bool accumulate, hipStream_t s) {
   const long N = (long)box.n[0] * box.n[1] * box.n[2];
-  adj_gather_kernel<<<dim3(nblk(N), NFIELD), 256, 0, s>>>(in, lam, bars, out, box, C, accumulate ? 1 : 0);
+  const dim3 g(nblk(N), NFIELD);
+  if (lam_padded)
+    adj_gather_kernel<true><<<g, 256, 0, s>>>(in, lam, bars, stride, out, box, C, accumulate ? 1 : 0);
+  else
+    adj_gather_kernel<false><<<g, 256, 0, s>>>(in, lam, bars, stride, out, box, C, accumulate ? 1 : 0);
 }
 void adj_launch_fwd_stage(const FieldsIn& Y0, const FieldsIn& k, const FieldsOut& acc, const FieldsOut& z,
                           const FieldsOut& Yn, const Box& box, int stage, double c, double tinny, hipStream_t s) {

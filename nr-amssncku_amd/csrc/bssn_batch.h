@@ -32,11 +32,7 @@ struct Roles {
 };
 constexpr int kResidualBase = 5 * NFIELD;  // first residual grid function of a slab
 
-// A box of points of one patch (ghost fill) or a copy between patches.
-struct FillBox {
-  int patch;
-  int lo[3], ext[3];
-};
+// A copy of a box between patches (the level adjoint's exchange).
 struct SegRef {
   int src, dst;
   int slo[3], dlo[3], ext[3];

@@ -22,13 +22,14 @@ __device__ __forceinline__ void tile_body(const PatchDesc* __restrict__ P, const
   const SlabFields in{slab + m.in * ln, d.n}, u0{slab + m.u0 * ln, d.n};
   const SlabFields acc{slab + m.acc * ln, d.n}, out{slab + m.out * ln, d.n};
   const SlabAux aux{slab + kResidualBase * d.n, d.n};
-  LdsEnv<TX, TY, TZ, SlabFields> E{in, in, C, b0, b0 + (kLdsBuffers / 2) * Tl::NPAD, p, ltv,
-                                   (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz, d.sy, d.sz, (int)d.n, tid};
-  E.prologue();
-  Rk4SinkT<STAGE, SlabFields, SlabFields, SlabAux> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
-  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
-  S.finish();
-  if (E.bad) *flag = 1;
+  LdsRk4<STAGE, TX, TY, TZ, SlabFields, SlabFields, SlabAux> X{
+      {in, in, C, b0, b0 + (kLdsBuffers / 2) * Tl::NPAD, p, ltv, (r.i0 - 3) + (r.j0 - 3) * d.sy + (r.k0 - 3) * d.sz,
+       d.sy, d.sz, (int)d.n, tid},
+      {u0, acc, out, aux, p, R.c, R.tinny, {}}};
+  X.prologue();
+  bssn_rhs_point<GAUGE, CO, false, false>(X, X);
+  X.finish();
+  if (X.bad) *flag = 1;
 }
 
 // PERSIST: a grid of resident workgroups strides over the tile list.  The

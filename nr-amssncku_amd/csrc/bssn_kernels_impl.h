@@ -100,15 +100,14 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
   const long p = i + j * box.sy + k * box.sz;
   const int N = box.n[0] * box.n[1] * box.n[2];
   lds_double* t0 = (lds_double*)tiles;
-  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p,
-                                  (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
-                                  (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N,
-                                  (int)threadIdx.x};
-  E.prologue();
-  Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
-  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
-  S.finish();
-  if (E.bad) *flag = 1;
+  LdsRk4<STAGE, kLdsTX, kLdsTY, kLdsTZ> X{
+      {in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p, (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
+       (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N, (int)threadIdx.x},
+      {u0, acc, out, aux, p, R.c, R.tinny, {}}};
+  X.prologue();
+  bssn_rhs_point<GAUGE, CO, false, false>(X, X);
+  X.finish();
+  if (X.bad) *flag = 1;
 }
 
 template <int GAUGE, bool CO>

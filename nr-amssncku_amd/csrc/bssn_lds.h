@@ -8,9 +8,14 @@
 // group g computes from one, the halo tile of group g+1 streams into the
 // other with gfx950's direct global->LDS loads (global_load_lds_dwordx4:
 // each lane moves 16 bytes, a wave fills a contiguous 1 KiB run starting at
-// the wave-uniform LDS base), so staging costs no VGPRs.  Centre values that
-// the algebra reads outside a group come from global memory.
+// the wave-uniform LDS base), so staging costs no VGPRs.  The centre value of
+// the open group's field comes from its tile; the other centre values a pair
+// reads (and the RK4 epilogue's U0 and accumulator values of its fields) are
+// loaded one pair ahead, so the barrier's full wait settles them and no
+// later wait for them stalls on the halo loads in flight.
 #pragma once
+#include <type_traits>
+
 #include "bssn_env.h"
 
 namespace amss {
@@ -21,6 +26,12 @@ __host__ __device__ constexpr int group_field(int g) {
                                        DZZ,   GAMX,  GAMY,  GAMZ, CHI, LAP, TRK, DTSFX,
                                        DTSFY, DTSFZ, AXX,   AXY,  AXZ, AYY, AYZ, AZZ};
   return seq[g];
+}
+
+// The centre value of another field that the point algebra reads inside a
+// group (the shift rhs reads B^i, the metric rhs A_ij), or -1.
+__host__ __device__ constexpr int group_extra(int v) {
+  return v >= BETAX && v <= BETAX + 2 ? DTSFX + (v - BETAX) : (v >= DXX && v <= DZZ ? AXX + (v - DXX) : -1);
 }
 
 // Pointers into LDS (32-bit local addresses; generic pointers would make
@@ -49,7 +60,9 @@ struct LdsTile {
 // or two single buffers.
 constexpr int kLdsBuffers = AMSS_LDS_PAIRS ? 4 : 2;
 
-template <int TX, int TY, int TZ, class FI = FieldsIn>
+// Self: a type deriving from this environment and from the RK4 epilogue,
+// whose prefetch(v0, v1) the environment calls for each pair (void: none).
+template <int TX, int TY, int TZ, class FI = FieldsIn, class Self = void>
 struct LdsEnv {
   static constexpr bool kFence = AMSS_LDS_FENCE;
   using T = LdsTile<TX, TY, TZ>;
@@ -68,6 +81,8 @@ struct LdsEnv {
   int tid;        // thread index in the workgroup
   int g = 0;      // next group to open
   bool bad = false;  // a NaN among the centre values of the first 21 fields (bssn_rhs.f90:110-113)
+  int xf[2] = {-1, -1}, nxf[2] = {-1, -1};  // group_extra of the open (next) pair's fields
+  double xv[2], nxv[2];                      // ... and their centre values, loaded a pair ahead
 
   // Stream the halo tile of field v into buffer b: every lane issues ROUNDS
   // 16-byte direct global->LDS loads.  The buffer is padded to whole rounds,
@@ -97,7 +112,29 @@ struct LdsEnv {
   }
   __device__ __forceinline__ lds_double* cur() const { return slot(g - 1); }  // g already advanced
 
+  // Centre values a pair needs from global memory, loaded one pair ahead
+  // (into the n* slots; rotate makes them current at the pair's barrier).
+  __device__ __forceinline__ void ahead(int g0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      nxf[h] = group_extra(group_field(g0 + h));
+      if (nxf[h] >= 0) nxv[h] = F.ptr(nxf[h])[p];
+    }
+    if constexpr (!std::is_void_v<Self>) static_cast<Self*>(this)->prefetch(group_field(g0), group_field(g0 + 1));
+  }
+  __device__ __forceinline__ void rotate() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      xf[h] = nxf[h];
+      xv[h] = nxv[h];
+      nxf[h] = -1;
+    }
+    if constexpr (!std::is_void_v<Self>) static_cast<Self*>(this)->sink_rotate();
+  }
+
   __device__ __forceinline__ void prologue() {
+    if (AMSS_LDS_PAIRS) ahead(0);
+    __builtin_amdgcn_sched_barrier(0);
     issue(group_field(0), slot(0));
     if (AMSS_LDS_PAIRS) issue(group_field(1), slot(1));
   }
@@ -109,6 +146,9 @@ struct LdsEnv {
     } else if ((g & 1) == 0) {
       __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA of groups g, g+1 (and everything else) landed
       __syncthreads();                // ... for every wave; and the readers of groups g-2, g-1 are done
+      rotate();
+      if (g + 2 < NFIELD) ahead(g + 2);
+      __builtin_amdgcn_sched_barrier(0);
       if (g + 2 < NFIELD) {
         issue(group_field(g + 2), slot(g + 2));
         issue(group_field(g + 3), slot(g + 3));
@@ -123,7 +163,14 @@ struct LdsEnv {
   __device__ __forceinline__ void end() const {}
 
   __device__ __forceinline__ double at(int, long o) const { return cur()[lt + o]; }
-  __device__ __forceinline__ double val(int v) const { return F.ptr(v)[p]; }
+  // the centre value of the open group's field is in its halo tile; the
+  // pair's extra centre values were loaded at its barrier
+  __device__ __forceinline__ double val(int v) const {
+    if (g > 0 && v == group_field(g - 1)) return cur()[lt];
+    if (v == xf[0]) return xv[0];
+    if (v == xf[1]) return xv[1];
+    return F.ptr(v)[p];
+  }
   __device__ __forceinline__ double mat(int m) const { return M.ptr(m)[p]; }
 
   __device__ __forceinline__ double d1(int v, long s) const {
@@ -166,6 +213,14 @@ struct LdsEnv {
     const double c20 = 20.0 * at(v, 0);
     return C.ko[0] * ko1(v, 1, c20) + C.ko[1] * ko1(v, SY, c20) + C.ko[2] * ko1(v, SZ, c20);
   }
+};
+
+// The LDS environment and the RK4 epilogue as one object (passed to the point
+// algebra as both), so the environment can hand the epilogue its prefetches.
+template <int STAGE, int TX, int TY, int TZ, class FI = FieldsIn, class FO = FieldsOut, class AX = AuxOut>
+struct LdsRk4 : LdsEnv<TX, TY, TZ, FI, LdsRk4<STAGE, TX, TY, TZ, FI, FO, AX>>, Rk4SinkT<STAGE, FI, FO, AX> {
+  __device__ __forceinline__ void sink_rotate() { Rk4SinkT<STAGE, FI, FO, AX>::rotate(); }
+  __device__ __forceinline__ void prefetch(int v0, int v1) { Rk4SinkT<STAGE, FI, FO, AX>::prefetch(v0, v1); }
 };
 
 }  // namespace amss
