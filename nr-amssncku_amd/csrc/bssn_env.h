@@ -334,14 +334,11 @@ struct Rk4SinkT {
   // U0 and ACC at p of the fields of the LDS environment's open pair of
   // groups (p*) and of the next pair (n*), loaded one pair ahead: the
   // barrier's full wait has settled them before they are used, so the
-  // epilogue never waits for memory while the next halo tiles stream in.
-  // The Gamma-tilde and trK rhs are stored after later groups: their values
-  // are kept from their own pair (d*).  (Scalars and constant indices only:
-  // an index that is not a constant keeps this object in memory.)
+  // epilogue does not wait for memory while the next halo tiles stream in.
+  // (The Gamma-tilde and trK rhs, stored in later pairs, load theirs then:
+  // keeping them live longer costs more in spills than it saves.)
   int pv[2] = {-1, -1}, nv[2] = {-1, -1};
   double pu[2], pa[2], nu[2], na[2];
-  bool dg0 = false, dg1 = false, dg2 = false, dk = false;
-  double dgu0, dgu1, dgu2, dku, dga0, dga1, dga2, dka;
 
   __device__ __forceinline__ void rotate() {
 #pragma unroll
@@ -352,39 +349,21 @@ struct Rk4SinkT {
       nv[h] = -1;
     }
   }
+  // v0, v1: the fields of the pair whose values these are
   __device__ __forceinline__ void prefetch(int v0, int v1) {
     const int v[2] = {v0, v1};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const double u = U0.ptr(v[h])[p];
-      const double a = STAGE > 0 ? (double)ACC.ptr(v[h])[p] : 0.0;
-      if (v[h] == GAMX) {
-        dg0 = true, dgu0 = u, dga0 = a;
-      } else if (v[h] == GAMY) {
-        dg1 = true, dgu1 = u, dga1 = a;
-      } else if (v[h] == GAMZ) {
-        dg2 = true, dgu2 = u, dga2 = a;
-      } else if (v[h] == TRK) {
-        dk = true, dku = u, dka = a;
-      } else {
-        nv[h] = v[h];
-        nu[h] = u;
-        na[h] = a;
-      }
+      if ((v[h] >= GAMX && v[h] <= GAMZ) || v[h] == TRK) continue;
+      nv[h] = v[h];
+      nu[h] = U0.ptr(v[h])[p];
+      na[h] = STAGE > 0 ? (double)ACC.ptr(v[h])[p] : 0.0;
     }
   }
 
   __device__ __forceinline__ void put(int v, double k) {
     double u0v, av = 0.0;
-    if (v == GAMX && dg0) {
-      u0v = dgu0, av = dga0;
-    } else if (v == GAMY && dg1) {
-      u0v = dgu1, av = dga1;
-    } else if (v == GAMZ && dg2) {
-      u0v = dgu2, av = dga2;
-    } else if (v == TRK && dk) {
-      u0v = dku, av = dka;
-    } else if (v == pv[0]) {
+    if (v == pv[0]) {
       u0v = pu[0], av = pa[0];
     } else if (v == pv[1]) {
       u0v = pu[1], av = pa[1];
