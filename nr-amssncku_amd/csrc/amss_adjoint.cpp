@@ -22,7 +22,8 @@ constexpr int kK = kZ + 4 * NFIELD;      // k of the current sub-step (24)
 constexpr int kAcc = kK + NFIELD;        // RK4 accumulator (24)
 constexpr int kLY = kAcc + NFIELD;       // U^{n+1} scratch (forward); dJ/dY of the sub-step being reversed
 constexpr int kLY0 = kLY + NFIELD;       // dJ/dY0 (24)
-constexpr int kFields = kLY0 + NFIELD;
+constexpr int kFeat = kLY0 + NFIELD;     // stencil values of the reverse algebra (kAdjInputs)
+constexpr int kFields = kFeat + kAdjInputs;
 // padded arrays (3 zero planes before and after each, never written outside
 // the interior, so the gather reads them at any stencil offset unchecked)
 constexpr int kBars = 0;                   // feature adjoints (kAdjFeatures)
@@ -151,7 +152,7 @@ void AdjointBlock::rev_final(const double* const* lam, double c, hipStream_t s) 
 void AdjointBlock::rev_rhs(int st, hipStream_t s) {
   const FieldsIn Ys = st == 0 ? fin(kY0) : fin(kY + (st - 1) * NFIELD);
   const FieldsIn lks = pin(kLK + st * NFIELD);
-  adj_launch_point(Ys, lks, padded(kBars), NP_, box_, C_, s);
+  adj_launch_point(Ys, lks, padded(kBars), NP_, box_, C_, s, buf_ + (long)kFeat * N_);
   if (st == 0)
     adj_launch_gather(Ys, lks, true, padded(kBars), NP_, fout(kLY0), box_, C_, true, s);
   else
@@ -168,7 +169,7 @@ void AdjointBlock::rev_open(const double* const* U, double* const* lam, hipStrea
 }
 
 void AdjointBlock::rhs_vjp(const double* const* U, const double* const* lam, double* const* out, hipStream_t s) {
-  adj_launch_point(as_in(U), as_in(lam), padded(kBars), NP_, box_, C_, s);
+  adj_launch_point(as_in(U), as_in(lam), padded(kBars), NP_, box_, C_, s, buf_ + (long)kFeat * N_);
   adj_launch_gather(as_in(U), as_in(lam), false, padded(kBars), NP_, as_out(out), box_, C_, false, s);
 }
 

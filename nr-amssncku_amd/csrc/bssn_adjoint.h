@@ -58,19 +58,15 @@ __device__ __forceinline__ double upwind_dbeta(double b, double up, double dn) {
   return (b >= 0.0 ? up : 0.0) - (b <= 0.0 ? dn : 0.0);
 }
 
+// sum_v lam_v d/d beta^a (the upwind term of v), without the 1/(12 h) factor
 template <class Env>
-__device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const AdjSink& O) {
-  constexpr double F1o3 = 1.0 / 3.0, F2o3 = 2.0 / 3.0;
-  constexpr double W[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};
-  auto S2 = [](int i, int j) { return sidx(i, j); };
-
-  // upwind stencils' dependence on beta^a(p), first: it needs only beta and
-  // the stencils, and leaves 3 numbers live
-  double ub[3] = {0.0, 0.0, 0.0};
-  {
+__device__ __forceinline__ void upwind_beta_sums(const Env& E, const double* lam, double ub[3]) {
+  if constexpr (Env::kFeatures) {
+    E.ubeta(ub);
+  } else {
     double bp[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) bp[a] = E.val(BETAX + a);
+    for (int a = 0; a < 3; ++a) bp[a] = E.val(BETAX + a), ub[a] = 0.0;
 #pragma unroll
     for (int v = 0; v < NFIELD; ++v) {
       double up[3], dn[3];
@@ -79,6 +75,41 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       for (int a = 0; a < 3; ++a) ub[a] += lam[v] * upwind_dbeta(bp[a], up[a], dn[a]);
     }
   }
+}
+
+// The stencil values bssn_rhs_adjoint_point reads, precomputed per point
+// (feature f of point p at f * n + p): centre values (24), gradients (15 x 3,
+// scaled), Hessians (11 x 6, scaled), upwind_beta_sums (3).
+constexpr int kAdjInputs = NFIELD + 3 * kAdjGradFields + 6 * kAdjHessFields + 3;  // 138
+struct FeatureEnv {
+  static constexpr bool kFence = true, kFeatures = true;
+  const double* f;
+  long n, p;
+  const Coef& C;
+  __device__ __forceinline__ double val(int v) const { return f[v * n + p]; }
+  __device__ __forceinline__ void grad(int v, double d[3]) const {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d[a] = f[adj_grad_index(adj_grad_slot(v), a) * n + p];
+  }
+  __device__ __forceinline__ void hess(int v, double h[6]) const {
+#pragma unroll
+    for (int t = 0; t < 6; ++t) h[t] = f[adj_hess_index(adj_hess_slot(v), t) * n + p];
+  }
+  __device__ __forceinline__ void ubeta(double ub[3]) const {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ub[a] = f[(kAdjFeatures + a) * n + p];
+  }
+};
+
+template <class Env>
+__device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const AdjSink& O) {
+  constexpr double F1o3 = 1.0 / 3.0, F2o3 = 2.0 / 3.0;
+  constexpr double W[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};
+  auto S2 = [](int i, int j) { return sidx(i, j); };
+
+  // upwind stencils' dependence on beta^a(p), first: it leaves 3 numbers live
+  double ub[3];
+  upwind_beta_sums(E, lam, ub);
 
   AMSS_FENCE();
   // ======================= forward recomputation ===========================

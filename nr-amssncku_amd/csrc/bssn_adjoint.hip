@@ -49,6 +49,117 @@ __global__ void __launch_bounds__(256) adj_point_kernel(FieldsIn in, FieldsIn la
   bssn_rhs_adjoint_point(E, l, O);
 }
 
+// Every stencil value the reverse algebra reads, at the interior points: a
+// light kernel (high occupancy hides the stencil loads' latency), so the
+// algebra kernel, which runs at one wave per SIMD, reads plain arrays.
+__global__ void __launch_bounds__(256) adj_feature_kernel(FieldsIn in, FieldsIn lam, double* feat, long fs, Box box,
+                                                          Coef C) {
+  int i, j, k;
+  if (!interior_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  InteriorEnv E{in, in, C, p, box.sy, box.sz};
+  for (int v = 0; v < NFIELD; ++v) {
+    feat[v * fs + p] = E.val(v);
+    const int gs = adj_grad_slot(v), hs = adj_hess_slot(v);
+    if (gs >= 0) {
+      double d[3];
+      E.grad(v, d);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) feat[adj_grad_index(gs, a) * fs + p] = d[a];
+    }
+    if (hs >= 0) {
+      double h[6];
+      E.hess(v, h);
+#pragma unroll
+      for (int t = 0; t < 6; ++t) feat[adj_hess_index(hs, t) * fs + p] = h[t];
+    }
+  }
+  double l[NFIELD], ub[3];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
+  upwind_beta_sums(E, l, ub);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) feat[(kAdjFeatures + a) * fs + p] = ub[a];
+}
+
+// The same on 8x8x4 tiles with every field's halo tile staged in LDS (the
+// forward kernel's environment: direct global->LDS loads, two field groups
+// per barrier); tiles in XCD-contiguous order.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+adj_feature_lds_kernel(FieldsIn in, FieldsIn lam, double* feat, long fs, Box box, Coef C) {
+  using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
+  __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
+  const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
+  const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
+  const int b = xcd_spread(blockIdx.x, gridDim.x);
+  const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;
+  const int k0 = box.lo[2] + (b / (gx * gy)) * kLdsTZ;
+  const long p = (i0 + tx) + (long)(j0 + ty) * box.sy + (long)(k0 + tz) * box.sz;
+  const int N = box.n[0] * box.n[1] * box.n[2];
+  lds_double* t0 = (lds_double*)tiles;
+  // lam and beta at the point, loaded before the first halo loads (so no wait
+  // for them stalls on those): every field's upwind term needs all three beta
+  double l[NFIELD], bp[3];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) bp[a] = in.p[BETAX + a][p];
+  __builtin_amdgcn_sched_barrier(0);
+  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p,
+                                  (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
+                                  (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N,
+                                  (int)threadIdx.x};
+  E.prologue();
+  double ub[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int g = 0; g < NFIELD; ++g) {
+    const int v = group_field(g);
+    E.begin(v);
+    feat[v * fs + p] = E.val(v);
+    const int gs = adj_grad_slot(v), hs = adj_hess_slot(v);
+    // (fences between the stencil kinds: issued all at once, a group's LDS
+    // reads would need more registers than there are)
+    if (gs >= 0) {
+      double d[3];
+      E.grad(v, d);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) feat[adj_grad_index(gs, a) * fs + p] = d[a];
+      amss_fence<decltype(E)>();
+    }
+    if (hs >= 0) {
+      double h[6];
+      E.hess(v, h);
+#pragma unroll
+      for (int t = 0; t < 6; ++t) feat[adj_hess_index(hs, t) * fs + p] = h[t];
+      amss_fence<decltype(E)>();
+    }
+    double up[3], dn[3];
+    E.updn(v, up, dn);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ub[a] += l[v] * upwind_dbeta(bp[a], up[a], dn[a]);
+      asm volatile("" : "+v"(ub[a]));  // summed here, not deferred (that keeps 144 stencil values live)
+    }
+    E.end();
+    amss_fence<decltype(E)>();
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) feat[(kAdjFeatures + a) * fs + p] = ub[a];
+}
+
+__global__ void __launch_bounds__(256) adj_algebra_kernel(const double* feat, long fs, FieldsIn lam,
+                                                          double* bars, long stride, Box box, Coef C) {
+  int i, j, k;
+  if (!interior_index(box, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  const FeatureEnv E{feat, fs, p, C};
+  double l[NFIELD];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
+  const AdjSink O{bars, stride, p};
+  bssn_rhs_adjoint_point(E, l, O);
+}
+
 // Loads of one array at q + o for uniform offsets o in [-3 sz, 3 sz]: a
 // buffer resource based 3 planes before element 0, the thread's q in the
 // vector offset and o in the scalar one (one instruction per load).
@@ -301,8 +412,33 @@ void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, 
   rhs_k_kernel<<<interior_grid(box), kRhsThreads, 0, s>>>(in, out, box, C);
 }
 void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, long stride, const Box& box,
-                      const Coef& C, hipStream_t s) {
-  adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, stride, box, C);
+                      const Coef& C, hipStream_t s, double* feat) {
+  if (feat) {
+    const long fs = (long)box.n[0] * box.n[1] * box.n[2];
+    Box core = box;
+    if (box.sy % 2 == 0) {  // 16-byte halo loads need an even row pitch
+      const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
+      for (int a = 0; a < 3; ++a) core.hi[a] = box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a];
+    } else {
+      core.hi[0] = core.lo[0];
+    }
+    if (!box_empty(core)) {
+      const unsigned gr = (unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
+                          ((core.hi[2] - core.lo[2]) / kLdsTZ);
+      adj_feature_lds_kernel<<<gr, 256, 0, s>>>(in, lam, feat, fs, core, C);
+    } else {
+      for (int a = 0; a < 3; ++a) core.hi[a] = core.lo[a];
+    }
+    for (int a = 0; a < 3; ++a) {  // remainder slab [core.hi[a], box.hi[a]) along axis a: direct kernel
+      Box rem = box;
+      for (int c = 0; c < a; ++c) rem.hi[c] = core.hi[c];
+      rem.lo[a] = core.hi[a];
+      if (!box_empty(rem)) adj_feature_kernel<<<interior_grid(rem), 256, 0, s>>>(in, lam, feat, fs, rem, C);
+    }
+    adj_algebra_kernel<<<interior_grid(box), 256, 0, s>>>(feat, fs, lam, bars, stride, box, C);
+  } else {
+    adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, stride, box, C);
+  }
 }
 void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, bool lam_padded, const double* bars, long stride,
                        const FieldsOut& out, const Box& box, const Coef& C, bool accumulate, hipStream_t s) {

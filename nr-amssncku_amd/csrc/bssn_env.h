@@ -15,7 +15,7 @@ namespace amss {
 // stride would cost a 64-bit per-lane address pair.
 // Requires 3 + 3 sy + 3 sz <= p and field size < 4 GiB.
 struct InteriorEnv {
-  static constexpr bool kFence = true;
+  static constexpr bool kFence = true, kFeatures = false;
   const FieldsIn& F;
   const FieldsIn& M;
   const Coef& C;

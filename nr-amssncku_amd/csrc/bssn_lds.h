@@ -60,6 +60,9 @@ struct LdsTile {
 // or two single buffers.
 constexpr int kLdsBuffers = AMSS_LDS_PAIRS ? 4 : 2;
 
+#ifndef AMSS_DMA_AT
+#define AMSS_DMA_AT 1
+#endif
 // Self: a type deriving from this environment and from the RK4 epilogue,
 // whose prefetch(v0, v1) the environment calls for each pair (void: none).
 template <int TX, int TY, int TZ, class FI = FieldsIn, class Self = void>
@@ -150,9 +153,15 @@ struct LdsEnv {
       if (g + 2 < NFIELD) ahead(g + 2);
       __builtin_amdgcn_sched_barrier(0);
       if (g + 2 < NFIELD) {
-        issue(group_field(g + 2), slot(g + 2));
-        issue(group_field(g + 3), slot(g + 3));
+        if (AMSS_DMA_AT < 2) issue(group_field(g + 2), slot(g + 2));
+        if (AMSS_DMA_AT < 1) issue(group_field(g + 3), slot(g + 3));
       }
+    } else if (AMSS_DMA_AT > 0 && g + 1 < NFIELD) {
+      // the next pair's halo loads start with the pair's second group
+      __builtin_amdgcn_sched_barrier(0);
+      if (AMSS_DMA_AT > 1) issue(group_field(g + 1), slot(g + 1));
+      issue(group_field(g + 2), slot(g + 2));
+      __builtin_amdgcn_sched_barrier(0);
     }
     ++g;
     if (v < 21) {
@@ -201,6 +210,17 @@ struct LdsEnv {
     const double up = (at(v, 3 * s) - 3.0 * at(v, -s)) + (18.0 * at(v, s) - 6.0 * at(v, 2 * s)) - f0;
     const double dn = (at(v, -3 * s) - 3.0 * at(v, s)) + (18.0 * at(v, -s) - 6.0 * at(v, -2 * s)) - f0;
     return c * (fmax(b, 0.0) * up - fmin(b, 0.0) * dn);
+  }
+  // the two one-sided stencils of every direction, unscaled (adjoint use)
+  __device__ __forceinline__ void updn(int v, double up[3], double dn[3]) const {
+    const long st[3] = {1, SY, SZ};
+    const double f0 = 10.0 * at(v, 0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const long s = st[a];
+      up[a] = (at(v, 3 * s) - 3.0 * at(v, -s)) + (18.0 * at(v, s) - 6.0 * at(v, 2 * s)) - f0;
+      dn[a] = (at(v, -3 * s) - 3.0 * at(v, s)) + (18.0 * at(v, -s) - 6.0 * at(v, -2 * s)) - f0;
+    }
   }
   __device__ __forceinline__ double adv(int v, double, const double b[3]) const {
     return up1(v, 1, b[0], C.d1[0]) + up1(v, SY, b[1], C.d1[1]) + up1(v, SZ, b[2], C.d1[2]);
