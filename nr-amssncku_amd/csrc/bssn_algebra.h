@@ -21,12 +21,16 @@
 // Sink receives put(v, k) once per evolved field and, when requested, the
 // physical Christoffels (gam), physical Ricci (ric) and residuals (res).
 #pragma once
+#include <type_traits>
 #include "bssn_device.h"
 
 namespace amss {
 
 constexpr double kPi = 3.141592653589793;  // acos(-1), bssn_rhs.f90:140
 
+#ifndef AMSS_PIN
+#define AMSS_PIN 2
+#endif
 // Group fence: the environment decides (Env::kFence) whether the compiler
 // may move work across field groups.
 template <class Env>
@@ -37,6 +41,30 @@ __device__ __forceinline__ void amss_fence() {
   }
 }
 #define AMSS_FENCE() amss_fence<Env>()
+
+// Pin values at a group boundary: sums finished here are not deferred to
+// their use (which would keep every summand live until then).
+template <class Env, int N>
+__device__ __forceinline__ void amss_pin(double (&x)[N]) {
+  if constexpr (Env::kFence && AMSS_PIN) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" : "+v"(x[i]));
+  }
+}
+template <class Env>
+__device__ __forceinline__ void amss_pin1(double& x) {
+  if constexpr (Env::kFence && AMSS_PIN) asm volatile("" : "+v"(x));
+}
+// How many of the pins a sink wants (Sink::kPin if it says; measured per RK4
+// sub-step: the first gains from fewer)
+template <class S, class = void>
+struct PinLevel {
+  static constexpr int v = AMSS_PIN;
+};
+template <class S>
+struct PinLevel<S, std::void_t<decltype(S::kPin)>> {
+  static constexpr int v = S::kPin < AMSS_PIN ? S::kPin : AMSS_PIN;
+};
 
 template <int GAUGE, bool CO, bool GAMRIC, bool MATTER, class Env, class Sink>
 __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
@@ -108,6 +136,9 @@ __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
       // beta^i rhs = 3/4 B^i (:793-795) + advection + KO
       S.put(BETAX + a, 0.75 * E.val(DTSFX + a) + E.adv(BETAX + a, 0.0, beta) + E.ko(BETAX + a));
       E.end();
+      amss_pin<Env>(gdiv);
+      amss_pin<Env>(db[a]);
+      amss_pin1<Env>(lapb[a]);
       AMSS_FENCE();
     }
 #pragma unroll
@@ -139,6 +170,10 @@ __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
       S.put(DXX + s, -2.0 * alp * E.val(AXX + s) + lie - F2o3 * g[s] * divb +
                          E.adv(DXX + s, sdiag(s) ? 1.0 : 0.0, beta) + E.ko(DXX + s));
       E.end();
+      amss_pin1<Env>(dg[0][s]);
+      amss_pin1<Env>(dg[1][s]);
+      amss_pin1<Env>(dg[2][s]);
+      amss_pin1<Env>(R[s]);
       AMSS_FENCE();
     }
     if (CO) {  // Gam^i_Res = Gam^i - gu^ik gu^jl d_j g_kl (:203-232)
@@ -222,6 +257,8 @@ __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
       for (int j = i; j < 3; ++j) R[sidx(i, j)] += 0.5 * (g[sidx(k, i)] * d[j] + g[sidx(k, j)] * d[i]);
     pGt[k] = E.adv(GAMX + k, 0.0, beta) + E.ko(GAMX + k);
     E.end();
+    amss_pin<Env>(R);
+    amss_pin1<Env>(pGt[k]);
     AMSS_FENCE();
   }
 
@@ -270,6 +307,10 @@ __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
           G2[k][s] += d;
         }
   }
+  if (PinLevel<Sink>::v > 1) {
+    amss_pin<Env>(R);
+    amss_pin<Env>(dchi);
+  }
   AMSS_FENCE();
 
   // ---- lapse group: D_i D_j alpha (:629-664), alpha rhs (:791) ---------------
@@ -283,6 +324,10 @@ __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
     E.end();
 #pragma unroll
     for (int s = 0; s < 6; ++s) DDa[s] = h[s] - (G2[0][s] * dalp[0] + G2[1][s] * dalp[1] + G2[2][s] * dalp[2]);
+  }
+  if (PinLevel<Sink>::v > 1) {
+    amss_pin<Env>(DDa);
+    amss_pin<Env>(dalp);
   }
   if (GAMRIC) {
 #pragma unroll
@@ -301,6 +346,10 @@ __device__ __forceinline__ void bssn_rhs_point(Env& E, Sink& S) {
     E.grad(TRK, dK);
     pK = E.adv(TRK, 0.0, beta) + E.ko(TRK);
     E.end();
+  }
+  if (PinLevel<Sink>::v > 1) {
+    amss_pin<Env>(dK);
+    amss_pin1<Env>(pK);
   }
   AMSS_FENCE();
 

@@ -324,6 +324,7 @@ struct Rk4Sink {
 // point computation per stage branch (and blows the register budget).
 template <int STAGE, class FI = FieldsIn, class FO = FieldsOut, class AX = AuxOut>
 struct Rk4SinkT {
+  static constexpr int kPin = STAGE == 0 ? 1 : 2;
   const FI& U0;
   const FO& ACC;
   const FO& OUT;
