@@ -13,6 +13,9 @@
 // A second pass (the gather kernel) applies the transposed stencils to these
 // bars and to lam (advection and KO) to give dJ/dU at every point.
 //
+#ifndef AMSS_ADJ_PIN
+#define AMSS_ADJ_PIN 2
+#endif
 // Notation as in bssn_algebra.h; a trailing _b marks an adjoint.  Every loop
 // is unrolled (arrays stay in registers) and sections are fenced so the
 // scheduler does not stretch live ranges across them.
@@ -307,7 +310,28 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
   double R_b[6] = {}, DDa_b[6] = {}, F_b = 0.0, trDDa_b = 0.0, trAA_b = 0.0, divb_b = 0.0;
   double dchi_b[3] = {}, dalp_b[3] = {}, dK_b[3] = {}, Gamxa_b[3] = {}, GA_b[3] = {}, Lb_b[3] = {};
   double G2_b[3][6] = {}, G1_b[3][6] = {};
+  // the running sums of the most-fed bars are formed at every section end
+  // (not deferred to the outputs, which would keep their terms live)
+  auto pin_bars = [&]() {
+    if constexpr (AMSS_ADJ_PIN) {
+      amss_pin<Env>(g_b);
+      amss_pin<Env>(gu_b);
+      amss_pin<Env>(dchi_b);
+      amss_pin1<Env>(ichi_b);
+      amss_pin1<Env>(chi1_b);
+      amss_pin1<Env>(alp_b);
+      amss_pin1<Env>(K_b);
+      if (AMSS_ADJ_PIN > 1) {
+        amss_pin<Env>(A_b);
+        amss_pin<Env>(Aup_b);
+        amss_pin<Env>(dalp_b);
+        amss_pin<Env>(R_b);
+        amss_pin<Env>(DDa_b);
+      }
+    }
+  };
 
+  pin_bars();
   AMSS_FENCE();
   // A_ij rhs
 #pragma unroll
@@ -339,6 +363,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       db_b[k][i] += l * A[S2(j, k)];
     }
   }
+  pin_bars();
   AMSS_FENCE();
   // F, f0
   trDDa_b -= F1o3 * F_b;
@@ -349,6 +374,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     K_b += 2.0 * F2o3 * K * f0_b;
     trAA_b -= f0_b;
   }
+  pin_bars();
   AMSS_FENCE();
   // trK rhs
   {
@@ -369,6 +395,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
 #pragma unroll
     for (int j = 0; j < 3; ++j) AU_b[i][j] += 2.0 * trAA_b * AU[j][i];
 
+  pin_bars();
   AMSS_FENCE();
   // Gamma-tilde and B rhs (GAUGE 0: eta = 2)
 #pragma unroll
@@ -396,6 +423,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       db_b[i][j] += ab_b * Gamxa[j];
     }
   }
+  pin_bars();
   AMSS_FENCE();
   // lapse rhs and D_i D_j alpha
   alp_b -= 2.0 * K * lam[LAP];
@@ -403,6 +431,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
   double ha_b[6];
 #pragma unroll
   for (int s = 0; s < 6; ++s) ha_b[s] = DDa_b[s];
+  pin_bars();
   AMSS_FENCE();
   // physical Christoffels: their adjoint is -DDa_b[s] dalp[k]
   {
@@ -437,6 +466,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
         dchi_b[l] += u_b[k] * ichi * gu[S2(k, l)];
       }
   }
+  pin_bars();
   AMSS_FENCE();
   // chi rhs
   {
@@ -446,6 +476,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     K_b += F2o3 * chi1 * alp * l;
     divb_b -= F2o3 * chi1 * l;
   }
+  pin_bars();
   AMSS_FENCE();
   // chi part of Ricci
   double hc_b[6];
@@ -488,6 +519,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       }
     }
   }
+  pin_bars();
   AMSS_FENCE();
   // Gamma-tilde derivatives in Ricci
   double dGt_b[3][3] = {};
@@ -503,6 +535,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
         g_b[S2(k, j)] += rb * dGt[k][i];
         dGt_b[k][i] += rb * g[S2(k, j)];
       }
+  pin_bars();
   AMSS_FENCE();
   // quadratic Ricci
 #pragma unroll
@@ -542,6 +575,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
           G2_b[k][S2(l, i)] += U_b[m][k] * gu[S2(l, m)];
         }
   }
+  pin_bars();
   AMSS_FENCE();
   // Gamxa, GA
 #pragma unroll
@@ -552,6 +586,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       Aup_b[s] += GA_b[m] * W[s] * G2[m][s];
       gu_b[s] += Gamxa_b[m] * W[s] * G2[m][s];
     }
+  pin_bars();
   AMSS_FENCE();
   // G2 = gu G1
 #pragma unroll
@@ -563,6 +598,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
         gu_b[S2(m, l)] += G2_b[m][s] * G1[l][s];
         G1_b[l][s] += G2_b[m][s] * gu[S2(m, l)];
       }
+  pin_bars();
   AMSS_FENCE();
   // G1 from d g: metric gradient bars
   double dg_b[3][6] = {};
@@ -577,6 +613,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
         dg_b[j][S2(i, k)] += b;
         dg_b[k][S2(i, j)] -= b;
       }
+  pin_bars();
   AMSS_FENCE();
   // metric rhs
 #pragma unroll
@@ -596,6 +633,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       db_b[k][i] += l * g[S2(j, k)];
     }
   }
+  pin_bars();
   AMSS_FENCE();
   // metric Hessians through R = -1/2 gu^kl d_k d_l g_ij: bars and gu_b
 #pragma unroll
@@ -608,6 +646,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       O.put(adj_hess_index(adj_hess_slot(DXX + s), t), -0.5 * R_b[s] * W[t] * gu[t]);
     }
   }
+  pin_bars();
   AMSS_FENCE();
   // shift: Lb = gu^jk d_j d_k beta^i + 1/3 gu^ij d_j div beta
 #pragma unroll
@@ -638,6 +677,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     for (int j = 0; j < 3; ++j) gu_b[S2(i, j)] += F1o3 * Lb_b[i] * gdiv[j];
 #pragma unroll
   for (int a = 0; a < 3; ++a) db_b[a][a] += divb_b;
+  pin_bars();
   AMSS_FENCE();
   // A^ij and A^i_j
 #pragma unroll
@@ -658,6 +698,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
         gu_b[S2(i, k)] += AU_b[i][j] * A[S2(k, j)];
         A_b[S2(k, j)] += AU_b[i][j] * gu[S2(i, k)];
       }
+  pin_bars();
   AMSS_FENCE();
   // inverse metric: d gu = -gu (d g) gu
   {

@@ -36,6 +36,51 @@ __global__ void __launch_bounds__(kRhsThreads) rhs_k_kernel(FieldsIn in, FieldsO
   bssn_rhs_point<0, false, false, false>(E, S);
 }
 
+// rhs_k_kernel on 8x8x4 tiles with the forward kernel's LDS environment.
+__global__ void __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+rhs_k_lds_kernel(FieldsIn in, FieldsOut out, Box box, Coef C) {
+  using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
+  __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
+  const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
+  const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
+  const int b = xcd_spread(blockIdx.x, gridDim.x);
+  const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;
+  const int k0 = box.lo[2] + (b / (gx * gy)) * kLdsTZ;
+  const long p = (i0 + tx) + (long)(j0 + ty) * box.sy + (long)(k0 + tz) * box.sz;
+  lds_double* t0 = (lds_double*)tiles;
+  LdsEnv<kLdsTX, kLdsTY, kLdsTZ> E{in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p,
+                                  (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
+                                  (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz,
+                                  box.n[0] * box.n[1] * box.n[2], (int)threadIdx.x};
+  E.prologue();
+  KSink S{out, p};
+  bssn_rhs_point<0, false, false, false>(E, S);
+}
+
+// Run `lds` (tile grid size, tile-aligned core box) on the largest sub-box of
+// whole 8x8x4 tiles and `direct` (box) on the remainder slabs; all direct
+// when the row pitch is odd (the 16-byte halo loads need an even one).
+template <class L, class D>
+void split_tiles(const Box& box, L lds, D direct) {
+  Box core = box;
+  const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
+  for (int a = 0; a < 3; ++a)
+    core.hi[a] = box.sy % 2 == 0 ? box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a] : box.lo[a];
+  if (!box_empty(core)) {
+    lds((unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
+            ((core.hi[2] - core.lo[2]) / kLdsTZ),
+        core);
+  } else {
+    for (int a = 0; a < 3; ++a) core.hi[a] = core.lo[a];
+  }
+  for (int a = 0; a < 3; ++a) {  // remainder slab [core.hi[a], box.hi[a]) along axis a
+    Box rem = box;
+    for (int c = 0; c < a; ++c) rem.hi[c] = core.hi[c];
+    rem.lo[a] = core.hi[a];
+    if (!box_empty(rem)) direct(rem);
+  }
+}
+
 __global__ void __launch_bounds__(256) adj_point_kernel(FieldsIn in, FieldsIn lam, double* bars, long stride, Box box,
                                                         Coef C) {
   int i, j, k;
@@ -409,32 +454,18 @@ unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, const Coef& C, hipStream_t s) {
-  rhs_k_kernel<<<interior_grid(box), kRhsThreads, 0, s>>>(in, out, box, C);
+  split_tiles(
+      box, [&](unsigned gr, const Box& core) { rhs_k_lds_kernel<<<gr, kRhsThreads, 0, s>>>(in, out, core, C); },
+      [&](const Box& rem) { rhs_k_kernel<<<interior_grid(rem), kRhsThreads, 0, s>>>(in, out, rem, C); });
 }
 void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, long stride, const Box& box,
                       const Coef& C, hipStream_t s, double* feat) {
   if (feat) {
     const long fs = (long)box.n[0] * box.n[1] * box.n[2];
-    Box core = box;
-    if (box.sy % 2 == 0) {  // 16-byte halo loads need an even row pitch
-      const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
-      for (int a = 0; a < 3; ++a) core.hi[a] = box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a];
-    } else {
-      core.hi[0] = core.lo[0];
-    }
-    if (!box_empty(core)) {
-      const unsigned gr = (unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
-                          ((core.hi[2] - core.lo[2]) / kLdsTZ);
-      adj_feature_lds_kernel<<<gr, 256, 0, s>>>(in, lam, feat, fs, core, C);
-    } else {
-      for (int a = 0; a < 3; ++a) core.hi[a] = core.lo[a];
-    }
-    for (int a = 0; a < 3; ++a) {  // remainder slab [core.hi[a], box.hi[a]) along axis a: direct kernel
-      Box rem = box;
-      for (int c = 0; c < a; ++c) rem.hi[c] = core.hi[c];
-      rem.lo[a] = core.hi[a];
-      if (!box_empty(rem)) adj_feature_kernel<<<interior_grid(rem), 256, 0, s>>>(in, lam, feat, fs, rem, C);
-    }
+    split_tiles(
+        box,
+        [&](unsigned gr, const Box& core) { adj_feature_lds_kernel<<<gr, 256, 0, s>>>(in, lam, feat, fs, core, C); },
+        [&](const Box& rem) { adj_feature_kernel<<<interior_grid(rem), 256, 0, s>>>(in, lam, feat, fs, rem, C); });
     adj_algebra_kernel<<<interior_grid(box), 256, 0, s>>>(feat, fs, lam, bars, stride, box, C);
   } else {
     adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, stride, box, C);
