@@ -337,12 +337,9 @@ int LevelAdjoint::forward(double dT, const double* const* U, double* const* Unex
   if (Unext)
     for (int p = 0; p < np; ++p) {
       const long N = blocks_[p]->npts();
-      for (int v = 0; v < NFIELD; ++v)
-        if (hipMemcpyAsync(Unext[p * NFIELD + v], blocks_[p]->stage_output(3) + v * N, N * sizeof(double),
-                           hipMemcpyDeviceToDevice, s) != hipSuccess) {
-          err_ = "copy of the step's output";
-          return AMSS_ERR_HIP;
-        }
+      FieldsIn src;
+      for (int v = 0; v < NFIELD; ++v) src.p[v] = blocks_[p]->stage_output(3) + v * N;
+      adj_launch_copy_fields(src, as_out(Unext + p * NFIELD), N, s);
     }
   if (hipGetLastError() != hipSuccess) {
     err_ = "level adjoint forward: kernel launch failed";
@@ -364,12 +361,9 @@ int LevelAdjoint::backward(double dT, const double* const* U, double* const* lam
   // dJ/dU^{n+1} into each Block's dJ/dY run, then the final exchange reversed
   for (int p = 0; p < np; ++p) {
     const long N = blocks_[p]->npts();
-    for (int v = 0; v < NFIELD; ++v)
-      if (hipMemcpyAsync(blocks_[p]->lY() + v * N, lam[p * NFIELD + v], N * sizeof(double), hipMemcpyDeviceToDevice,
-                         s) != hipSuccess) {
-        err_ = "copy of dJ/dU";
-        return AMSS_ERR_HIP;
-      }
+    FieldsOut dst;
+    for (int v = 0; v < NFIELD; ++v) dst.p[v] = blocks_[p]->lY() + v * N;
+    adj_launch_copy_fields(as_in(lam + p * NFIELD), dst, N, s);
   }
   exchange_adjoint(s);
   for (int p = 0; p < np; ++p) {

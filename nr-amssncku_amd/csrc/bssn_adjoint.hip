@@ -449,6 +449,13 @@ __global__ void __launch_bounds__(256) copy_enforce_kernel(FieldsIn U, FieldsOut
   for (int v = 0; v < NFIELD; ++v) Y.p[v][q] = u[v];
 }
 
+// 24 fields src -> dst (blockIdx.y = field): one launch for a whole state
+__global__ void __launch_bounds__(256) copy_fields_kernel(FieldsIn src, FieldsOut dst, long N) {
+  const double* __restrict__ a = src.p[blockIdx.y];
+  double* __restrict__ b = dst.p[blockIdx.y];
+  for (long q = blockIdx.x * 256L + threadIdx.x; q < N; q += gridDim.x * 256L) b[q] = a[q];
+}
+
 unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -497,6 +504,10 @@ void adj_launch_stage(const FieldsIn& lY, const FieldsIn& z, const FieldsIn& Y0,
 }
 void adj_launch_open(const FieldsIn& U, const FieldsIn& lY0, const FieldsOut& lam, long N, hipStream_t s) {
   adj_open_kernel<<<nblk(N), 256, 0, s>>>(U, lY0, lam, N);
+}
+void adj_launch_copy_fields(const FieldsIn& src, const FieldsOut& dst, long N, hipStream_t s) {
+  const unsigned bx = nblk(N) < 1024u ? nblk(N) : 1024u;
+  copy_fields_kernel<<<dim3(bx, NFIELD), 256, 0, s>>>(src, dst, N);
 }
 void adj_launch_copy_enforce(const FieldsIn& U, const FieldsOut& Y, long N, hipStream_t s) {
   copy_enforce_kernel<<<nblk(N), 256, 0, s>>>(U, Y, N);

@@ -25,5 +25,6 @@ void adj_launch_stage(const FieldsIn& lY, const FieldsIn& z, const FieldsIn& Y0,
                       const FieldsOut& lk, const Box& box, double c, double tinny, hipStream_t s);
 void adj_launch_open(const FieldsIn& U, const FieldsIn& lY0, const FieldsOut& lam, long N, hipStream_t s);
 void adj_launch_copy_enforce(const FieldsIn& U, const FieldsOut& Y, long N, hipStream_t s);
+void adj_launch_copy_fields(const FieldsIn& src, const FieldsOut& dst, long N, hipStream_t s);
 
 }  // namespace amss
