@@ -601,10 +601,20 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
     const Roles m{slot_of(in), slot_of(StateList), slot_of(RHSList), slot_of(out)};
     const Rk4Param R{it, 1, cst[it], tinny};
     const bool res = want_res && it == 0;
+    // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap after
+    // sub-step 1), whose fill points already hold the same values when
+    // nothing else writes ghost points (frozen ghosts, no edge kernel).
+    const bool frozen = bc_ == AMSS_BC_FROZEN_GHOSTS && gw_ >= 3;
+    const bool fill = !(it == 2 && frozen);
+    // With frozen ghosts the fill reads only the state and writes only
+    // points no tile writes: it runs in extra workgroups of the tile launch.
+    const int ntiles = (int)htiles_[sh].size();
+    const bool fill_in_tiles = frozen && ghost_fused_ && launch_rk4_tiles_fuses_fill(tune_, ntiles);
+    FillWork fw;
+    if (fill_in_tiles && fill) fw = FillWork{dgbox_, dgwork_, nfill_work_};
     Ev* ev = nullptr;
     if (int r = time_event_begin(it, &ev)) return r;
-    launch_rk4_tiles(gauge_, dpd_, dtiles_[sh], 0, (int)htiles_[sh].size(), m, coef_[0], R, res, tune_, dflag_,
-                     stream_);
+    launch_rk4_tiles(gauge_, dpd_, dtiles_[sh], 0, ntiles, m, coef_[0], R, res, tune_, dflag_, stream_, fw);
     if (ev) RCHK(hipEventRecord(ev->b, stream_));
     AuxOut aux;
     for (int a = 0; a < 18; ++a) aux.gam[a] = nullptr;
@@ -633,13 +643,11 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
                    coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false);
       }
     }
-    // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap after
-    // sub-step 1), whose fill points already hold the same values when
-    // nothing else writes ghost points (frozen ghosts, no edge kernel).  The
-    // state is the same in all sub-steps, so sub-step 0 tests it for NaN at
-    // the exchange points as well and the later sub-steps need not.
-    const bool fill = !(it == 2 && bc_ == AMSS_BC_FROZEN_GHOSTS && gw_ >= 3);
-    if (ghost_fused_ && fill) {
+    // The state is the same in all sub-steps, so sub-step 0 tests it for NaN
+    // at the exchange points as well and the later sub-steps need not.
+    if (fill_in_tiles) {
+      launch_ghost_boxes(dpd_, dgbox_, dgwork_ + nfill_work_, nseg_work_, m, it, tinny, it == 0, dflag_, stream_);
+    } else if (ghost_fused_ && fill) {
       launch_ghost_boxes(dpd_, dgbox_, dgwork_, nfill_work_ + nseg_work_, m, it, tinny, it == 0, dflag_, stream_);
     } else {
       if (fill) launch_ghost_boxes(dpd_, dgbox_, dgwork_, nfill_work_, m, it, tinny, false, dflag_, stream_);

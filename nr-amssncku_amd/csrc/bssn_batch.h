@@ -52,6 +52,44 @@ struct GhostWork {
   int box, start;
 };
 
+// One fill point of work item w (box kind 0): the state's values with chi
+// bounded below and, for sub-steps 0-2, enforce_ga; every field is loaded
+// before any is stored (all loads of a wave in flight together).
+__device__ __forceinline__ void fill_ghost_point(const PatchDesc* __restrict__ P, const GhostBox* __restrict__ B,
+                                                 const GhostWork& w, unsigned lane, const Roles& m, int stage,
+                                                 double tinny, int* flag) {
+  const GhostBox b = B[w.box];
+  const unsigned t = (unsigned)w.start + lane;
+  const unsigned e01 = (unsigned)b.ext[0] * b.ext[1];
+  if (t >= e01 * b.ext[2]) return;
+  const int x = (int)(t % b.ext[0]), y = (int)((t / b.ext[0]) % b.ext[1]), z = (int)(t / e01);
+  const PatchDesc d = P[b.dst];
+  const long pd = (b.dlo[0] + x) + (long)(b.dlo[1] + y) * d.sy + (long)(b.dlo[2] + z) * d.sz;
+  const long ln = (long)NFIELD * d.n;
+  const gdouble* u0 = (const gdouble*)d.slab + m.u0 * ln;
+  gdouble* out = (gdouble*)d.slab + m.out * ln;
+  double u[NFIELD];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) u[v] = u0[v * d.n + pd];
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < 21; ++v) bad |= !(u[v] == u[v]);  // bssn_rhs.f90:110-113
+  if (bad) *flag = 1;
+  if (stage < 3) enforce_point(u + DXX, u + AXX);
+  u[CHI] = u[CHI] < tinny ? tinny : u[CHI];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) out[v * d.n + pd] = u[v];
+}
+
+// Fill work that a tile launch runs in extra workgroups after its tiles
+// (frozen ghosts: the fill reads only the state and writes only ghost points
+// no tile writes, so it overlaps the latency-bound tiles and their tail).
+struct FillWork {
+  const GhostBox* B = nullptr;
+  const GhostWork* W = nullptr;
+  int n = 0;  // 256-point work items
+};
+
 // Tile shapes of the batched kernel (points per tile along x, y, z).
 constexpr int kNumTileShapes = 4;
 __host__ __device__ constexpr int tile_dim(int shape, int a) {
@@ -74,13 +112,18 @@ struct BatchTune {
 };
 
 // Fused RHS + RK4 sub-step over tiles [t0, t0 + ntiles) of the tile list.
+// fw: fill work run in extra workgroups (only with one workgroup per tile:
+// launch_rk4_tiles_fuses_fill).
 void launch_rk4_tiles(int gauge, const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
                       const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune, int* flag,
-                      hipStream_t s);
+                      hipStream_t s, const FillWork& fw = FillWork{});
+inline bool launch_rk4_tiles_fuses_fill(const BatchTune& tune, int ntiles) {
+  return !(tune.grid > 0 && tune.grid < ntiles) && ntiles > 0;
+}
 #define AMSS_DECL_TILES(G, S)                                                                             \
   void launch_rk4_tiles_g##G##_s##S(const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m, \
                                     const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune,  \
-                                    int* flag, hipStream_t s);
+                                    int* flag, hipStream_t s, const FillWork& fw);
 AMSS_DECL_TILES(0, 0)
 AMSS_DECL_TILES(0, 1)
 AMSS_DECL_TILES(0, 2)

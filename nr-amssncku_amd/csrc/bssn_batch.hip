@@ -19,6 +19,10 @@ __global__ void __launch_bounds__(256) ghost_boxes_kernel(const PatchDesc* __res
                                                           double tinny, int check_u0, int* flag) {
   const GhostWork w = W[blockIdx.x];
   const GhostBox b = B[w.box];
+  if (b.kind == 0) {
+    fill_ghost_point(P, B, w, threadIdx.x, m, stage, tinny, flag);
+    return;
+  }
   const unsigned t = (unsigned)w.start + threadIdx.x;
   const unsigned e01 = (unsigned)b.ext[0] * b.ext[1];
   if (t >= e01 * b.ext[2]) return;
@@ -26,32 +30,21 @@ __global__ void __launch_bounds__(256) ghost_boxes_kernel(const PatchDesc* __res
   const PatchDesc d = P[b.dst];
   const long pd = (b.dlo[0] + x) + (long)(b.dlo[1] + y) * d.sy + (long)(b.dlo[2] + z) * d.sz;
   const long ln = (long)NFIELD * d.n;
-  const gdouble* u0 = (const gdouble*)d.slab + m.u0 * ln;
   double u[NFIELD];
-  if (b.kind == 0) {
+  const PatchDesc a = P[b.src];
+  const long ps = (b.slo[0] + x) + (long)(b.slo[1] + y) * a.sy + (long)(b.slo[2] + z) * a.sz;
+  const gdouble* src = (const gdouble*)a.slab + m.out * (long)NFIELD * a.n;
 #pragma unroll
-    for (int v = 0; v < NFIELD; ++v) u[v] = u0[v * d.n + pd];
+  for (int v = 0; v < NFIELD; ++v) u[v] = src[v * a.n + ps];
+  if (check_u0) {
+    const gdouble* u0 = (const gdouble*)d.slab + m.u0 * ln;
     bool bad = false;
 #pragma unroll
-    for (int v = 0; v < 21; ++v) bad |= !(u[v] == u[v]);  // bssn_rhs.f90:110-113
-    if (bad) *flag = 1;
-    if (stage < 3) enforce_point(u + DXX, u + AXX);
-    u[CHI] = u[CHI] < tinny ? tinny : u[CHI];
-  } else {
-    const PatchDesc a = P[b.src];
-    const long ps = (b.slo[0] + x) + (long)(b.slo[1] + y) * a.sy + (long)(b.slo[2] + z) * a.sz;
-    const gdouble* src = (const gdouble*)a.slab + m.out * (long)NFIELD * a.n;
-#pragma unroll
-    for (int v = 0; v < NFIELD; ++v) u[v] = src[v * a.n + ps];
-    if (check_u0) {
-      bool bad = false;
-#pragma unroll
-      for (int v = 0; v < 21; ++v) {
-        const double c = u0[v * d.n + pd];
-        bad |= !(c == c);
-      }
-      if (bad) *flag = 1;
+    for (int v = 0; v < 21; ++v) {
+      const double c = u0[v * d.n + pd];
+      bad |= !(c == c);
     }
+    if (bad) *flag = 1;
   }
   gdouble* out = (gdouble*)d.slab + m.out * ln;
 #pragma unroll
@@ -123,10 +116,10 @@ unsigned xblocks(long maxpts) {
 
 void launch_rk4_tiles(int gauge, const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
                       const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune, int* flag,
-                      hipStream_t s) {
+                      hipStream_t s, const FillWork& fw) {
 #define AMSS_CASE(G, S)                                                                      \
   case S:                                                                                    \
-    launch_rk4_tiles_g##G##_s##S(P, T, t0, ntiles, m, C, R, want_res, tune, flag, s); \
+    launch_rk4_tiles_g##G##_s##S(P, T, t0, ntiles, m, C, R, want_res, tune, flag, s, fw); \
     break;
   if (gauge == 2) {
     switch (tune.shape) { AMSS_CASE(2, 0) AMSS_CASE(2, 1) AMSS_CASE(2, 2) AMSS_CASE(2, 3) }

@@ -41,10 +41,17 @@ __device__ __forceinline__ void tile_body(const PatchDesc* __restrict__ P, const
 template <int GAUGE, int STAGE, bool CO, int TX, int TY, int TZ, bool PERSIST>
 __global__ void __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(2, 2)))
 rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T, int t0, int ntiles, Roles m,
-                 Coef C, Rk4Param R, int remap, int* flag) {
+                 Coef C, Rk4Param R, int remap, int* flag, FillWork fw) {
   using Tl = LdsTile<TX, TY, TZ>;
   __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
   lds_double* b0 = (lds_double*)tiles;
+  if (!PERSIST && (int)blockIdx.x >= ntiles) {
+    // fill work: TX TY TZ / 256 items of 256 points per workgroup
+    constexpr int kItems = TX * TY * TZ / 256;
+    const int item = ((int)blockIdx.x - ntiles) * kItems + (int)threadIdx.x / 256;
+    if (item < fw.n) fill_ghost_point(P, fw.B, fw.W[item], threadIdx.x % 256, m, STAGE, R.tinny, flag);
+    return;
+  }
   if (PERSIST) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
       // keep the compiler from hoisting per-thread LDS addresses out of the
@@ -63,39 +70,48 @@ rk4_tiles_kernel(const PatchDesc* __restrict__ P, const TileRef* __restrict__ T,
 
 template <int G, int TX, int TY, int TZ, bool PERSIST>
 void launch_tiles_p(unsigned g, const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
-                    const Coef& C, const Rk4Param& R, bool want_res, int remap, int* flag, hipStream_t s) {
+                    const Coef& C, const Rk4Param& R, bool want_res, int remap, int* flag, hipStream_t s,
+                    const FillWork& fw) {
   const int nt = TX * TY * TZ;
+#define AMSS_TILES(ST, CO) \
+  rk4_tiles_kernel<G, ST, CO, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag, fw)
   switch (R.stage) {
     case 0:
-      if (want_res)
-        rk4_tiles_kernel<G, 0, true, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag);
-      else
-        rk4_tiles_kernel<G, 0, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag);
+      if (want_res) AMSS_TILES(0, true);
+      else AMSS_TILES(0, false);
       break;
-    case 1: rk4_tiles_kernel<G, 1, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag); break;
-    case 2: rk4_tiles_kernel<G, 2, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag); break;
-    default: rk4_tiles_kernel<G, 3, false, TX, TY, TZ, PERSIST><<<g, nt, 0, s>>>(P, T, t0, ntiles, m, C, R, remap, flag); break;
+    case 1: AMSS_TILES(1, false); break;
+    case 2: AMSS_TILES(2, false); break;
+    default: AMSS_TILES(3, false); break;
   }
+#undef AMSS_TILES
 }
 
+// Fill work rides along only with one workgroup per tile (the caller runs it
+// separately otherwise: launch_rk4_tiles_fuses_fill).
 template <int G, int TX, int TY, int TZ>
 void launch_rk4_tiles_shape(const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m,
                             const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune, int* flag,
-                            hipStream_t s) {
+                            hipStream_t s, const FillWork& fw) {
   if (ntiles <= 0) return;
-  if (tune.grid > 0 && tune.grid < ntiles)
-    launch_tiles_p<G, TX, TY, TZ, true>((unsigned)tune.grid, P, T, t0, ntiles, m, C, R, want_res, tune.remap, flag, s);
-  else
-    launch_tiles_p<G, TX, TY, TZ, false>((unsigned)ntiles, P, T, t0, ntiles, m, C, R, want_res, tune.remap, flag, s);
+  constexpr int kItems = TX * TY * TZ / 256;
+  if (tune.grid > 0 && tune.grid < ntiles) {
+    launch_tiles_p<G, TX, TY, TZ, true>((unsigned)tune.grid, P, T, t0, ntiles, m, C, R, want_res, tune.remap, flag, s,
+                                        FillWork{});
+  } else {
+    const unsigned extra = (unsigned)((fw.n + kItems - 1) / kItems);
+    launch_tiles_p<G, TX, TY, TZ, false>((unsigned)ntiles + extra, P, T, t0, ntiles, m, C, R, want_res, tune.remap,
+                                         flag, s, fw);
+  }
 }
 
 // One translation unit per gauge and tile shape (they compile in parallel).
 #define AMSS_BATCH_TU(G, S)                                                                                  \
   void launch_rk4_tiles_g##G##_s##S(const PatchDesc* P, const TileRef* T, int t0, int ntiles, const Roles& m, \
                                     const Coef& C, const Rk4Param& R, bool want_res, const BatchTune& tune,  \
-                                    int* flag, hipStream_t s) {                                              \
+                                    int* flag, hipStream_t s, const FillWork& fw) {                          \
     launch_rk4_tiles_shape<G, tile_dim(S, 0), tile_dim(S, 1), tile_dim(S, 2)>(P, T, t0, ntiles, m, C, R,      \
-                                                                              want_res, tune, flag, s);      \
+                                                                              want_res, tune, flag, s, fw);  \
   }
 
 }  // namespace amss
