@@ -20,7 +20,7 @@
 // is unrolled (arrays stay in registers) and sections are fenced so the
 // scheduler does not stretch live ranges across them.
 #pragma once
-#include "bssn_env.h"
+#include "bssn_lds.h"
 
 namespace amss {
 
@@ -104,8 +104,14 @@ struct FeatureEnv {
   }
 };
 
+// LDS stash of the reverse algebra: the Christoffel symbols of the first and
+// second kind and their adjoints (4 x 18 doubles per point), lane-strided
+// (element e of thread t at st[e * kAdjStride + t]: consecutive lanes hit
+// consecutive banks).  Holding them in LDS instead of registers halves the
+// kernel's spill traffic at its one wave per SIMD.
+constexpr int kAdjStash = 72, kAdjStride = 256;  // doubles per point; threads per workgroup
 template <class Env>
-__device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const AdjSink& O) {
+__device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const AdjSink& O, lds_double* st) {
   constexpr double F1o3 = 1.0 / 3.0, F2o3 = 2.0 / 3.0;
   constexpr double W[6] = {1.0, 2.0, 2.0, 1.0, 2.0, 1.0};
   auto S2 = [](int i, int j) { return sidx(i, j); };
@@ -166,7 +172,8 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
   const double divb = db[0][0] + db[1][1] + db[2][2];
 
   AMSS_FENCE();
-  double G1[3][6], R[6];
+  auto G1 = [&](int k, int s) -> lds_double& { return st[(0 * 18 + k * 6 + s) * kAdjStride]; };
+  double R[6];
   {
     double dg[3][6];
 #pragma unroll
@@ -183,22 +190,22 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = i; j < 3; ++j) G1[k][S2(i, j)] = 0.5 * (dg[i][S2(j, k)] + dg[j][S2(i, k)] - dg[k][S2(i, j)]);
+        for (int j = i; j < 3; ++j) G1(k, S2(i, j)) = 0.5 * (dg[i][S2(j, k)] + dg[j][S2(i, k)] - dg[k][S2(i, j)]);
   }
   AMSS_FENCE();
-  double G2[3][6];
+  auto G2 = [&](int k, int s) -> lds_double& { return st[(1 * 18 + k * 6 + s) * kAdjStride]; };
 #pragma unroll
   for (int m = 0; m < 3; ++m)
 #pragma unroll
-    for (int s = 0; s < 6; ++s) G2[m][s] = gu[S2(m, 0)] * G1[0][s] + gu[S2(m, 1)] * G1[1][s] + gu[S2(m, 2)] * G1[2][s];
+    for (int s = 0; s < 6; ++s) G2(m, s) = gu[S2(m, 0)] * G1(0, s) + gu[S2(m, 1)] * G1(1, s) + gu[S2(m, 2)] * G1(2, s);
   AMSS_FENCE();
   double Gamxa[3], GA[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    Gamxa[m] = tr(G2[m]);
+    Gamxa[m] = gu[0] * G2(m, 0) + gu[3] * G2(m, 3) + gu[5] * G2(m, 5) + 2.0 * (gu[1] * G2(m, 1) + gu[2] * G2(m, 2) + gu[4] * G2(m, 4));
     double t = 0.0;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) t += W[s] * G2[m][s] * Aup[s];
+    for (int s = 0; s < 6; ++s) t += W[s] * G2(m, s) * Aup[s];
     GA[m] = t;
   }
   AMSS_FENCE();
@@ -209,7 +216,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        U[m][k] = gu[S2(0, m)] * G2[k][S2(0, i)] + gu[S2(1, m)] * G2[k][S2(1, i)] + gu[S2(2, m)] * G2[k][S2(2, i)];
+        U[m][k] = gu[S2(0, m)] * G2(k, S2(0, i)) + gu[S2(1, m)] * G2(k, S2(1, i)) + gu[S2(2, m)] * G2(k, S2(2, i));
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       double P = 0.0, H = 0.0;
@@ -217,10 +224,10 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          P += U[m][k] * (G1[j][S2(k, m)] + G1[k][S2(m, j)]);
-          H += U[m][k] * G1[j][S2(k, m)];
+          P += U[m][k] * (G1(j, S2(k, m)) + G1(k, S2(m, j)));
+          H += U[m][k] * G1(j, S2(k, m));
         }
-      const double gg = 0.5 * (Gamxa[0] * G1[i][S2(j, 0)] + Gamxa[1] * G1[i][S2(j, 1)] + Gamxa[2] * G1[i][S2(j, 2)]);
+      const double gg = 0.5 * (Gamxa[0] * G1(i, S2(j, 0)) + Gamxa[1] * G1(i, S2(j, 1)) + Gamxa[2] * G1(i, S2(j, 2)));
       if (j >= i) R[S2(i, j)] += P + gg;
       if (j <= i) R[S2(j, i)] += H + gg;
     }
@@ -246,7 +253,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
 #pragma unroll
       for (int j = i; j < 3; ++j) {
         const int s = S2(i, j);
-        hh[s] = h[s] - (G2[0][s] * dchi[0] + G2[1][s] * dchi[1] + G2[2][s] * dchi[2]);
+        hh[s] = h[s] - (G2(0, s) * dchi[0] + G2(1, s) * dchi[1] + G2(2, s) * dchi[2]);
         fs += W[s] * gu[s] * (hh[s] - 1.5 * ichi * dchi[i] * dchi[j]);
       }
 #pragma unroll
@@ -268,7 +275,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     double d = 0.5 * g[s] * u[k];
     if (k == i) d -= 0.5 * ichi * dchi[j];
     if (k == j) d -= 0.5 * ichi * dchi[i];
-    return G2[k][s] + d;
+    return G2(k, s) + d;
   };
   AMSS_FENCE();
   double dalp[3], DDa[6];
@@ -309,7 +316,10 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
   double db_b[3][3] = {}, AU_b[3][3] = {}, Aup_b[6] = {};
   double R_b[6] = {}, DDa_b[6] = {}, F_b = 0.0, trDDa_b = 0.0, trAA_b = 0.0, divb_b = 0.0;
   double dchi_b[3] = {}, dalp_b[3] = {}, dK_b[3] = {}, Gamxa_b[3] = {}, GA_b[3] = {}, Lb_b[3] = {};
-  double G2_b[3][6] = {}, G1_b[3][6] = {};
+  auto G1_b = [&](int k, int s) -> lds_double& { return st[(2 * 18 + k * 6 + s) * kAdjStride]; };
+  auto G2_b = [&](int k, int s) -> lds_double& { return st[(3 * 18 + k * 6 + s) * kAdjStride]; };
+#pragma unroll
+  for (int e = 0; e < 18; ++e) G1_b(e / 6, e % 6) = 0.0, G2_b(e / 6, e % 6) = 0.0;
   // the running sums of the most-fed bars are formed at every section end
   // (not deferred to the outputs, which would keep their terms live)
   auto pin_bars = [&]() {
@@ -445,7 +455,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
           const int s = S2(i, j);
           dalp_b[k] -= DDa_b[s] * P2(k, s);
           const double pb = -DDa_b[s] * dalp[k];
-          G2_b[k][s] += pb;
+          G2_b(k, s) += pb;
           g_b[s] += 0.5 * u[k] * pb;
           u_b[k] += 0.5 * g[s] * pb;
           if (k == i) {
@@ -514,8 +524,8 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       hc_b[s] = hh_b[s];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        G2_b[m][s] -= hh_b[s] * dchi[m];
-        dchi_b[m] -= hh_b[s] * G2[m][s];
+        G2_b(m, s) -= hh_b[s] * dchi[m];
+        dchi_b[m] -= hh_b[s] * G2(m, s);
       }
     }
   }
@@ -545,7 +555,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        U[m][k] = gu[S2(0, m)] * G2[k][S2(0, i)] + gu[S2(1, m)] * G2[k][S2(1, i)] + gu[S2(2, m)] * G2[k][S2(2, i)];
+        U[m][k] = gu[S2(0, m)] * G2(k, S2(0, i)) + gu[S2(1, m)] * G2(k, S2(1, i)) + gu[S2(2, m)] * G2(k, S2(2, i));
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const double P_b = j >= i ? R_b[S2(i, j)] : 0.0;
@@ -553,16 +563,16 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       const double gg_b = P_b + H_b;
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        Gamxa_b[l] += 0.5 * gg_b * G1[i][S2(j, l)];
-        G1_b[i][S2(j, l)] += 0.5 * gg_b * Gamxa[l];
+        Gamxa_b[l] += 0.5 * gg_b * G1(i, S2(j, l));
+        G1_b(i, S2(j, l)) += 0.5 * gg_b * Gamxa[l];
       }
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          U_b[m][k] += P_b * (G1[j][S2(k, m)] + G1[k][S2(m, j)]) + H_b * G1[j][S2(k, m)];
-          G1_b[j][S2(k, m)] += (P_b + H_b) * U[m][k];
-          G1_b[k][S2(m, j)] += P_b * U[m][k];
+          U_b[m][k] += P_b * (G1(j, S2(k, m)) + G1(k, S2(m, j))) + H_b * G1(j, S2(k, m));
+          G1_b(j, S2(k, m)) += (P_b + H_b) * U[m][k];
+          G1_b(k, S2(m, j)) += P_b * U[m][k];
         }
     }
 #pragma unroll
@@ -571,8 +581,8 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
       for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
-          gu_b[S2(l, m)] += U_b[m][k] * G2[k][S2(l, i)];
-          G2_b[k][S2(l, i)] += U_b[m][k] * gu[S2(l, m)];
+          gu_b[S2(l, m)] += U_b[m][k] * G2(k, S2(l, i));
+          G2_b(k, S2(l, i)) += U_b[m][k] * gu[S2(l, m)];
         }
   }
   pin_bars();
@@ -582,9 +592,9 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
   for (int m = 0; m < 3; ++m)
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-      G2_b[m][s] += GA_b[m] * W[s] * Aup[s] + Gamxa_b[m] * W[s] * gu[s];
-      Aup_b[s] += GA_b[m] * W[s] * G2[m][s];
-      gu_b[s] += Gamxa_b[m] * W[s] * G2[m][s];
+      G2_b(m, s) += GA_b[m] * W[s] * Aup[s] + Gamxa_b[m] * W[s] * gu[s];
+      Aup_b[s] += GA_b[m] * W[s] * G2(m, s);
+      gu_b[s] += Gamxa_b[m] * W[s] * G2(m, s);
     }
   pin_bars();
   AMSS_FENCE();
@@ -595,8 +605,8 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     for (int s = 0; s < 6; ++s)
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        gu_b[S2(m, l)] += G2_b[m][s] * G1[l][s];
-        G1_b[l][s] += G2_b[m][s] * gu[S2(m, l)];
+        gu_b[S2(m, l)] += G2_b(m, s) * G1(l, s);
+        G1_b(l, s) += G2_b(m, s) * gu[S2(m, l)];
       }
   pin_bars();
   AMSS_FENCE();
@@ -608,7 +618,7 @@ __device__ void bssn_rhs_adjoint_point(const Env& E, const double* lam, const Ad
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = i; j < 3; ++j) {
-        const double b = 0.5 * G1_b[k][S2(i, j)];
+        const double b = 0.5 * G1_b(k, S2(i, j));
         dg_b[i][S2(j, k)] += b;
         dg_b[j][S2(i, k)] += b;
         dg_b[k][S2(i, j)] -= b;

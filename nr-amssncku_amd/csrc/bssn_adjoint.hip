@@ -1,6 +1,7 @@
 // bssn_adjoint.hip -- kernels of the discrete adjoint of one RK4 step
 // (GAUGE 0, vacuum, Symmetry 0, frozen ghost zones of width 3):
-//   * adj_point_kernel: reverse of the pointwise algebra -> feature bars,
+//   * adj_feature_lds_kernel / adj_algebra_kernel: the stencil values of the
+//     reverse algebra, then the reverse of the pointwise algebra -> feature bars,
 //   * adj_gather_kernel: transposed stencils (first/second derivatives,
 //     upwind advection, KO) -> dJ/d(input state) at every point,
 //   * the forward re-evaluation of the step's stages, and the pointwise
@@ -79,19 +80,6 @@ void split_tiles(const Box& box, L lds, D direct) {
     rem.lo[a] = core.hi[a];
     if (!box_empty(rem)) direct(rem);
   }
-}
-
-__global__ void __launch_bounds__(256) adj_point_kernel(FieldsIn in, FieldsIn lam, double* bars, long stride, Box box,
-                                                        Coef C) {
-  int i, j, k;
-  if (!interior_index(box, i, j, k)) return;
-  const long p = i + j * box.sy + k * box.sz;
-  InteriorEnv E{in, in, C, p, box.sy, box.sz};
-  double l[NFIELD];
-#pragma unroll
-  for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
-  const AdjSink O{bars, stride, p};
-  bssn_rhs_adjoint_point(E, l, O);
 }
 
 // Every stencil value the reverse algebra reads, at the interior points: a
@@ -192,7 +180,8 @@ adj_feature_lds_kernel(FieldsIn in, FieldsIn lam, double* feat, long fs, Box box
   for (int a = 0; a < 3; ++a) feat[(kAdjFeatures + a) * fs + p] = ub[a];
 }
 
-__global__ void __launch_bounds__(256) adj_algebra_kernel(const double* feat, long fs, FieldsIn lam,
+static_assert(kAdjStride == kRhsThreads, "the algebra kernel's stash is strided by its workgroup size");
+__global__ void __launch_bounds__(kAdjStride) adj_algebra_kernel(const double* feat, long fs, FieldsIn lam,
                                                           double* bars, long stride, Box box, Coef C) {
   int i, j, k;
   if (!interior_index(box, i, j, k)) return;
@@ -202,7 +191,8 @@ __global__ void __launch_bounds__(256) adj_algebra_kernel(const double* feat, lo
 #pragma unroll
   for (int v = 0; v < NFIELD; ++v) l[v] = lam.p[v][p];
   const AdjSink O{bars, stride, p};
-  bssn_rhs_adjoint_point(E, l, O);
+  __shared__ double stash[kAdjStash * kAdjStride];
+  bssn_rhs_adjoint_point(E, l, O, (lds_double*)stash + threadIdx.x);
 }
 
 // Loads of one array at q + o for uniform offsets o in [-3 sz, 3 sz]: a
@@ -467,17 +457,13 @@ void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, 
 }
 void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, long stride, const Box& box,
                       const Coef& C, hipStream_t s, double* feat) {
-  if (feat) {
-    const long fs = (long)box.n[0] * box.n[1] * box.n[2];
-    split_tiles(
-        box,
-        [&](unsigned gr, const Box& core) { adj_feature_lds_kernel<<<gr, 256, 0, s>>>(in, lam, feat, fs, core, C); },
-        [&](const Box& rem) { adj_feature_kernel<<<interior_grid(rem), 256, 0, s>>>(in, lam, feat, fs, rem, C); });
-    adj_algebra_kernel<<<interior_grid(box), 256, 0, s>>>(feat, fs, lam, bars, stride, box, C);
-  } else {
-    adj_point_kernel<<<interior_grid(box), 256, 0, s>>>(in, lam, bars, stride, box, C);
-  }
+  const long fs = (long)box.n[0] * box.n[1] * box.n[2];
+  split_tiles(
+      box, [&](unsigned gr, const Box& core) { adj_feature_lds_kernel<<<gr, 256, 0, s>>>(in, lam, feat, fs, core, C); },
+      [&](const Box& rem) { adj_feature_kernel<<<interior_grid(rem), 256, 0, s>>>(in, lam, feat, fs, rem, C); });
+  adj_algebra_kernel<<<interior_grid(box), kAdjStride, 0, s>>>(feat, fs, lam, bars, stride, box, C);
 }
+
 void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, bool lam_padded, const double* bars, long stride,
                        const FieldsOut& out, const Box& box, const Coef& C, bool accumulate, hipStream_t s) {
   const long N = (long)box.n[0] * box.n[1] * box.n[2];

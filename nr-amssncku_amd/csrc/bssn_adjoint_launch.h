@@ -7,10 +7,10 @@ namespace amss {
 // RHS k at the interior points (no auxiliary outputs), GAUGE 0.
 void adj_launch_rhs_k(const FieldsIn& in, const FieldsOut& out, const Box& box, const Coef& C, hipStream_t s);
 // Reverse of the pointwise algebra: feature bars at interior points (feature
-// f of point p at bars[f * stride + p]).  feat (kAdjInputs x N doubles, or
-// null): work array for the stencil values, computed by a separate kernel.
+// f of point p at bars[f * stride + p]).  feat (kAdjInputs x N doubles): work
+// array for the stencil values, computed by a separate kernel.
 void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, long stride, const Box& box,
-                      const Coef& C, hipStream_t s, double* feat = nullptr);
+                      const Coef& C, hipStream_t s, double* feat);
 // Transposed stencils: out (= or +=) dJ/d(in) at every point.  The bars
 // arrays must be zero outside the interior and readable 3 planes beyond the
 // array on either side; so must lam when lam_padded (else lam is read only
