@@ -60,6 +60,9 @@ struct LdsTile {
 // or two single buffers.
 constexpr int kLdsBuffers = AMSS_LDS_PAIRS ? 4 : 2;
 
+#ifndef AMSS_TOP
+#define AMSS_TOP 1
+#endif
 #ifndef AMSS_DMA_AT
 #define AMSS_DMA_AT 1
 #endif
@@ -84,6 +87,7 @@ struct LdsEnv {
   int tid;        // thread index in the workgroup
   int g = 0;      // next group to open
   bool bad = false;  // a NaN among the centre values of the first 21 fields (bssn_rhs.f90:110-113)
+  double top[NFIELD];  // centre values the algebra reads before its first group (in_top)
   int xf[2] = {-1, -1}, nxf[2] = {-1, -1};  // group_extra of the open (next) pair's fields
   double xv[2], nxv[2];                      // ... and their centre values, loaded a pair ahead
 
@@ -135,7 +139,16 @@ struct LdsEnv {
     if constexpr (!std::is_void_v<Self>) static_cast<Self*>(this)->sink_rotate();
   }
 
+  // the centre values bssn_rhs_point reads before it opens a group
+  __device__ __forceinline__ static constexpr bool in_top(int v) {
+    return v == CHI || v == TRK || v == LAP || (v >= DXX && v <= AZZ) || (v >= BETAX && v <= BETAX + 2);
+  }
   __device__ __forceinline__ void prologue() {
+    if (AMSS_TOP) {
+#pragma unroll
+      for (int v = 0; v < NFIELD; ++v)
+        if (in_top(v)) top[v] = F.ptr(v)[p];
+    }
     if (AMSS_LDS_PAIRS) ahead(0);
     __builtin_amdgcn_sched_barrier(0);
     issue(group_field(0), slot(0));
@@ -175,6 +188,7 @@ struct LdsEnv {
   // the centre value of the open group's field is in its halo tile; the
   // pair's extra centre values were loaded at its barrier
   __device__ __forceinline__ double val(int v) const {
+    if (AMSS_TOP && g == 0 && in_top(v)) return top[v];
     if (g > 0 && v == group_field(g - 1)) return cur()[lt];
     if (v == xf[0]) return xv[0];
     if (v == xf[1]) return xv[1];
