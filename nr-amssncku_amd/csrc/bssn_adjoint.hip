@@ -215,80 +215,99 @@ struct Taps {
 // of (stencil weight) x (feature bar at p).  The bars are zero outside the
 // interior and readable 3 planes beyond the array, so they are read at every
 // offset unchecked; lam likewise when LAMPAD (else lam, and beta with it, is
-// read only at interior points).  blockIdx.y = field; a workgroup is 256
-// consecutive points (one thread per point per field beat both one thread per
-// point for all fields and 16 x 4 x 4 bricks).
+// read only at interior points).  A workgroup is 256 consecutive points of
+// one group of 4 fields (blockIdx.y): the upwind weights of beta at the 21
+// output points a field's advection term reads are formed once per point and
+// used for all 4 fields; the groups mix fields with and without Hessians so
+// that they cost about the same.
+constexpr int kGatherGroups = 6, kGatherPer = 4;
+__constant__ signed char kGatherField[kGatherGroups][kGatherPer] = {
+    {CHI, DXX, TRK, AXX}, {GXY, GXZ, GAMX, AXY}, {DYY, GYZ, GAMY, AXZ},
+    {DZZ, LAP, GAMZ, AYY}, {BETAX, BETAX + 1, AYZ, AZZ}, {BETAX + 2, DTSFX, DTSFX + 1, DTSFX + 2}};
 template <bool LAMPAD>
 __global__ void __launch_bounds__(256) adj_gather_kernel(FieldsIn in, FieldsIn lam, const double* __restrict__ bars,
                                                          long bs, FieldsOut out, Box box, Coef C, int accumulate) {
   const long N = (long)box.n[0] * box.n[1] * box.n[2];
   const long q = blockIdx.x * 256L + threadIdx.x;
   if (q >= N) return;
-  const int v = blockIdx.y;
   const int i = (int)(q % box.n[0]), j = (int)((q / box.n[0]) % box.n[1]), k = (int)(q / ((long)box.n[0] * box.n[1]));
   const int sy = (int)box.sy, sz = (int)box.sz, sz3 = 3 * sz;
   const int st[3] = {1, sy, sz};
-  double acc = bars[v * bs + q];
-  const int gs = adj_grad_slot(v);
-  if (gs >= 0) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const Taps b(bars + adj_grad_index(gs, a) * bs, q, sz3);
-      const int s = st[a];
-      acc += C.d1[a] * ((b(2 * s) - b(-2 * s)) + 8.0 * (b(-s) - b(s)));
-    }
-  }
-  const int hs = adj_hess_slot(v);
-  if (hs >= 0) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {  // diagonal second derivatives
-      const Taps b(bars + adj_hess_index(hs, a == 0 ? 0 : (a == 1 ? 3 : 5)) * bs, q, sz3);
-      const int s = st[a];
-      acc += C.d2[a] * (16.0 * (b(-s) + b(s)) - (b(-2 * s) + b(2 * s)) - 30.0 * b(0));
-    }
-    const int ta[3] = {0, 0, 1}, tb[3] = {1, 2, 2}, tt[3] = {1, 2, 4};
-    constexpr int os[4] = {-2, -1, 1, 2};
-    constexpr double w[4] = {1.0, -8.0, 8.0, -1.0};  // d1 weights at offsets -2, -1, 1, 2
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {  // mixed: the 4 x 4 cross stencil is its own transpose
-      const Taps b(bars + adj_hess_index(hs, tt[m]) * bs, q, sz3);
-      const int sa = st[ta[m]], sb = st[tb[m]];
-      double s = 0.0;
-#pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        double r = 0.0;
-#pragma unroll
-        for (int y = 0; y < 4; ++y) r += w[y] * b(os[x] * sa + os[y] * sb);
-        s += w[x] * r;
-      }
-      acc += C.dm[m] * s;
-    }
-  }
-  // advection and KO: both read lam at the output points p = q - o e_a
-  // (the KO stencil is symmetric, so its transpose uses the same points)
+  // advection + KO weights at p = q - o e_a: d1 (up(o) max(beta,0) - up(-o) min(beta,0)) + ko kw(o)
   constexpr double up[7] = {0.0, 0.0, -3.0, -10.0, 18.0, -6.0, 1.0};  // offsets -3..3 from p
   constexpr double kw[7] = {1.0, -6.0, 15.0, -20.0, 15.0, -6.0, 1.0};
+  double wt[3][7];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const Taps L(lam.p[v], q, sz3), B(in.p[BETAX + a], q, sz3);
-    double s = 0.0, t = 0.0;
+    const Taps B(in.p[BETAX + a], q, sz3);
 #pragma unroll
     for (int o = -3; o <= 3; ++o) {
+      bool use = true;
       if (!LAMPAD) {
         const int pi = i - (a == 0 ? o : 0), pj = j - (a == 1 ? o : 0), pk = k - (a == 2 ? o : 0);
-        if (!inside(box, pi, pj, pk)) continue;
+        use = inside(box, pi, pj, pk);
       }
-      const double b = B(-o * st[a]), l = L(-o * st[a]);
-      // up weight at offset o; the downwind stencil is its mirror
-      s += up[o + 3] * fmax(b, 0.0) * l - up[3 - o] * fmin(b, 0.0) * l;
-      t += kw[o + 3] * l;
+      const double b = use ? B(-o * st[a]) : 0.0;
+      wt[a][o + 3] = use ? C.d1[a] * (up[o + 3] * fmax(b, 0.0) - up[3 - o] * fmin(b, 0.0)) + C.ko[a] * kw[o + 3] : 0.0;
     }
-    acc += C.d1[a] * s + C.ko[a] * t;
   }
-  if (accumulate)
-    out.p[v][q] += acc;
-  else
-    out.p[v][q] = acc;
+  for (int f = 0; f < kGatherPer; ++f) {
+    const int v = kGatherField[blockIdx.y][f];
+    double acc = bars[v * bs + q];
+    const int gs = adj_grad_slot(v);
+    if (gs >= 0) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const Taps b(bars + adj_grad_index(gs, a) * bs, q, sz3);
+        const int s = st[a];
+        acc += C.d1[a] * ((b(2 * s) - b(-2 * s)) + 8.0 * (b(-s) - b(s)));
+      }
+    }
+    const int hs = adj_hess_slot(v);
+    if (hs >= 0) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {  // diagonal second derivatives
+        const Taps b(bars + adj_hess_index(hs, a == 0 ? 0 : (a == 1 ? 3 : 5)) * bs, q, sz3);
+        const int s = st[a];
+        acc += C.d2[a] * (16.0 * (b(-s) + b(s)) - (b(-2 * s) + b(2 * s)) - 30.0 * b(0));
+      }
+      const int ta[3] = {0, 0, 1}, tb[3] = {1, 2, 2}, tt[3] = {1, 2, 4};
+      constexpr int os[4] = {-2, -1, 1, 2};
+      constexpr double w[4] = {1.0, -8.0, 8.0, -1.0};  // d1 weights at offsets -2, -1, 1, 2
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {  // mixed: the 4 x 4 cross stencil is its own transpose
+        const Taps b(bars + adj_hess_index(hs, tt[m]) * bs, q, sz3);
+        const int sa = st[ta[m]], sb = st[tb[m]];
+        double s = 0.0;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          double r = 0.0;
+#pragma unroll
+          for (int y = 0; y < 4; ++y) r += w[y] * b(os[x] * sa + os[y] * sb);
+          s += w[x] * r;
+        }
+        acc += C.dm[m] * s;
+      }
+    }
+    // advection and KO read lam at the output points p = q - o e_a (the KO
+    // stencil is symmetric, so its transpose uses the same points)
+    const Taps L(lam.p[v], q, sz3);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int o = -3; o <= 3; ++o) {
+        if (!LAMPAD) {
+          const int pi = i - (a == 0 ? o : 0), pj = j - (a == 1 ? o : 0), pk = k - (a == 2 ? o : 0);
+          if (!inside(box, pi, pj, pk)) continue;
+        }
+        acc += wt[a][o + 3] * L(-o * st[a]);
+      }
+    }
+    if (accumulate)
+      out.p[v][q] += acc;
+    else
+      out.p[v][q] = acc;
+  }
 }
 
 // ---- enforce_ga and its vector-Jacobian product ------------------------------
@@ -467,7 +486,7 @@ void adj_launch_point(const FieldsIn& in, const FieldsIn& lam, double* bars, lon
 void adj_launch_gather(const FieldsIn& in, const FieldsIn& lam, bool lam_padded, const double* bars, long stride,
                        const FieldsOut& out, const Box& box, const Coef& C, bool accumulate, hipStream_t s) {
   const long N = (long)box.n[0] * box.n[1] * box.n[2];
-  const dim3 g(nblk(N), NFIELD);
+  const dim3 g(nblk(N), kGatherGroups);
   if (lam_padded)
     adj_gather_kernel<true><<<g, 256, 0, s>>>(in, lam, bars, stride, out, box, C, accumulate ? 1 : 0);
   else
