@@ -51,7 +51,10 @@ int amss_set_gauge(int gauge);
  * rho, Sx..Sz, Sxx..Szz, the 18 physical Christoffels, the 6 physical Ricci
  * components, ham/mom x3/Gam x3 residuals (written only when co == 0),
  * Symmetry, Lev, eps, co.  Returns 1 (nothing written) when the sum of the
- * first 21 state arrays is NaN.  Symmetry != 0 returns 4. */
+ * first 21 state arrays is NaN.  Symmetry 1 (equatorial: z) and 2 (octant:
+ * x, y, z) mirror the stencils across a lower face whose first coordinate is
+ * within one spacing of 0 (diff_new.f90:975-980), with each field's parity;
+ * other values return 2. */
 int compute_rhs_bssn_(int *ex, double *T, double *X, double *Y, double *Z,
                       double *chi, double *trK, double *dxx, double *gxy, double *gxz, double *dyy,
                       double *gyz, double *dzz, double *Axx, double *Axy, double *Axz, double *Ayy,
@@ -103,11 +106,14 @@ void sommerfeld_rout_(int *ex, double *X, double *Y, double *Z, double *xmin, do
  * ex: shape with ghosts; h: dX, dY, dZ (= X[1]-X[0] ...); state/rhs: 24
  * device pointers in the reference field order; matter: 10 pointers or NULL
  * (vacuum); gam (18), ric (6): required; res (7): required when co == 0.
+ * symmetry: the reference's Symmetry (0, 1, 2), as in compute_rhs_bssn_;
+ * origin: X[0], Y[0], Z[0] of the block (read only when symmetry != 0).
  * stream: a hipStream_t or NULL.  flag (device int, may be NULL) is set to 1
  * when an input is NaN; the call itself does not synchronise. */
 int amss_rhs_bssn_dev(const int *ex, const double *h, const double *const *state, double *const *rhs,
                       const double *const *matter, double *const *gam, double *const *ric, double *const *res,
-                      int symmetry, double eps, int co, int gauge, int *flag, void *stream);
+                      int symmetry, const double *origin, double eps, int co, int gauge, int *flag,
+                      void *stream);
 
 /* enforce_ga on device arrays (12 pointers: dxx gxy gxz dyy gyz dzz Axx..Azz). */
 int amss_enforce_ga_dev(const int *ex, double *const *ga, void *stream);

@@ -79,3 +79,35 @@ def sommerfeld_copy(f0, f, X, Y, Z, bbox, symmetry=0):
     dummy = np.zeros_like(f0)
     lib().sommerfeld_rout_(_ex(f.shape), _p(X), _p(Y), _p(Z), *[_d(b) for b in bbox], _d(0.0), _p(dummy),
                            _p(dummy), _p(f0), _p(f), soa, _i(symmetry), _i(1))
+
+
+def compute_rhs_dev(state, h, eps, co=1, symmetry=0, origin=(0.0, 0.0, 0.0), gauge=0, matter=None, stream=None):
+    """amss_rhs_bssn_dev, the device-resident form of compute_rhs_bssn_, on
+    torch tensors: state is (24, nz, ny, nx) float64 on the GPU (matter: 10
+    such arrays or None for vacuum).  Returns dict(flag, rhs, gam, ric, res)
+    of device tensors after synchronising the stream."""
+    import torch
+    assert state.dtype == torch.float64 and state.is_cuda and state.is_contiguous() and state.shape[0] == NF
+    shape = tuple(state.shape[1:])
+    dev = state.device
+
+    def ptrs(t):
+        return (C.c_void_p * t.shape[0])(*[t[i].data_ptr() for i in range(t.shape[0])])
+
+    rhs = torch.zeros((NF,) + shape, dtype=torch.float64, device=dev)
+    gam = torch.zeros((18,) + shape, dtype=torch.float64, device=dev)
+    ric = torch.zeros((6,) + shape, dtype=torch.float64, device=dev)
+    res = torch.zeros((7,) + shape, dtype=torch.float64, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    mat = None
+    if matter is not None:
+        assert matter.shape[0] == 10 and matter.is_contiguous()
+        mat = ptrs(matter)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    hh = (C.c_double * 3)(*h)
+    org = (C.c_double * 3)(*origin)
+    check(lib().amss_rhs_bssn_dev(_ex(shape), hh, ptrs(state), ptrs(rhs), mat, ptrs(gam), ptrs(ric), ptrs(res),
+                                  symmetry, org, eps, co, gauge, C.c_void_p(flag.data_ptr()),
+                                  C.c_void_p(s.cuda_stream)), "amss_rhs_bssn_dev")
+    s.synchronize()
+    return dict(flag=int(flag.item()), rhs=rhs, gam=gam, ric=ric, res=res)

@@ -61,6 +61,20 @@ static Box make_box(const int* ex, int lo, int hi_off) {
   return b;
 }
 
+// Symmetry planes at the lower faces (diff_new.f90:975-980,
+// lopsidediff.f90:230-232, kodiss.f90:156-158): z for Symmetry 1 and 2, x and
+// y for Symmetry 2, where the first coordinate lies within one spacing of 0.
+// Stencils there read mirror points down to index -3 (mirrors of 0..2).
+static int set_mirrors(Box& b, int symmetry, const double* x0, const double* h, const char* who) {
+  if (symmetry < 0 || symmetry > 2) return fail(AMSS_ERR_ARG, std::string(who) + ": Symmetry must be 0, 1 or 2");
+  for (int a = 0; a < 3; ++a) {
+    const bool on = (a == 2 ? symmetry > 0 : symmetry > 1) && std::fabs(x0[a]) < h[a];
+    if (on && b.n[a] < 3) return fail(AMSS_ERR_ARG, std::string(who) + ": a symmetry axis needs 3 points");
+    b.mlo[a] = on ? -3 : 0;
+  }
+  return 0;
+}
+
 // ---- a grow-only device workspace + stream for the host-pointer entry points
 struct HostShim {
   std::mutex mu;
@@ -100,7 +114,10 @@ static int rhs_host(const int* ex, double* X, double* Y, double* Z, double* cons
                     double* const* mat, double* const* gam, double* const* ric, double* const* res, int symmetry,
                     double eps, int co) {
   if (!ex || ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "compute_rhs_bssn_: shape below 2");
-  if (symmetry != 0) return fail(AMSS_ERR_UNSUPPORTED, "compute_rhs_bssn_: only Symmetry = 0 is provided");
+  const double h[3] = {X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0]};
+  const double x0[3] = {X[0], Y[0], Z[0]};
+  Box box = make_box(ex, 3, 3);
+  if (int r = set_mirrors(box, symmetry, x0, h, "compute_rhs_bssn_")) return r;
   const long n = (long)ex[0] * ex[1] * ex[2];
   if (sum_is_nan(st, n)) return AMSS_ERR_NONFINITE;
   std::lock_guard<std::mutex> lk(g_shim.mu);
@@ -125,10 +142,8 @@ static int rhs_host(const int* ex, double* X, double* Y, double* Z, double* cons
   for (int a = 0; a < 18; ++a) aux.gam[a] = o + (NFIELD + a) * n;
   for (int a = 0; a < 6; ++a) aux.ric[a] = o + (NFIELD + 18 + a) * n;
   for (int a = 0; a < 7; ++a) aux.res[a] = o + (NFIELD + 24 + a) * n;
-  const double h[3] = {X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0]};
   const Coef C = make_coef(h, eps > 0 ? eps : 0.0);
   HIPCHK(hipMemsetAsync(g_shim.flag, 0, sizeof(int), g_shim.s));
-  const Box box = make_box(ex, 3, 3);
   launch_rhs(in, mt, out, aux, box, C, g_gauge, co, g_shim.flag, true, g_shim.s);
   HIPCHK(hipGetLastError());
   for (int v = 0; v < NFIELD; ++v) HIPCHK(hipMemcpyAsync(rhs[v], out.p[v], bytes, hipMemcpyDeviceToHost, g_shim.s));
@@ -340,12 +355,15 @@ static std::mutex g_zero_mu;
 
 int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state, double* const* rhs,
                       const double* const* matter, double* const* gam, double* const* ric, double* const* res,
-                      int symmetry, double eps, int co, int gauge, int* flag, void* stream) {
-  if (!ex || !h || !state || !rhs || !gam || !ric || (co == 0 && !res))
+                      int symmetry, const double* origin, double eps, int co, int gauge, int* flag,
+                      void* stream) {
+  if (!ex || !h || !state || !rhs || !gam || !ric || (co == 0 && !res) || (symmetry != 0 && !origin))
     return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: null argument");
-  if (symmetry != 0) return fail(AMSS_ERR_UNSUPPORTED, "amss_rhs_bssn_dev: only Symmetry = 0 is provided");
   if (gauge != 0 && gauge != 2) return fail(AMSS_ERR_UNSUPPORTED, "amss_rhs_bssn_dev: gauge 0 or 2");
   if (ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: shape below 2");
+  Box box = make_box(ex, 3, 3);
+  if (symmetry != 0)
+    if (int r = set_mirrors(box, symmetry, origin, h, "amss_rhs_bssn_dev")) return r;
   const long n = (long)ex[0] * ex[1] * ex[2];
   if (n >= (1L << 29)) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: more than 2^29 points per block");
   hipStream_t s = (hipStream_t)stream;
@@ -381,7 +399,7 @@ int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state
     fl = g_shim.flag;
   }
   const Coef C = make_coef(h, eps > 0 ? eps : 0.0);
-  launch_rhs(in, mt, out, aux, make_box(ex, 3, 3), C, gauge, co, fl, true, s);
+  launch_rhs(in, mt, out, aux, box, C, gauge, co, fl, true, s);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -102,19 +102,24 @@ struct EdgeEnv {
   const Coef& C;
   int i, j, k;
   int n[3];
+  int lo[3];  // lowest readable index per axis (0, or -3 on a symmetry plane)
   long sy, sz;
   bool last, kook;
   int ord;
 
   __device__ EdgeEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, int i_, int j_, int k_,
-                     const int* n_, long sy_, long sz_)
+                     const int* n_, const int* lo_, long sy_, long sz_)
       : F(F_), M(M_), C(C_), i(i_), j(j_), k(k_), sy(sy_), sz(sz_) {
-    n[0] = n_[0];
-    n[1] = n_[1];
-    n[2] = n_[2];
+    for (int a = 0; a < 3; ++a) {
+      n[a] = n_[a];
+      lo[a] = lo_[a];
+    }
     last = (i == n[0] - 1) || (j == n[1] - 1) || (k == n[2] - 1);
+    // the joint three-axis room test of fderivs/fdderivs/kodis
+    // (diff_new.f90:996-1100, 1397-1524, kodiss.f90:162-212)
     auto rm = [&](int r) {
-      return i - r >= 0 && i + r <= n[0] - 1 && j - r >= 0 && j + r <= n[1] - 1 && k - r >= 0 && k + r <= n[2] - 1;
+      return i - r >= lo[0] && i + r <= n[0] - 1 && j - r >= lo[1] && j + r <= n[1] - 1 && k - r >= lo[2] &&
+             k + r <= n[2] - 1;
     };
     ord = last ? 0 : (rm(2) ? 2 : (rm(1) ? 1 : 0));
     kook = rm(3);
@@ -205,9 +210,9 @@ struct EdgeEnv {
       if (pos + 2 <= nn - 1) return b * c * ce();
       if (pos + 1 <= nn - 1) return -b * c * dn();
     } else if (b < 0.0) {
-      if (pos - 3 >= 0) return -b * c * dn();
-      if (pos - 2 >= 0) return b * c * ce();
-      if (pos - 1 >= 0) return b * c * up();
+      if (pos - 3 >= lo[a]) return -b * c * dn();
+      if (pos - 2 >= lo[a]) return b * c * ce();
+      if (pos - 1 >= lo[a]) return b * c * up();
     }
     return 0.0;
   }

@@ -10,11 +10,17 @@ constexpr int kTileY = 8;
 constexpr int kRhsThreads = kTileX * kTileY;
 
 // An array of shape n (with ghosts), strides sy, sz, and a box [lo, hi).
+// mlo: lowest index a boundary stencil may read along each axis, 0 or -3
+// where the array's lower face lies on a symmetry plane (Symmetry 1: z,
+// Symmetry 2: x, y, z; diff_new.f90:975-980, lopsidediff.f90:230-232,
+// kodiss.f90:156-158); a negative index reads the mirror point with the
+// field's parity on that axis.
 struct Box {
   int n[3];
   int lo[3];
   int hi[3];
   long sy, sz;
+  int mlo[3] = {0, 0, 0};
 };
 
 // Widths of the boundary layers filled after a sub-step, per face.

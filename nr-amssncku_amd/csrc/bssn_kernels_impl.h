@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) rhs_edge_kernel(FieldsIn in, FieldsIn mat
   if (!edge_index(box, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
-  EdgeEnv E(in, mat, C, i, j, k, box.n, box.sy, box.sz);
+  EdgeEnv E(in, mat, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
   RhsSink S{out, aux, p};
   bssn_rhs_point<GAUGE, CO, true, true>(E, S);
 }
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) rk4_edge_kernel(FieldsIn in, FieldsIn u0,
   if (!edge_index(box, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
-  EdgeEnv E(in, in, C, i, j, k, box.n, box.sy, box.sz);
+  EdgeEnv E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
   Rk4Sink S{u0, acc, out, aux, p, R.stage, R.c, R.tinny, {}};
   bssn_rhs_point<GAUGE, CO, false, false>(E, S);
   S.finish();

@@ -48,6 +48,23 @@ def test_version_and_error_text_without_gpu():
     assert L.amss_set_gauge(0) == 0
 
 
+def test_rhs_symmetry_arguments_rejected_before_any_gpu_work():
+    """Symmetry outside 0..2, and a symmetry axis of fewer than 3 points,
+    are argument errors (code 2), raised before the library touches a GPU."""
+    import numpy as np
+    import amssbssn
+    from amssbssn import fortran_abi as fa
+    X = (np.arange(6) + 0.5) * 0.3
+    st = [np.zeros((6, 6, 6)) for _ in range(24)]
+    with pytest.raises(RuntimeError, match="Symmetry must be 0, 1 or 2"):
+        fa.compute_rhs(st, X, X, X, 0.02, symmetry=3)
+    Z = (np.arange(2) + 0.5) * 0.3
+    st2 = [np.zeros((2, 6, 6)) for _ in range(24)]
+    with pytest.raises(RuntimeError, match="symmetry axis needs 3 points"):
+        fa.compute_rhs(st2, X, X, Z, 0.02, symmetry=1)
+    assert "3 points" in amssbssn.last_error()
+
+
 def test_config_geometry():
     import amssbssn as A
     c0 = A.config0()

@@ -85,6 +85,66 @@ def test_rhs_flat_space_is_zero(fa):
     assert max(float(np.max(np.abs(a))) for a in g["rhs"]) == 0.0
 
 
+def _plane_grid(n, h, at_plane):
+    """Cell-centred coordinates starting half a spacing above a symmetry
+    plane, or well away from it."""
+    return (np.arange(n) + 0.5) * h if at_plane else -1.7 + np.arange(n) * h
+
+
+@pytest.mark.parametrize("sym", [1, 2])
+@pytest.mark.parametrize("gauge", [0, 2])
+@pytest.mark.parametrize("co", [0, 1])
+def test_rhs_symmetry_planes_match_oracle(fa, orc, sym, gauge, co):
+    """Equatorial (z) and octant (x, y, z) symmetry: stencils at the lower
+    faces read mirror points with each field's parity, every degraded-order
+    and lopsided branch moving with the lowered bound."""
+    nx, ny, nz = 11, 12, 13
+    X, Y, Z = _plane_grid(nx, 0.3, True), _plane_grid(ny, 0.27, True), _plane_grid(nz, 0.33, True)
+    st = random_state((nz, ny, nx), seed=60 + 4 * sym + 2 * gauge + co)
+    fa.set_gauge(gauge)
+    g = fa.compute_rhs(st, X, Y, Z, 0.02, co=co, symmetry=sym)
+    o = orc.compute_rhs(st, X, Y, Z, 0.02, co=co, symmetry=sym, gauge=gauge)
+    assert g["gont"] == 0 and o["gont"] == 0
+    worst = _cmp_rhs(g, o, co, f"symmetry {sym} gauge {gauge} co {co}")
+    print(f"symmetry {sym} gauge {gauge} co {co}: worst rel err {worst:.2e}")
+    # the planes change the lower-face values only
+    z = orc.compute_rhs(st, X, Y, Z, 0.02, co=co, symmetry=0, gauge=gauge)
+    assert not np.array_equal(z["rhs"][0][0], o["rhs"][0][0])
+
+
+@pytest.mark.parametrize("faces", [(False, False, True), (True, False, True), (False, True, False)])
+def test_rhs_octant_with_some_faces_off_the_plane(fa, orc, faces):
+    """Symmetry 2 mirrors only the axes whose first coordinate is within one
+    spacing of 0."""
+    nx, ny, nz = 10, 9, 11
+    X, Y, Z = (_plane_grid(n, hh, on) for n, hh, on in zip((nx, ny, nz), (0.3, 0.27, 0.33), faces))
+    st = random_state((nz, ny, nx), seed=71)
+    fa.set_gauge(0)
+    g = fa.compute_rhs(st, X, Y, Z, 0.02, co=0, symmetry=2)
+    o = orc.compute_rhs(st, X, Y, Z, 0.02, co=0, symmetry=2, gauge=0)
+    _cmp_rhs(g, o, 0, f"octant faces {faces}")
+
+
+@pytest.mark.parametrize("sym", [0, 2])
+def test_device_rhs_entry_point_matches_oracle(fa, orc, sym):
+    """amss_rhs_bssn_dev on device arrays, with matter and the residuals."""
+    import torch
+    nx, ny, nz = 12, 10, 11
+    X, Y, Z = _plane_grid(nx, 0.3, sym > 0), _plane_grid(ny, 0.27, sym > 0), _plane_grid(nz, 0.33, sym > 0)
+    st = random_state((nz, ny, nx), seed=80 + sym)
+    rng = np.random.default_rng(81)
+    mat = [0.01 * rng.standard_normal((nz, ny, nx)) for _ in range(10)]
+    dev = torch.device("cuda:0")
+    S = torch.from_numpy(np.stack(st)).to(dev)
+    M = torch.from_numpy(np.stack(mat)).to(dev)
+    g = fa.compute_rhs_dev(S, (X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0]), 0.02, co=0, symmetry=sym,
+                           origin=(X[0], Y[0], Z[0]), gauge=0, matter=M)
+    o = orc.compute_rhs(st, X, Y, Z, 0.02, co=0, symmetry=sym, gauge=0, matter=mat)
+    assert g["flag"] == 0
+    gh = {k: [a.cpu().numpy() for a in g[k]] for k in ("rhs", "gam", "ric", "res")}
+    _cmp_rhs(gh, o, 0, f"device symmetry {sym}")
+
+
 def test_pointwise_abi_entry_points_match_oracle(fa, orc):
     nx, ny, nz = 10, 11, 12
     X, Y, Z, bbox = grid(nx, ny, nz)
