@@ -98,6 +98,16 @@ void sommerfeld_rout_(int *ex, double *X, double *Y, double *Z, double *xmin, do
                       double *xmax, double *ymax, double *zmax, double *dT, double *chi0, double *Lap0,
                       double *f0, double *f, double *SoA, int *Symmetry, int *precor);
 
+/* Replaces f_sommerfeld_routbam (sommerfeld_rout.h:34-36; sommerfeld_rout.f90:223-449):
+ * on the outermost layer of every physical face of the bbox, f_rhs becomes
+ * -velocity (x.grad f0 + f0) / r (second-order one-sided differences from
+ * the side the outgoing wave arrives from); SoA: f0's parity per axis, used
+ * for the mirrored layer below a symmetry plane.  Shapes below 3 or Symmetry
+ * outside 0..2 leave f_rhs untouched and set amss_last_error(). */
+void sommerfeld_routbam_(int *ex, double *X, double *Y, double *Z, double *xmin, double *ymin, double *zmin,
+                         double *xmax, double *ymax, double *zmax, double *f_rhs, double *f0, double *velocity,
+                         double *SoA, int *Symmetry);
+
 /* ---------------------------------------------------------------------------
  * 2. Device-resident API
  * ------------------------------------------------------------------------- */
@@ -127,10 +137,15 @@ typedef struct amss_level amss_level;
  *    outer boundary of the benchmark configurations);
  *  AMSS_BC_COPY_LAYER: the reference's sommerfeld_rout(precor = 1) one-layer
  *    copy on the patch faces, with the reference's stencils up to the edge;
- *  AMSS_BC_NONE: no boundary step. */
+ *  AMSS_BC_NONE: no boundary step;
+ *  AMSS_BC_SOMMERFELD: the reference's level-0 outer boundary: each stage's
+ *    RHS on the outermost layer of the patch faces is replaced by
+ *    sommerfeld_routbam's before the update (bssn_class.C:3106-3117,
+ *    3476-3484), with the reference's stencils up to the edge. */
 #define AMSS_BC_FROZEN_GHOSTS 0
 #define AMSS_BC_COPY_LAYER 1
 #define AMSS_BC_NONE 2
+#define AMSS_BC_SOMMERFELD 3
 
 /* Grid-function lists of a Block (bssn_class.C:412-535). */
 #define AMSS_LIST_STATE 0 /* StateList     (t^n)            */

@@ -1,7 +1,8 @@
 """Calls of the library's Fortran-ABI replacements (compute_rhs_bssn_,
 rungekutta4_rout_, enforce_ga_, lowerboundset_, l2normhelper_,
-sommerfeld_rout_) on host numpy arrays, with the same Python signatures as
-the reference build's, so a caller switches by changing the library.
+sommerfeld_rout_, sommerfeld_routbam_) on host numpy arrays, with the same
+Python signatures as the reference build's, so a caller switches by
+changing the library.
 Arrays: float64, shape (nz, ny, nx), C-contiguous.
 """
 from __future__ import annotations
@@ -79,6 +80,13 @@ def sommerfeld_copy(f0, f, X, Y, Z, bbox, symmetry=0):
     dummy = np.zeros_like(f0)
     lib().sommerfeld_rout_(_ex(f.shape), _p(X), _p(Y), _p(Z), *[_d(b) for b in bbox], _d(0.0), _p(dummy),
                            _p(dummy), _p(f0), _p(f), soa, _i(symmetry), _i(1))
+
+
+def sommerfeld_bam(f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
+    """sommerfeld_routbam_ on f_rhs in place."""
+    sa = (C.c_double * 3)(*soa)
+    lib().sommerfeld_routbam_(_ex(f0.shape), _p(X), _p(Y), _p(Z), *[_d(b) for b in bbox], _p(f_rhs), _p(f0),
+                              _d(velocity), sa, _i(symmetry))
 
 
 def compute_rhs_dev(state, h, eps, co=1, symmetry=0, origin=(0.0, 0.0, 0.0), gauge=0, matter=None, stream=None):

@@ -12,7 +12,7 @@ import numpy as np
 from ._lib import check, lib, AmssError
 from .grid import NF, Config
 
-BC_FROZEN_GHOSTS, BC_COPY_LAYER, BC_NONE = 0, 1, 2
+BC_FROZEN_GHOSTS, BC_COPY_LAYER, BC_NONE, BC_SOMMERFELD = 0, 1, 2, 3
 LIST_STATE, LIST_OLD, LIST_PRE, LIST_COR, LIST_RHS, LIST_RES = range(6)
 MODE_PLAIN, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
 

@@ -348,6 +348,56 @@ void sommerfeld_rout_(int* ex, double* X, double* Y, double* Z, double* xmin, do
   sommerfeld_host(ex, X, Y, Z, bb, f0, f, *Symmetry);
 }
 
+// Faces and mirrored layer of sommerfeld_routbam (sommerfeld_rout.f90:255-325):
+// the physical faces are those of the copy branch; below a symmetry plane one
+// layer is read as the mirror of index 0.
+static SommerGeom sommer_geom(const int* ex, const double* X, const double* Y, const double* Z, const double* bb,
+                              int symmetry) {
+  SommerGeom g;
+  bool on[6];
+  copy_faces(ex, X, Y, Z, bb, symmetry, on);
+  const double* C[3] = {X, Y, Z};
+  for (int a = 0; a < 3; ++a) {
+    g.x0[a] = C[a][0];
+    g.h[a] = C[a][1] - C[a][0];
+    const bool plane = (a == 2 ? symmetry > 0 : symmetry > 1) && std::fabs(C[a][0]) < g.h[a];
+    g.lo[a] = plane ? -1 : 0;
+  }
+  for (int f = 0; f < 6; ++f)
+    if (on[f]) g.faces |= 1 << f;
+  return g;
+}
+
+static int sommer_bam_host(int* ex, double* X, double* Y, double* Z, const double* bb, double* f_rhs, double* f0,
+                           double vel, const double* SoA, int symmetry) {
+  if (!ex || ex[0] < 3 || ex[1] < 3 || ex[2] < 3) return fail(AMSS_ERR_ARG, "sommerfeld_routbam_: shape below 3");
+  if (symmetry < 0 || symmetry > 2) return fail(AMSS_ERR_ARG, "sommerfeld_routbam_: Symmetry must be 0, 1 or 2");
+  const SommerGeom g = sommer_geom(ex, X, Y, Z, bb, symmetry);
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure(2 * (size_t)n)) return r;
+  double *d0 = g_shim.buf, *d1 = d0 + n;
+  const size_t b = n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(d0, f0, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(d1, f_rhs, b, hipMemcpyHostToDevice, g_shim.s));
+  Shape sh;
+  for (int a = 0; a < 3; ++a) sh.n[a] = ex[a];
+  sh.sy = ex[0];
+  sh.sz = (long)ex[0] * ex[1];
+  launch_sommer_bam(d0, d1, sh, g, vel, SoA, g_shim.s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(f_rhs, d1, b, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+void sommerfeld_routbam_(int* ex, double* X, double* Y, double* Z, double* xmin, double* ymin, double* zmin,
+                         double* xmax, double* ymax, double* zmax, double* f_rhs, double* f0, double* velocity,
+                         double* SoA, int* Symmetry) {
+  const double bb[6] = {*xmin, *ymin, *zmin, *xmax, *ymax, *zmax};
+  sommer_bam_host(ex, X, Y, Z, bb, f_rhs, f0, *velocity, SoA, *Symmetry);
+}
+
 // ---- device-resident API ----------------------------------------------------
 static double* g_zero = nullptr;  // zero matter arrays for vacuum calls
 static size_t g_zero_cap = 0;

@@ -130,6 +130,7 @@ class BssnLevel {
   std::vector<var*> vars_;
   std::vector<GhostSeg> segs_;
   std::vector<Coef> coef_;
+  std::vector<SommerGeom> sommer_;  // per patch; no faces unless bc_ == AMSS_BC_SOMMERFELD
   int gauge_, bc_, gw_;
   double eps_, chitiny_;
   hipStream_t stream_ = nullptr;

@@ -124,7 +124,7 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
                      int bc, int gw)
     : gauge_(gauge), bc_(bc), gw_(gw), eps_(eps), chitiny_(chitiny) {
   if (gauge != 0 && gauge != 2) { err_ = "gauge must be 0 or 2"; return; }
-  if (bc < 0 || bc > 2) { err_ = "unknown boundary mode"; return; }
+  if (bc < 0 || bc > 3) { err_ = "unknown boundary mode"; return; }
   if (gw < 0 || gw > 8) { err_ = "ghost width out of range"; return; }
   // grid-function registry: 5 time levels x 24 fields, then 7 residuals
   // (bssn_class.C:183-361), parity vectors as in the reference.
@@ -205,6 +205,18 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
       C.dm[q] = (1.0 / 144.0) / (hh[pa[q]] * hh[pb[q]]);
       C.dmlo[q] = 0.25 / (hh[pa[q]] * hh[pb[q]]);
     }
+    // Sommerfeld faces: those of the patch bbox the array reaches
+    // (sommerfeld_rout.f90:272-325, Symmetry 0)
+    SommerGeom g;
+    if (bc == AMSS_BC_SOMMERFELD) {
+      for (int a = 0; a < 3; ++a) {
+        g.x0[a] = b->X[a][0];
+        g.h[a] = hh[a];
+        if (std::fabs(b->X[a][sh[a] - 1] - bb[3 + a]) < hh[a]) g.faces |= 1 << a;
+        if (std::fabs(b->X[a][0] - bb[a]) < hh[a]) g.faces |= 1 << (3 + a);
+      }
+    }
+    sommer_.push_back(g);
   }
   LCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   LCHK(hipMalloc(&dflag_, sizeof(int)));
@@ -366,7 +378,8 @@ int BssnLevel::Step(double dT, bool want_res) {
         Box all = box;
         for (int a = 0; a < 3; ++a) { all.lo[a] = 3; all.hi[a] = b.shape[a] - 3; }
         launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, all,
-                   coef_[pi], R, gauge_, want_res && it == 0, dflag_, true, stream_, /*interior=*/false);
+                   coef_[pi], R, gauge_, want_res && it == 0, dflag_, true, stream_, /*interior=*/false,
+                   sommer_[pi]);
       }
       if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
         Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
@@ -495,7 +508,7 @@ void BssnLevel::build_batch_tables() {
   // most 6 disjoint slabs (z faces whole, then y faces, then x faces), less
   // the points an exchange box overwrites
   std::vector<std::vector<IBox>> fill(np), cover(np);
-  if (bc_ != AMSS_BC_NONE) {
+  if (bc_ == AMSS_BC_FROZEN_GHOSTS || bc_ == AMSS_BC_COPY_LAYER) {
     for (int pi = 0; pi < np; ++pi) {
       const Block& b = *blocks_[pi];
       int wl[3], wh[3];
@@ -640,7 +653,7 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
         all.sz = (long)b.shape[0] * b.shape[1];
         for (int a = 0; a < 7; ++a) aux.res[a] = b.fgfs[kResidualBase + a];
         launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, all,
-                   coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false);
+                   coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false, sommer_[pi]);
       }
     }
     // The state is the same in all sub-steps, so sub-step 0 tests it for NaN

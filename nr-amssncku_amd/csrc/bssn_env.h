@@ -324,6 +324,86 @@ struct Rk4Sink {
   }
 };
 
+// ---------------------------------------------------------------------------
+// BAM Sommerfeld outer boundary (sommerfeld_rout.f90:223-449): on the
+// outermost point layer of each physical face the RHS of field v becomes
+// -s_v (x.grad f + f) / r, f the stage input, s_v the field's propagation
+// speed, with second-order differences taken from the side an outgoing wave
+// arrives from (x_a > 0: backward), centred or reversed where the array
+// leaves no room.
+struct SommerGeom {
+  double x0[3] = {0.0, 0.0, 0.0};  // coordinate of index 0 along each axis
+  double h[3] = {1.0, 1.0, 1.0};
+  int faces = 0;           // bit a: the upper face of axis a is physical; bit 3 + a: the lower one
+  int lo[3] = {0, 0, 0};   // lowest readable index: -1 below a symmetry plane (one mirrored layer)
+};
+
+__host__ __device__ __forceinline__ bool sommer_point(const SommerGeom& G, const int n[3], int i, int j, int k) {
+  const int p[3] = {i, j, k};
+  bool on = false;
+  for (int a = 0; a < 3; ++a)
+    on = on || (((G.faces >> a) & 1) && p[a] == n[a] - 1) || (((G.faces >> (3 + a)) & 1) && p[a] == 0);
+  return on;
+}
+
+// Propagation speed: sqrt(2) for chi, trK and the lapse (the 1+log gauge
+// speed, bssn_class.C:390-410), 1 for the other fields (var.C:30).
+__host__ __device__ __forceinline__ double prop_speed(int v) {
+  return (v == CHI || v == TRK || v == LAP) ? 1.4142135623730951 : 1.0;
+}
+
+// The boundary RHS at point p (sommerfeld_rout.f90:376-440).  rd(a, m): the
+// stage input m points along axis a (|m| <= 2; index -1 is the mirror of 0).
+// Where w = s x_a / r is 0 the reference leaves the derivative unset; x_a = 0
+// there, so its term vanishes, and here it is 0.
+template <class Rd>
+__device__ __forceinline__ double sommer_rhs(const SommerGeom& G, const int n[3], const int p[3], double vel,
+                                             Rd rd) {
+  double x[3];
+  for (int a = 0; a < 3; ++a) x[a] = G.x0[a] + p[a] * G.h[a];
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double f = rd(0, 0);
+  double s = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double w = vel * x[a] / r, d2 = 0.5 / G.h[a];
+    const bool room_dn2 = p[a] - 2 >= G.lo[a], room_dn1 = p[a] - 1 >= G.lo[a];
+    const bool room_up2 = p[a] + 2 <= n[a] - 1, room_up1 = p[a] + 1 <= n[a] - 1;
+    // 0: backward, 1: centred, 2: forward
+    int form = -1;
+    if (w > 0.0) form = room_dn2 ? 0 : (room_dn1 ? 1 : 2);
+    else if (w < 0.0) form = room_up2 ? 2 : (room_up1 ? 1 : 0);
+    double d = 0.0;
+    if (form == 0) d = d2 * (3.0 * f - 4.0 * rd(a, -1) + rd(a, -2));
+    else if (form == 1) d = d2 * (-rd(a, -1) + rd(a, 1));
+    else if (form == 2) d = d2 * (-rd(a, 2) + 4.0 * rd(a, 1) - 3.0 * f);
+    s += d * x[a];
+  }
+  return -(vel * (s + f)) / r;
+}
+
+// RK4 epilogue of a boundary-kernel point: on a Sommerfeld face layer each
+// field's RHS is replaced by the boundary condition's before the update, as
+// the reference applies sommerfeld_routbam to the stage's RHS before
+// rungekutta4_rout (bssn_class.C:3106-3117, 3476-3484).
+template <class Env>
+struct SommerRk4Sink {
+  Rk4Sink S;
+  const Env& E;
+  const SommerGeom& G;
+  bool on;
+  __device__ __forceinline__ void put(int v, double k) {
+    if (on) {
+      const int p[3] = {E.i, E.j, E.k};
+      k = sommer_rhs(G, E.n, p, prop_speed(v), [&](int a, int m) { return E.ax(v, a, m); });
+    }
+    S.put(v, k);
+  }
+  __device__ __forceinline__ void gam(int a, double x) { S.gam(a, x); }
+  __device__ __forceinline__ void ric(int a, double x) { S.ric(a, x); }
+  __device__ __forceinline__ void res(int a, double x) { S.res(a, x); }
+  __device__ __forceinline__ void finish() { S.finish(); }
+};
+
 // Same epilogue with the sub-step fixed at compile time: the interior kernels
 // use this form, since a runtime stage makes the compiler duplicate the whole
 // point computation per stage branch (and blows the register budget).

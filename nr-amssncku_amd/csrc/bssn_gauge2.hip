@@ -10,8 +10,8 @@ void launch_rhs_g2(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out
 
 void launch_rk4_g2(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                    const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                   bool edge, hipStream_t s, bool interior) {
-  launch_rk4_gauge<2>(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior);
+                   bool edge, hipStream_t s, bool interior, const SommerGeom& sg) {
+  launch_rk4_gauge<2>(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior, sg);
 }
 
 }  // namespace amss

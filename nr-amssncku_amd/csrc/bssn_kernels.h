@@ -50,15 +50,20 @@ void launch_rhs_g0(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out
 void launch_rhs_g2(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux, const Box& box,
                    const Coef& C, int co, int* flag, bool edge, hipStream_t s);
 // RHS fused with RK4 sub-step R.stage (residuals written for stage 0 when want_res).
+// sg: Sommerfeld faces, applied by the boundary kernel (edge) on their layers.
 void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                 const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
-                int* flag, bool edge, hipStream_t s, bool interior = true);
+                int* flag, bool edge, hipStream_t s, bool interior = true, const SommerGeom& sg = SommerGeom{});
 void launch_rk4_g0(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                    const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                   bool edge, hipStream_t s, bool interior);
+                   bool edge, hipStream_t s, bool interior, const SommerGeom& sg);
 void launch_rk4_g2(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                    const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                   bool edge, hipStream_t s, bool interior);
+                   bool edge, hipStream_t s, bool interior, const SommerGeom& sg);
+// sommerfeld_routbam on one field: f_rhs on the face layers of sg from f0
+// (parity S per axis for the mirrored layer), velocity vel.
+void launch_sommer_bam(const double* f0, double* f_rhs, const Shape& n, const SommerGeom& sg, double vel,
+                       const double S[3], hipStream_t s);
 void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,
                        double tinny, int* flag, hipStream_t s);
 void launch_seg_copy(const FieldsIn& src, const FieldsOut& dst, const SegDesc& d, hipStream_t s);

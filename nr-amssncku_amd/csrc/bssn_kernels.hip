@@ -160,9 +160,40 @@ void launch_rhs(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, c
 
 void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                 const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
-                int* flag, bool edge, hipStream_t s, bool interior) {
-  if (gauge == 2) launch_rk4_g2(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior);
-  else launch_rk4_g0(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior);
+                int* flag, bool edge, hipStream_t s, bool interior, const SommerGeom& sg) {
+  if (gauge == 2) launch_rk4_g2(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior, sg);
+  else launch_rk4_g0(in, u0, acc, out, aux, box, C, R, want_res, flag, edge, s, interior, sg);
+}
+
+// sommerfeld_routbam (sommerfeld_rout.f90:223-449) on one field: a thread per
+// array point; points off the face layers return at once.
+__global__ void __launch_bounds__(256) sommer_bam_kernel(const double* __restrict__ f0, double* __restrict__ frhs,
+                                                         Shape n, SommerGeom sg, double vel, double S0, double S1,
+                                                         double S2) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long N = (long)n.n[0] * n.n[1] * n.n[2];
+  if (t >= N) return;
+  const int p[3] = {(int)(t % n.n[0]), (int)((t / n.n[0]) % n.n[1]), (int)(t / ((long)n.n[0] * n.n[1]))};
+  if (!sommer_point(sg, n.n, p[0], p[1], p[2])) return;
+  const double S[3] = {S0, S1, S2};
+  auto rd = [&](int a, int m) {
+    int q[3] = {p[0], p[1], p[2]};
+    q[a] += m;
+    double sgn = 1.0;
+    if (q[a] < 0) {  // the one mirrored layer below a symmetry plane
+      q[a] = -q[a] - 1;
+      sgn = S[a];
+    }
+    return f0[q[0] + q[1] * n.sy + q[2] * n.sz] * sgn;
+  };
+  frhs[t] = sommer_rhs(sg, n.n, p, vel, rd);
+}
+
+void launch_sommer_bam(const double* f0, double* f_rhs, const Shape& n, const SommerGeom& sg, double vel,
+                       const double S[3], hipStream_t s) {
+  const long N = (long)n.n[0] * n.n[1] * n.n[2];
+  if (N > 0 && sg.faces != 0)
+    sommer_bam_kernel<<<blocks_for(N, 256), 256, 0, s>>>(f0, f_rhs, n, sg, vel, S[0], S[1], S[2]);
 }
 
 void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,

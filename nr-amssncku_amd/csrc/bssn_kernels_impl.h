@@ -110,15 +110,17 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
   if (X.bad) *flag = 1;
 }
 
+// sg: the Sommerfeld faces of the array (none when sg.faces == 0)
 template <int GAUGE, bool CO>
 __global__ void __launch_bounds__(256) rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
-                                                       AuxOut aux, Box box, Coef C, Rk4Param R, int* flag) {
+                                                       AuxOut aux, Box box, Coef C, Rk4Param R, int* flag,
+                                                       SommerGeom sg) {
   int i, j, k;
   if (!edge_index(box, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
   EdgeEnv E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
-  Rk4Sink S{u0, acc, out, aux, p, R.stage, R.c, R.tinny, {}};
+  SommerRk4Sink<EdgeEnv> S{{u0, acc, out, aux, p, R.stage, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
   bssn_rhs_point<GAUGE, CO, false, false>(E, S);
   S.finish();
 }
@@ -146,7 +148,7 @@ void launch_rhs_gauge(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& 
 template <int G>
 void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                       const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, bool want_res, int* flag,
-                      bool edge, hipStream_t s, bool interior) {
+                      bool edge, hipStream_t s, bool interior, const SommerGeom& sg) {
   // The LDS kernel's 16-byte direct loads start at even x of a row, which is
   // 16-byte aligned only when the row pitch is even; odd pitches go direct.
   if (interior && !box_empty(box) && R.variant == 1 && (box.sy % 2) == 0) {
@@ -174,7 +176,7 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
       Box rem = box;
       for (int c = 0; c < a; ++c) rem.hi[c] = core.hi[c];
       rem.lo[a] = core.hi[a];
-      if (!box_empty(rem)) launch_rk4_gauge<G>(in, u0, acc, out, aux, rem, C, Rd, want_res, flag, false, s, true);
+      if (!box_empty(rem)) launch_rk4_gauge<G>(in, u0, acc, out, aux, rem, C, Rd, want_res, flag, false, s, true, sg);
     }
   } else if (interior && !box_empty(box)) {
     const dim3 gr = interior_grid(box);  // also variant 1 on an odd row pitch
@@ -191,9 +193,9 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
   if (edge) {
     const long N = (long)box.n[0] * box.n[1] * box.n[2];
     if (want_res && R.stage == 0)
-      rk4_edge_kernel<G, true><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);
+      rk4_edge_kernel<G, true><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag, sg);
     else
-      rk4_edge_kernel<G, false><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag);
+      rk4_edge_kernel<G, false><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag, sg);
   }
 }
 

@@ -938,6 +938,75 @@ void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const 
     }
 }
 
+/* One axis of sommerfeld_routbam's derivative (sommerfeld_rout.f90:383-436):
+ * second-order differences taken from the side the outgoing wave arrives
+ * from (w = v x_a / r > 0: backward), centred or reversed where the array
+ * (lowest index lo, -1 below a symmetry plane) leaves no room.  The reference
+ * leaves fx unset when w == 0, i.e. x_a == 0, where fx x_a vanishes anyway;
+ * here it is 0. */
+static double sbam_dir(const line_t *L, double w, int pos, int lo, int hi, double d2)
+{
+    if (w > 0.0) {
+        if (pos - 2 >= lo) return d2 * (3.0 * lp(L, 0) - 4.0 * lp(L, -1) + lp(L, -2));
+        if (pos - 1 >= lo) return d2 * (-lp(L, -1) + lp(L, 1));
+        return d2 * (-lp(L, 2) + 4.0 * lp(L, 1) - 3.0 * lp(L, 0));
+    } else if (w < 0.0) {
+        if (pos + 2 <= hi) return d2 * (-lp(L, 2) + 4.0 * lp(L, 1) - 3.0 * lp(L, 0));
+        if (pos + 1 <= hi) return d2 * (-lp(L, -1) + lp(L, 1));
+        return d2 * (3.0 * lp(L, 0) - 4.0 * lp(L, -1) + lp(L, -2));
+    }
+    return 0.0;
+}
+
+void orc_sommerfeld_routbam(const int *ex, const double *X, const double *Y, const double *Z,
+                            const double *bbox, double *f_rhs, const double *f0, double velocity,
+                            const double *SoA, int symmetry)
+{
+    const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+    const double d2[3] = {1.0 / 2.0 / dX, 1.0 / 2.0 / dY, 1.0 / 2.0 / dZ};
+    /* lowest readable index: one mirrored layer below a symmetry plane
+     * (sommerfeld_rout.f90:262-268, symmetry_bd(1, ...)) */
+    const int lo[3] = {symmetry > 1 && fabs(X[0]) < dX ? -1 : 0, symmetry > 1 && fabs(Y[0]) < dY ? -1 : 0,
+                       symmetry > 0 && fabs(Z[0]) < dZ ? -1 : 0};
+    const int n0 = ex[0] - 1, n1 = ex[1] - 1, n2 = ex[2] - 1;
+    const int hi[3] = {n0, n1, n2};
+    /* the outermost layer of each physical face (:272-325) */
+    int lay[6][6], on[6] = {0, 0, 0, 0, 0, 0};
+    if (fabs(X[n0] - bbox[3]) < dX) { on[0] = 1; int t[6] = {n0, 0, 0, n0, n1, n2}; memcpy(lay[0], t, sizeof t); }
+    if (fabs(Y[n1] - bbox[4]) < dY) { on[1] = 1; int t[6] = {0, n1, 0, n0, n1, n2}; memcpy(lay[1], t, sizeof t); }
+    if (fabs(Z[n2] - bbox[5]) < dZ) { on[2] = 1; int t[6] = {0, 0, n2, n0, n1, n2}; memcpy(lay[2], t, sizeof t); }
+    if (fabs(X[0] - bbox[0]) < dX && !(symmetry == 2 && fabs(bbox[0]) < dX / 2)) {
+        on[3] = 1; int t[6] = {0, 0, 0, 0, n1, n2}; memcpy(lay[3], t, sizeof t); }
+    if (fabs(Y[0] - bbox[1]) < dY && !(symmetry == 2 && fabs(bbox[1]) < dY / 2)) {
+        on[4] = 1; int t[6] = {0, 0, 0, n0, 0, n2}; memcpy(lay[4], t, sizeof t); }
+    if (fabs(Z[0] - bbox[2]) < dZ && !(symmetry > 0 && fabs(bbox[2]) < dZ / 2)) {
+        on[5] = 1; int t[6] = {0, 0, 0, n0, n1, 0}; memcpy(lay[5], t, sizeof t); }
+    for (int gp = 0; gp < 6; gp++) {
+        if (!on[gp]) continue;
+        for (int k = lay[gp][2]; k <= lay[gp][5]; k++)
+            for (int j = lay[gp][1]; j <= lay[gp][4]; j++)
+                for (int i = lay[gp][0]; i <= lay[gp][3]; i++) {
+                    const double x = X[i], y = Y[j], z = Z[k];
+                    const double R = sqrt(x * x + y * y + z * z);
+                    const double w[3] = {velocity * x / R, velocity * y / R, velocity * z / R};
+                    const int pos[3] = {i, j, k};
+                    double fd[3];
+                    line_t L = {f0, ex, SoA, 0.0, i, j, k, 0};
+                    for (int a = 0; a < 3; a++) {
+                        L.ax = a;
+                        fd[a] = sbam_dir(&L, w[a], pos[a], lo[a], hi[a], d2[a]);
+                    }
+                    f_rhs[IX(ex, i, j, k)] =
+                        -(velocity * (fd[0] * x + fd[1] * y + fd[2] * z + f0[IX(ex, i, j, k)])) / R;
+                }
+    }
+}
+
+/* Propagation speed of each field in the Sommerfeld condition: sqrt(2) for
+ * chi, trK and the lapse (the 1+log gauge speed), 1 for the rest
+ * (bssn_class.C:390-410, var.C:30). */
+static double propspeed(int v) { return (v == 0 || v == 1 || v == 17) ? sqrt(2.0) : 1.0; }
+
 /* Frozen-ghost boundary: every point within gw of an array face takes f0. */
 static void ghost_copy(const int *ex, int gw, const double *f0, double *f)
 {
@@ -973,6 +1042,8 @@ int orc_bssn_step(const int *ex, const double *X, const double *Y, const double 
     err |= orc_compute_rhs_bssn(ex, 0.0, X, Y, Z, (const double *const *)U, acc, NULL, gam, ric, res, symmetry,
                                 lev, eps, 0, gauge);
     for (int v = 0; v < NF; v++) {
+        /* level-0 BAM Sommerfeld on the RHS before the update (bssn_class.C:3106-3117) */
+        if (bc == 3) orc_sommerfeld_routbam(ex, X, Y, Z, bbox, acc[v], U[v], propspeed(v), PAR[v], symmetry);
         orc_rungekutta4_rout(ex, dT, U[v], pre[v], acc[v], 0);
         if (bc == 0) ghost_copy(ex, gw, U[v], pre[v]);
         else if (bc == 1) orc_sommerfeld_copy(ex, X, Y, Z, bbox, U[v], pre[v], symmetry);
@@ -985,6 +1056,8 @@ int orc_bssn_step(const int *ex, const double *X, const double *Y, const double 
         err |= orc_compute_rhs_bssn(ex, 0.0, X, Y, Z, (const double *const *)pre, cor, NULL, gam, ric, res,
                                     symmetry, lev, eps, 1, gauge);
         for (int v = 0; v < NF; v++) {
+            /* bssn_class.C:3476-3484: the stage's k, from the stage input */
+            if (bc == 3) orc_sommerfeld_routbam(ex, X, Y, Z, bbox, cor[v], pre[v], propspeed(v), PAR[v], symmetry);
             orc_rungekutta4_rout(ex, dT, U[v], cor[v], acc[v], it);
             if (bc == 0) ghost_copy(ex, gw, U[v], cor[v]);
             else if (bc == 1) orc_sommerfeld_copy(ex, X, Y, Z, bbox, U[v], cor[v], symmetry);

@@ -62,13 +62,20 @@ double orc_l2normhelper(const int *ex, const double *X, const double *Y, const d
 /* sommerfeld_rout.f90:9-221, precor == 1 (copy) branch */
 void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const double *Z,
                          const double *bbox, const double *f0, double *f, int symmetry);
+/* sommerfeld_rout.f90:223-449 (sommerfeld_routbam): on the outermost layer
+ * of each physical face of bbox, f_rhs = -velocity (x.grad f0 + f0) / r. */
+void orc_sommerfeld_routbam(const int *ex, const double *X, const double *Y, const double *Z,
+                            const double *bbox, double *f_rhs, const double *f0, double velocity,
+                            const double *SoA, int symmetry);
 
 /* One RK4 step of one Block in the order of bssn_class::Step
  * (bssn_class.C:2981-3901) for a single rank and a single patch.
  * U: 24 arrays, overwritten with the new state.  bc: 0 = copy the U0 value
  * onto every point within gw of the array faces after each sub-step
  * (frozen ghost zones), 1 = the reference's one-layer precor copy
- * (sommerfeld_rout with cor) against bbox, 2 = none.  ham (optional): the
+ * (sommerfeld_rout with cor) against bbox, 2 = none, 3 = the level-0 BAM
+ * Sommerfeld condition on each stage's RHS (sommerfeld_routbam, with each
+ * field's propagation speed and parity).  ham (optional): the
  * Hamiltonian residual written by the predictor's co=0 RHS call.
  * Returns the OR of the RHS NaN flags. */
 int orc_bssn_step(const int *ex, const double *X, const double *Y, const double *Z,

@@ -30,6 +30,9 @@ PARITY = [(1, 1, 1), (1, 1, 1), (1, 1, 1), (-1, -1, 1), (-1, 1, -1), (1, 1, 1),
           (1, -1, -1), (1, 1, 1), (1, 1, 1), (-1, -1, 1), (-1, 1, -1), (1, 1, 1),
           (1, -1, -1), (1, 1, 1), (-1, 1, 1), (1, -1, 1), (1, 1, -1), (1, 1, 1),
           (-1, 1, 1), (1, -1, 1), (1, 1, -1), (-1, 1, 1), (1, -1, 1), (1, 1, -1)]
+# Propagation speed in the Sommerfeld condition: sqrt(2) for chi, trK and the
+# lapse (bssn_class.C:390-410), 1 otherwise (var.C:30).
+PROPSPEED = [2 ** 0.5 if v in (0, 1, 17) else 1.0 for v in range(NF)]
 GAM_NAMES = ["Gamxxx", "Gamxxy", "Gamxxz", "Gamxyy", "Gamxyz", "Gamxzz",
              "Gamyxx", "Gamyxy", "Gamyxz", "Gamyyy", "Gamyyz", "Gamyzz",
              "Gamzxx", "Gamzxy", "Gamzxz", "Gamzyy", "Gamzyz", "Gamzzz"]
@@ -137,6 +140,13 @@ class Orc:
         self.lib.orc_sommerfeld_copy(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, _ptr(f0), _ptr(f),
                                      C.c_int(symmetry))
 
+    def sommerfeld_bam(self, f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
+        """sommerfeld_routbam on f_rhs in place."""
+        bb = (C.c_double * 6)(*bbox)
+        sa = (C.c_double * 3)(*soa)
+        self.lib.orc_sommerfeld_routbam(_ex(f0.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, _ptr(f_rhs), _ptr(f0),
+                                        C.c_double(velocity), sa, C.c_int(symmetry))
+
     def step(self, U, X, Y, Z, bbox, dT, eps, chitiny=1e-5, symmetry=0, gauge=None, bc=0, gw=3, want_ham=False):
         """One RK4 step in place on U (list of 24 arrays). Returns (gont, ham)."""
         gauge = self.gauge_default if gauge is None else gauge
@@ -242,6 +252,12 @@ class Ref:
                                   self._d(0.0), _ptr(dummy), _ptr(dummy), _ptr(f0), _ptr(f), soa,
                                   self._i(symmetry), self._i(1))
 
+    def sommerfeld_bam(self, f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
+        """sommerfeld_routbam_ on f_rhs in place."""
+        sa = (C.c_double * 3)(*soa)
+        self.lib.sommerfeld_routbam_(_ex(f0.shape), _ptr(X), _ptr(Y), _ptr(Z), *[self._d(b) for b in bbox],
+                                     _ptr(f_rhs), _ptr(f0), self._d(velocity), sa, self._i(symmetry))
+
     def symmetry_bd(self, ord_, f, sym):
         nz, ny, nx = f.shape
         out = np.zeros((nz + ord_, ny + ord_, nx + ord_))
@@ -278,12 +294,17 @@ def step_driver(be, U, X, Y, Z, bbox, dT, eps, chitiny, symmetry, bc, gw, want_h
         elif bc == 1:
             be.sommerfeld_copy(f0, f, X, Y, Z, bbox, symmetry)
 
+    def bam(f_rhs, f0, v):
+        if bc == 3:  # level-0 Sommerfeld on the stage's RHS (bssn_class.C:3106-3117, 3476-3484)
+            be.sommerfeld_bam(f_rhs, f0, X, Y, Z, bbox, PROPSPEED[v], PARITY[v], symmetry)
+
     be.enforce_ga(U)
     o = be.compute_rhs(U, X, Y, Z, eps, co=0, symmetry=symmetry)
     gont = o["gont"]
     acc = o["rhs"]
     ham = o["res"][0].copy() if want_ham else None
     for v in range(NF):
+        bam(acc[v], U[v], v)
         be.rk4(U[v], pre[v], acc[v], dT, 0)
         bcf(U[v], pre[v])
     be.lowerboundset(pre[0], tinny)
@@ -293,6 +314,7 @@ def step_driver(be, U, X, Y, Z, bbox, dT, eps, chitiny, symmetry, bc, gw, want_h
         gont |= o["gont"]
         for v in range(NF):
             np.copyto(cor[v], o["rhs"][v])
+            bam(cor[v], pre[v], v)
             be.rk4(U[v], cor[v], acc[v], dT, it)
             bcf(U[v], cor[v])
         be.lowerboundset(cor[0], tinny)

@@ -96,7 +96,7 @@ def test_batched_split_with_remainders_equals_whole(mode):
         assert np.max(np.abs(w[:, :, n // 2:] - b)) <= 1e-12 * m, v
 
 
-@pytest.mark.parametrize("bc", [1, 2])
+@pytest.mark.parametrize("bc", [1, 2, 3])
 def test_batched_other_boundary_modes_match_per_patch(bc):
     import amssbssn as A
     cfg = _uneven(npatch=3, seed=7)

@@ -199,7 +199,7 @@ def _level_vs_oracle(orc, cfg, gauge, bc, nsteps, tol, variant=0):
 # tiles of 8x8x4 points on the tile-aligned part of the interior, variant 0 on
 # the remainder slabs (n = 18 leaves a 2-point remainder on every axis).
 @pytest.mark.parametrize("variant", [0, 1])
-@pytest.mark.parametrize("bc", [0, 1, 2])
+@pytest.mark.parametrize("bc", [0, 1, 2, 3])
 def test_one_rk4_step_matches_oracle(orc, bc, variant):
     import amssbssn as A
     cfg = A.config1(n=18)
@@ -227,6 +227,45 @@ def test_gauge2_steps_match_oracle(orc, variant):
     import amssbssn as A
     cfg = A.config1(n=14)
     _level_vs_oracle(orc, cfg, 2, 0, 5, 1e-10, variant)
+
+
+@pytest.mark.parametrize("gauge", [0, 2])
+def test_sommerfeld_boundary_steps_match_oracle(orc, gauge):
+    """The level-0 BAM Sommerfeld boundary over several steps, where the
+    boundary layers feed back into the interior."""
+    import amssbssn as A
+    cfg = A.config1(n=16)
+    worst = _level_vs_oracle(orc, cfg, gauge, A.BC_SOMMERFELD, 6, 1e-10, 1)
+    print(f"Sommerfeld gauge {gauge}, 6 steps: worst {worst:.2e}")
+    # the boundary is visible at that tolerance: without it the state differs
+    p = cfg.patches[0]
+    X, Y, Z = p.coords()
+    U3, U2 = cfg.initial_state(0), cfg.initial_state(0)
+    for _ in range(6):
+        orc.step(U3, X, Y, Z, p.bbox, cfg.dt, cfg.eps, cfg.chitiny, gauge=gauge, bc=A.BC_SOMMERFELD, gw=p.gw)
+        orc.step(U2, X, Y, Z, p.bbox, cfg.dt, cfg.eps, cfg.chitiny, gauge=gauge, bc=A.BC_NONE, gw=p.gw)
+    assert max(rel_err(a, b) for a, b in zip(U3, U2)) > 1e-6
+
+
+def test_sommerfeld_routbam_abi_matches_oracle(fa, orc):
+    """sommerfeld_routbam_ on host arrays: faces on and off the bbox,
+    symmetry planes (one mirrored layer with the parity), both speeds."""
+    rng = np.random.default_rng(21)
+    X, Y, Z, bbox = grid(9, 10, 11)
+    P = [(np.arange(n) + 0.5) * h for n, h in ((9, 0.3), (10, 0.27), (11, 0.33))]
+    pb = (0.0, 0.0, 0.0, 9 * 0.3, 10 * 0.27, 11 * 0.33)
+    cases = [(X, Y, Z, bbox, 0), (P[0], P[1], P[2], pb, 1), (P[0], P[1], P[2], pb, 2),
+             (X, Y, Z, (bbox[0] - 1.0, bbox[1], bbox[2], bbox[3], bbox[4] + 1.0, bbox[5]), 0)]
+    for X_, Y_, Z_, bb, sym in cases:
+        shape = (len(Z_), len(Y_), len(X_))
+        f0 = rng.standard_normal(shape)
+        r0 = rng.standard_normal(shape)
+        for vel, par in ((1.0, (1, 1, 1)), (2 ** 0.5, (-1, 1, -1)), (1.0, (1, -1, -1))):
+            g, o = r0.copy(), r0.copy()
+            fa.sommerfeld_bam(g, f0, X_, Y_, Z_, bb, vel, par, sym)
+            orc.sommerfeld_bam(o, f0, X_, Y_, Z_, bb, vel, par, sym)
+            assert rel_err(g, o) <= 1e-14, (sym, vel, par)
+            assert np.array_equal(g == r0, o == r0)  # the same points written
 
 
 def test_lds_variant_matches_direct_kernel():

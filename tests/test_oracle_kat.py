@@ -159,6 +159,39 @@ def test_puncture_rhs_reflection_symmetry(orc):
             assert np.max(np.abs(r[flip] - par * r)) <= 1e-12 * m, (v, axis)
 
 
+@pytest.mark.parametrize("sym", [0, 2])
+def test_sommerfeld_bam_exact_on_quadratics(orc, sym):
+    """Every difference of sommerfeld_routbam (backward, centred, forward;
+    second order) is exact on quadratics, so on each face layer the result
+    is -v (x.grad f + f) / r of the polynomial; interior points keep their
+    value.  Octant symmetry with an even polynomial: the mirrored reads at
+    the planes see the same polynomial."""
+    if sym == 0:
+        X, Y, Z, bbox = grid(9, 10, 11)
+        f_of = lambda x, y, z: 0.3 + 2 * x ** 2 - x * y + 3 * z ** 2 - 0.5 * y + z  # noqa: E731
+        grad = lambda x, y, z: (4 * x - y, -x - 0.5, 6 * z + 1)  # noqa: E731
+        par = (1, 1, 1)
+    else:
+        X, Y, Z = ((np.arange(n) + 0.5) * h for n, h in ((9, 0.3), (10, 0.27), (11, 0.33)))
+        bbox = (0.0, 0.0, 0.0, 9 * 0.3, 10 * 0.27, 11 * 0.33)
+        f_of = lambda x, y, z: 0.3 + 2 * x ** 2 + y ** 2 - 3 * z ** 2  # noqa: E731
+        grad = lambda x, y, z: (4 * x, 2 * y, -6 * z)  # noqa: E731
+        par = (1, 1, 1)
+    x, y, z = mesh(X, Y, Z)
+    f = f_of(x, y, z)
+    r = np.sqrt(x * x + y * y + z * z)
+    gx, gy, gz = grad(x, y, z)
+    want = -1.3 * (gx * x + gy * y + gz * z + f) / r
+    out = np.full(f.shape, 7.0)
+    orc.sommerfeld_bam(out, f, X, Y, Z, bbox, 1.3, par, sym)
+    layer = np.zeros(f.shape, bool)
+    layer[-1], layer[:, -1], layer[:, :, -1] = True, True, True
+    if sym == 0:
+        layer[0], layer[:, 0], layer[:, :, 0] = True, True, True
+    assert np.allclose(out[layer], want[layer], rtol=0, atol=1e-12 * np.max(np.abs(want)))
+    assert np.all(out[~layer] == 7.0)
+
+
 def test_philox_known_answers(orc):
     import amssbssn.grid as G
     kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),

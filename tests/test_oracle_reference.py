@@ -114,7 +114,39 @@ def test_pointwise_routines_bitwise(orc, ref):
         assert np.array_equal(g1, g2)
 
 
-@pytest.mark.parametrize("bc", [0, 1, 2])
+def _bam_cases():
+    """(X, Y, Z, bbox, symmetry): a box whose faces all lie on bbox and
+    straddle the origin, boxes starting at the symmetry planes, and a box
+    with some faces inside bbox."""
+    X, Y, Z, bbox = grid(9, 10, 11)
+    yield X, Y, Z, bbox, 0
+    P = [(np.arange(n) + 0.5) * h for n, h in ((9, 0.3), (10, 0.27), (11, 0.33))]
+    pb = (0.0, 0.0, 0.0, 9 * 0.3, 10 * 0.27, 11 * 0.33)
+    for sym in (0, 1, 2):
+        yield P[0], P[1], P[2], pb, sym
+    yield X, Y, Z, (bbox[0] - 1.0, bbox[1], bbox[2] - 1.0, bbox[3], bbox[4] + 1.0, bbox[5]), 0
+
+
+def test_sommerfeld_routbam_bitwise(orc, ref):
+    """sommerfeld_routbam: every face layer, both wave directions per axis,
+    every one-sided / centred fallback, mirrored reads at symmetry planes."""
+    import oracle
+    rng = np.random.default_rng(12)
+    for X, Y, Z, bbox, sym in _bam_cases():
+        shape = (len(Z), len(Y), len(X))
+        f0 = rng.standard_normal(shape)
+        r0 = rng.standard_normal(shape)
+        for vel in (1.0, 2 ** 0.5):
+            for par in PARS:
+                a, b = r0.copy(), r0.copy()
+                orc.sommerfeld_bam(a, f0, X, Y, Z, bbox, vel, par, sym)
+                ref.sommerfeld_bam(b, f0, X, Y, Z, bbox, vel, par, sym)
+                assert np.array_equal(a, b), (sym, vel, par)
+                assert not np.array_equal(a, r0)
+    assert oracle.PROPSPEED[0] == oracle.PROPSPEED[1] == oracle.PROPSPEED[17] == np.sqrt(2.0)
+
+
+@pytest.mark.parametrize("bc", [0, 1, 2, 3])
 def test_step_bitwise(orc, ref, bc):
     """The oracle's Step (C) against bssn_class::Step driven over the
     reference routines, three RK4 steps."""
