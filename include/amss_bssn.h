@@ -158,9 +158,13 @@ typedef struct amss_level amss_level;
 /* One refinement level of npatch patches (one Block each).  shapes: 3 ints
  * per patch (points, ghosts included); bboxes: 6 doubles per patch (xmin
  * ymin zmin xmax ymax zmax of the cell-centred array, X = xmin + (i+1/2) h).
- * All patches share one grid spacing.  Returns NULL on error. */
+ * All patches share one grid spacing.  symmetry: the reference's Symmetry
+ * (0; 1 equatorial, 2 octant), whose planes are the lower faces with a first
+ * coordinate within one spacing of 0: stencils there read mirror points with
+ * each field's parity and the one-layer boundaries skip those faces; it
+ * needs bc 1, 2 or 3.  Returns NULL on error. */
 amss_level *amss_level_create(int npatch, const int *shapes, const double *bboxes, int gauge, double eps,
-                              double chitiny, int bc, int gw);
+                              double chitiny, int bc, int gw, int symmetry);
 void amss_level_destroy(amss_level *L);
 /* Device pointer of grid function `field` of `list` on `patch`. */
 double *amss_level_field(amss_level *L, int patch, int list, int field);

@@ -31,7 +31,8 @@ SIGNATURES = {
     "amss_rhs_bssn_dev": ([_ip, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                            C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     "amss_enforce_ga_dev": ([_ip, C.c_void_p, C.c_void_p], C.c_int),
-    "amss_level_create": ([C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int], C.c_void_p),
+    "amss_level_create": ([C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
+                          C.c_void_p),
     "amss_level_destroy": ([C.c_void_p], None),
     "amss_level_field": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_void_p),
     "amss_level_copy": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int], C.c_int),

@@ -19,12 +19,12 @@ MODE_PLAIN, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
 
 class Level:
     def __init__(self, patches, gauge: int = 0, eps: float = 0.02, chitiny: float = 1e-5,
-                 bc: int = BC_FROZEN_GHOSTS, gw: int = 3):
+                 bc: int = BC_FROZEN_GHOSTS, gw: int = 3, symmetry: int = 0):
         L = lib()
         self.patches = list(patches)
         shapes = (C.c_int * (3 * len(patches)))(*[s for p in patches for s in p.shape])
         bbs = (C.c_double * (6 * len(patches)))(*[b for p in patches for b in p.bbox])
-        self.h = L.amss_level_create(len(patches), shapes, bbs, gauge, eps, chitiny, bc, gw)
+        self.h = L.amss_level_create(len(patches), shapes, bbs, gauge, eps, chitiny, bc, gw, symmetry)
         if not self.h:
             raise AmssError("amss_level_create: " + (L.amss_last_error() or b"").decode())
 
@@ -40,8 +40,9 @@ class Level:
             pass
 
     @classmethod
-    def from_config(cls, cfg: Config, gauge: int = 0, bc: int = BC_FROZEN_GHOSTS):
-        lv = cls(cfg.patches, gauge=gauge, eps=cfg.eps, chitiny=cfg.chitiny, bc=bc, gw=cfg.patches[0].gw)
+    def from_config(cls, cfg: Config, gauge: int = 0, bc: int = BC_FROZEN_GHOSTS, symmetry: int = 0):
+        lv = cls(cfg.patches, gauge=gauge, eps=cfg.eps, chitiny=cfg.chitiny, bc=bc, gw=cfg.patches[0].gw,
+                 symmetry=symmetry)
         for i in range(len(cfg.patches)):
             lv.set_state(i, cfg.initial_state(i))
         return lv

@@ -470,12 +470,12 @@ struct amss_level {
 };
 
 amss_level* amss_level_create(int npatch, const int* shapes, const double* bboxes, int gauge, double eps,
-                              double chitiny, int bc, int gw) {
+                              double chitiny, int bc, int gw, int symmetry) {
   if (npatch < 1 || !shapes || !bboxes) {
     fail(AMSS_ERR_ARG, "amss_level_create: bad arguments");
     return nullptr;
   }
-  auto* L = new BssnLevel(npatch, shapes, bboxes, gauge, eps, chitiny, bc, gw);
+  auto* L = new BssnLevel(npatch, shapes, bboxes, gauge, eps, chitiny, bc, gw, symmetry);
   if (!L->ok()) {
     fail(AMSS_ERR_ARG, L->err());
     delete L;

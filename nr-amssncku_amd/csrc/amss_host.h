@@ -80,7 +80,7 @@ class BssnLevel {
  public:
   struct Ev { hipEvent_t a, b; int stage; };
   BssnLevel(int npatch, const int* shapes, const double* bboxes, int gauge, double eps, double chitiny, int bc,
-            int gw);
+            int gw, int symmetry);
   ~BssnLevel();
   bool ok() const { return err_.empty(); }
   const std::string& err() const { return err_; }
@@ -131,7 +131,9 @@ class BssnLevel {
   std::vector<GhostSeg> segs_;
   std::vector<Coef> coef_;
   std::vector<SommerGeom> sommer_;  // per patch; no faces unless bc_ == AMSS_BC_SOMMERFELD
-  int gauge_, bc_, gw_;
+  std::vector<Box> edge_box_;       // per patch: the boundary kernel's box, mirror bounds at symmetry planes
+  std::vector<int> faces_;          // per patch: one-layer boundary faces (bit a upper, bit 3 + a lower)
+  int gauge_, bc_, gw_, symmetry_;
   double eps_, chitiny_;
   hipStream_t stream_ = nullptr;
   int* dflag_ = nullptr;

@@ -121,11 +121,16 @@ void Block::swapList(MyList<var>* a, MyList<var>* b) {
   } while (0)
 
 BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int gauge, double eps, double chitiny,
-                     int bc, int gw)
-    : gauge_(gauge), bc_(bc), gw_(gw), eps_(eps), chitiny_(chitiny) {
+                     int bc, int gw, int symmetry)
+    : gauge_(gauge), bc_(bc), gw_(gw), symmetry_(symmetry), eps_(eps), chitiny_(chitiny) {
   if (gauge != 0 && gauge != 2) { err_ = "gauge must be 0 or 2"; return; }
   if (bc < 0 || bc > 3) { err_ = "unknown boundary mode"; return; }
   if (gw < 0 || gw > 8) { err_ = "ghost width out of range"; return; }
+  if (symmetry < 0 || symmetry > 2) { err_ = "Symmetry must be 0, 1 or 2"; return; }
+  if (symmetry != 0 && bc == AMSS_BC_FROZEN_GHOSTS) {
+    err_ = "a symmetry plane needs a boundary mode of the reference (copy layer, none or Sommerfeld)";
+    return;
+  }
   // grid-function registry: 5 time levels x 24 fields, then 7 residuals
   // (bssn_class.C:183-361), parity vectors as in the reference.
   int sg = 0;
@@ -205,15 +210,37 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
       C.dm[q] = (1.0 / 144.0) / (hh[pa[q]] * hh[pb[q]]);
       C.dmlo[q] = 0.25 / (hh[pa[q]] * hh[pb[q]]);
     }
-    // Sommerfeld faces: those of the patch bbox the array reaches
-    // (sommerfeld_rout.f90:272-325, Symmetry 0)
+    // Symmetry planes (diff_new.f90:975-980): z for Symmetry 1 and 2, x and y
+    // for 2, where the first coordinate lies within one spacing of 0.  The
+    // boundary kernel's stencils read mirror points there.
+    Box eb;
+    for (int a = 0; a < 3; ++a) {
+      eb.n[a] = sh[a];
+      eb.lo[a] = 3;
+      eb.hi[a] = sh[a] - 3;
+      const bool plane = (a == 2 ? symmetry > 0 : symmetry > 1) && std::fabs(b->X[a][0]) < hh[a];
+      eb.mlo[a] = plane ? -3 : 0;
+    }
+    eb.sy = sh[0];
+    eb.sz = (long)sh[0] * sh[1];
+    edge_box_.push_back(eb);
+    // Faces of the one-layer boundaries: those of the patch bbox the array
+    // reaches, less the lower faces on a symmetry plane
+    // (sommerfeld_rout.f90:61-119, 272-325)
+    int fc = 0;
+    for (int a = 0; a < 3; ++a) {
+      const bool sym_face = (a == 2 ? symmetry > 0 : symmetry == 2) && std::fabs(bb[a]) < hh[a] / 2;
+      if (std::fabs(b->X[a][sh[a] - 1] - bb[3 + a]) < hh[a]) fc |= 1 << a;
+      if (std::fabs(b->X[a][0] - bb[a]) < hh[a] && !sym_face) fc |= 1 << (3 + a);
+    }
+    faces_.push_back(fc);
     SommerGeom g;
     if (bc == AMSS_BC_SOMMERFELD) {
+      g.faces = fc;
       for (int a = 0; a < 3; ++a) {
         g.x0[a] = b->X[a][0];
         g.h[a] = hh[a];
-        if (std::fabs(b->X[a][sh[a] - 1] - bb[3 + a]) < hh[a]) g.faces |= 1 << a;
-        if (std::fabs(b->X[a][0] - bb[a]) < hh[a]) g.faces |= 1 << (3 + a);
+        g.lo[a] = eb.mlo[a] < 0 ? -1 : 0;  // one mirrored layer (symmetry_bd(1, ...))
       }
     }
     sommer_.push_back(g);
@@ -375,23 +402,19 @@ int BssnLevel::Step(double dT, bool want_res) {
                  coef_[pi], R, gauge_, want_res && it == 0, dflag_, false, stream_);
       if (ev) RCHK(hipEventRecord(ev->b, stream_));
       if (edge) {
-        Box all = box;
-        for (int a = 0; a < 3; ++a) { all.lo[a] = 3; all.hi[a] = b.shape[a] - 3; }
-        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, all,
-                   coef_[pi], R, gauge_, want_res && it == 0, dflag_, true, stream_, /*interior=*/false,
-                   sommer_[pi]);
+        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux,
+                   edge_box_[pi], coef_[pi], R, gauge_, want_res && it == 0, dflag_, true, stream_,
+                   /*interior=*/false, sommer_[pi]);
       }
       if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
         Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
         launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
       } else if (bc_ == AMSS_BC_COPY_LAYER) {
-        // sommerfeld_rout(precor = 1) faces (sommerfeld_rout.f90:61-137), no symmetry
+        // sommerfeld_rout(precor = 1) faces (sommerfeld_rout.f90:61-137)
         Faces w{{0, 0, 0}, {0, 0, 0}};
-        const Patch& P = patches_[pi];
         for (int a = 0; a < 3; ++a) {
-          const double d = b.getdX(a);
-          if (std::fabs(b.X[a][b.shape[a] - 1] - P.bbox[3 + a]) < d) w.hi[a] = 1;
-          if (std::fabs(b.X[a][0] - P.bbox[a]) < d) w.lo[a] = 1;
+          w.hi[a] = (faces_[pi] >> a) & 1;
+          w.lo[a] = (faces_[pi] >> (3 + a)) & 1;
         }
         launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
       }
@@ -516,9 +539,8 @@ void BssnLevel::build_batch_tables() {
         if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
           wl[a] = wh[a] = gw_;
         } else {
-          const double d = b.getdX(a);
-          wh[a] = std::fabs(b.X[a][b.shape[a] - 1] - patches_[pi].bbox[3 + a]) < d ? 1 : 0;
-          wl[a] = std::fabs(b.X[a][0] - patches_[pi].bbox[a]) < d ? 1 : 0;
+          wh[a] = (faces_[pi] >> a) & 1;
+          wl[a] = (faces_[pi] >> (3 + a)) & 1;
         }
       }
       int lo[3] = {0, 0, 0}, hi[3] = {b.shape[0], b.shape[1], b.shape[2]};
@@ -643,17 +665,10 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
     if (bc_ != AMSS_BC_FROZEN_GHOSTS || gw_ < 3) {
       for (int pi = 0; pi < np; ++pi) {
         const Block& b = *blocks_[pi];
-        Box all;
-        for (int a = 0; a < 3; ++a) {
-          all.n[a] = b.shape[a];
-          all.lo[a] = 3;
-          all.hi[a] = b.shape[a] - 3;
-        }
-        all.sy = b.shape[0];
-        all.sz = (long)b.shape[0] * b.shape[1];
         for (int a = 0; a < 7; ++a) aux.res[a] = b.fgfs[kResidualBase + a];
-        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, all,
-                   coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false, sommer_[pi]);
+        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux,
+                   edge_box_[pi], coef_[pi], R, gauge_, res, dflag_, true, stream_, /*interior=*/false,
+                   sommer_[pi]);
       }
     }
     // The state is the same in all sub-steps, so sub-step 0 tests it for NaN

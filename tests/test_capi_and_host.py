@@ -65,6 +65,17 @@ def test_rhs_symmetry_arguments_rejected_before_any_gpu_work():
     assert "3 points" in amssbssn.last_error()
 
 
+def test_level_symmetry_needs_a_reference_boundary_mode():
+    """Frozen ghost zones have no meaning on a symmetry plane; the level
+    refuses the pair before it allocates anything."""
+    import amssbssn as A
+    p = A.PatchGeom((10, 10, 10), (0.0, 0.0, 0.0, 3.0, 3.0, 3.0), 3)
+    with pytest.raises(A.AmssError, match="symmetry plane"):
+        A.Level([p], bc=A.BC_FROZEN_GHOSTS, symmetry=1)
+    with pytest.raises(A.AmssError, match="Symmetry must be"):
+        A.Level([p], bc=A.BC_NONE, symmetry=3)
+
+
 def test_config_geometry():
     import amssbssn as A
     c0 = A.config0()

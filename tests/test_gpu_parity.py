@@ -247,6 +247,36 @@ def test_sommerfeld_boundary_steps_match_oracle(orc, gauge):
     assert max(rel_err(a, b) for a, b in zip(U3, U2)) > 1e-6
 
 
+@pytest.mark.parametrize("sym", [1, 2])
+@pytest.mark.parametrize("bc", [1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_level_with_symmetry_planes_matches_oracle(orc, sym, bc, mode):
+    """A level whose patch starts at the symmetry planes (equatorial: z;
+    octant: x, y, z), two steps, per-patch and batched: mirrored boundary
+    stencils, one-layer boundaries skipping the plane faces."""
+    import amssbssn as A
+    n = (20, 18, 22)
+    h = 0.3
+    p = A.PatchGeom(n, (0.0, 0.0, 0.0, n[0] * h, n[1] * h, n[2] * h), 3)
+    X, Y, Z = p.coords()
+    st = random_state((n[2], n[1], n[0]), seed=100 + 10 * sym + bc)
+    lv = A.Level([p], gauge=0, eps=0.02, bc=bc, symmetry=sym)
+    lv.set_state(0, st)
+    U = [x.copy() for x in st]
+    for _ in range(2):
+        orc.step(U, X, Y, Z, p.bbox, 0.075, 0.02, 1e-5, symmetry=sym, gauge=0, bc=bc, gw=3)
+    assert lv.step(0.075, 2, mode=mode) == 0
+    G = lv.get_state(0)
+    lv.close()
+    errs = [rel_err(g, u) for g, u in zip(G, U)]
+    assert max(errs) <= 1e-12, f"per-field rel errs {errs}"
+    # the planes matter at that tolerance
+    V = [x.copy() for x in st]
+    for _ in range(2):
+        orc.step(V, X, Y, Z, p.bbox, 0.075, 0.02, 1e-5, symmetry=0, gauge=0, bc=bc, gw=3)
+    assert max(rel_err(a, b) for a, b in zip(U, V)) > 1e-8
+
+
 def test_sommerfeld_routbam_abi_matches_oracle(fa, orc):
     """sommerfeld_routbam_ on host arrays: faces on and off the bbox,
     symmetry planes (one mirrored layer with the parity), both speeds."""

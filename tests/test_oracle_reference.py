@@ -114,6 +114,25 @@ def test_pointwise_routines_bitwise(orc, ref):
         assert np.array_equal(g1, g2)
 
 
+@pytest.mark.parametrize("sym", [1, 2])
+@pytest.mark.parametrize("bc", [1, 2, 3])
+def test_step_with_symmetry_planes_bitwise(orc, ref, sym, bc):
+    """Step on a block starting at the symmetry planes: mirrored stencils,
+    boundary layers skipping the plane faces."""
+    import oracle
+    X, Y, Z = ((np.arange(n) + 0.5) * 0.3 for n in (12, 13, 11))
+    bbox = (0.0, 0.0, 0.0, 12 * 0.3, 13 * 0.3, 11 * 0.3)
+    st = random_state((11, 13, 12), seed=90 + sym)
+    U1 = [x.copy() for x in st]
+    U2 = [x.copy() for x in st]
+    for _ in range(2):
+        g1, _ = orc.step(U1, X, Y, Z, bbox, 0.075, 0.02, symmetry=sym, gauge=2, bc=bc)
+        g2, _ = oracle.step_driver(ref, U2, X, Y, Z, bbox, 0.075, 0.02, 1e-5, sym, bc, 3, False)
+        assert g1 == g2 == 0
+    for a, b in zip(U1, U2):
+        assert np.array_equal(a, b)
+
+
 def _bam_cases():
     """(X, Y, Z, bbox, symmetry): a box whose faces all lie on bbox and
     straddle the origin, boxes starting at the symmetry planes, and a box
