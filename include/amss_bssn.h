@@ -186,6 +186,31 @@ int amss_level_step(amss_level *L, double dT, int nsteps, int mode, int want_ham
 int amss_level_ham_l2(amss_level *L, double *out);
 /* The stream the level's work is ordered on (a hipStream_t). */
 void *amss_level_stream(amss_level *L);
+
+/* ---- on-disk formats of the reference ------------------------------------ */
+/* One grid function of one patch in the layout of Parallel::writefile
+ * (Parallel.C:1049-1071, read by plot_binary_data.py:37-43): time (double),
+ * nx ny nz (int32), xmin xmax ymin ymax zmin zmax (double), then nx*ny*nz
+ * doubles with x fastest.  The reference names such files
+ * "<dir>/Lev%02d-%02d_<var>_%05d.bin" (Parallel.C:1361-1364); path is the
+ * caller's.  Synchronous. */
+int amss_level_write_bin(amss_level *L, const char *path, int patch, int list, int field, double time);
+/* The level as a one-level checkpoint in the layout of
+ * checkpoint::writecheck_cgh (checkpoint.C:132-191): time, levels = 1,
+ * movls, BH_num_in = nbh (int32), grids = npatch (int32), Lt = time, per
+ * patch bbox (6 doubles), shape (3 int32), handle (3 doubles; handles or
+ * zeros), nbh puncture positions porg (3 doubles each), then per patch the
+ * 24 StateList and the 24 OldStateList grid functions (the reference's
+ * CheckList, bssn_class.C:682-683).  The reference's file name is
+ * "<name>_cgh.CHK".  Synchronous. */
+int amss_level_checkpoint_write(amss_level *L, const char *path, double time, int movls, int nbh,
+                                const double *porg, const double *handles);
+/* Restore StateList and OldStateList from such a file (checkpoint.C:193-292)
+ * after checking that it holds one level whose patches have this level's
+ * shapes and bboxes.  time, movls, nbh: out; porg: room for 3*maxbh doubles
+ * (may be NULL when maxbh is 0; more punctures than maxbh is an error). */
+int amss_level_checkpoint_read(amss_level *L, const char *path, double *time, int *movls, int *nbh, double *porg,
+                               int maxbh);
 /* Interior kernel of the fused RHS + RK4 sub-step: 0 reads every stencil
  * point through the vector L1/L2 (raw buffer loads); 1 stages a halo tile of
  * each field group in LDS (8x8x4-point tiles, double-buffered, filled by

@@ -39,6 +39,10 @@ SIGNATURES = {
     "amss_level_step": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int], C.c_int),
     "amss_level_ham_l2": ([C.c_void_p, _dp], C.c_int),
     "amss_level_stream": ([C.c_void_p], C.c_void_p),
+    "amss_level_write_bin": ([C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double], C.c_int),
+    "amss_level_checkpoint_write": ([C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, _dp, _dp], C.c_int),
+    "amss_level_checkpoint_read": ([C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp,
+                                   C.c_int], C.c_int),
     "amss_level_timing": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_set_variant": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_variant": ([C.c_void_p], C.c_int),

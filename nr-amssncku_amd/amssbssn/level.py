@@ -88,6 +88,31 @@ class Level:
     def stream(self) -> int:
         return lib().amss_level_stream(self.h)
 
+    # -- the reference's on-disk formats ---------------------------------------
+    def write_bin(self, path: str, patch: int, list_: int, v: int, time: float) -> None:
+        """One grid function in Parallel::writefile's binary layout."""
+        check(lib().amss_level_write_bin(self.h, path.encode(), patch, list_, v, time), "amss_level_write_bin")
+
+    def checkpoint_write(self, path: str, time: float, movls: int = 0, porg=(), handles=None) -> None:
+        """StateList and OldStateList in checkpoint::writecheck_cgh's layout
+        (one level); porg: puncture positions, handles: 3 per patch."""
+        pts = [x for p in porg for x in p]
+        pa = (C.c_double * max(len(pts), 1))(*pts)
+        hd = None
+        if handles is not None:
+            hl = [x for p in handles for x in p]
+            hd = (C.c_double * len(hl))(*hl)
+        check(lib().amss_level_checkpoint_write(self.h, path.encode(), time, movls, len(porg), pa, hd),
+              "amss_level_checkpoint_write")
+
+    def checkpoint_read(self, path: str, maxbh: int = 16):
+        """Restore StateList and OldStateList; returns (time, movls, porg)."""
+        t, mv, nb = C.c_double(0.0), C.c_int(0), C.c_int(0)
+        pa = (C.c_double * (3 * max(maxbh, 1)))()
+        check(lib().amss_level_checkpoint_read(self.h, path.encode(), C.byref(t), C.byref(mv), C.byref(nb), pa,
+                                               maxbh), "amss_level_checkpoint_read")
+        return t.value, mv.value, [tuple(pa[3 * b:3 * b + 3]) for b in range(nb.value)]
+
     def set_variant(self, variant: int) -> None:
         """Interior kernel: 0 direct L1/L2 stencil reads, 1 LDS-staged tiles."""
         check(lib().amss_level_set_variant(self.h, variant), "amss_level_set_variant")

@@ -4,6 +4,7 @@
 // return codes plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -554,6 +555,118 @@ int amss_level_get_tune(amss_level* L, int* choice) {
   choice[0] = L->L->tune().shape;
   choice[1] = L->L->tune().grid;
   choice[2] = L->L->tune().remap;
+  return 0;
+}
+
+// ---- on-disk formats ---------------------------------------------------------
+}  // extern "C"
+namespace {
+struct File {
+  std::FILE* f = nullptr;
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+template <class T>
+bool put(std::FILE* f, const T* p, size_t n) { return std::fwrite(p, sizeof(T), n, f) == n; }
+template <class T>
+bool get(std::FILE* f, T* p, size_t n) { return std::fread(p, sizeof(T), n, f) == n; }
+}  // namespace
+extern "C" {
+
+int amss_level_write_bin(amss_level* L, const char* path, int patch, int list, int field, double time) {
+  if (!L || !path) return fail(AMSS_ERR_ARG, "amss_level_write_bin: null argument");
+  double* d = L->L->field(patch, list, field);
+  if (!d) return fail(AMSS_ERR_ARG, "amss_level_write_bin: no such grid function");
+  const Patch& P = L->L->patch(patch);
+  const size_t n = (size_t)L->L->npts(patch);
+  std::vector<double> h(n);
+  HIPCHK(hipStreamSynchronize(L->L->stream()));
+  HIPCHK(hipMemcpy(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost));
+  File F;
+  F.f = std::fopen(path, "wb");
+  if (!F.f) return fail(AMSS_ERR_ARG, std::string("amss_level_write_bin: cannot open ") + path);
+  // Parallel.C:1059-1069: time, nx ny nz, xmin xmax ymin ymax zmin zmax, data
+  const double bb[6] = {P.bbox[0], P.bbox[3], P.bbox[1], P.bbox[4], P.bbox[2], P.bbox[5]};
+  if (!put(F.f, &time, 1) || !put(F.f, P.shape, 3) || !put(F.f, bb, 6) || !put(F.f, h.data(), n))
+    return fail(AMSS_ERR_ARG, std::string("amss_level_write_bin: short write to ") + path);
+  return 0;
+}
+
+int amss_level_checkpoint_write(amss_level* L, const char* path, double time, int movls, int nbh,
+                                const double* porg, const double* handles) {
+  if (!L || !path || nbh < 0 || (nbh > 0 && !porg)) return fail(AMSS_ERR_ARG, "amss_level_checkpoint_write: bad arguments");
+  const int np = L->L->npatch();
+  File F;
+  F.f = std::fopen(path, "wb");
+  if (!F.f) return fail(AMSS_ERR_ARG, std::string("amss_level_checkpoint_write: cannot open ") + path);
+  const int levels = 1;
+  // header (checkpoint.C:150-168): one level
+  bool ok = put(F.f, &time, 1) && put(F.f, &levels, 1) && put(F.f, &movls, 1) && put(F.f, &nbh, 1) &&
+            put(F.f, &np, 1) && put(F.f, &time, 1);
+  for (int p = 0; ok && p < np; ++p) {
+    const Patch& P = L->L->patch(p);
+    const double zero[3] = {0.0, 0.0, 0.0};
+    ok = put(F.f, P.bbox, 6) && put(F.f, P.shape, 3) && put(F.f, handles ? handles + 3 * p : zero, 3);
+  }
+  for (int b = 0; ok && b < nbh; ++b) ok = put(F.f, porg + 3 * b, 3);
+  // data (checkpoint.C:171-186): per patch, StateList then OldStateList
+  HIPCHK(hipStreamSynchronize(L->L->stream()));
+  std::vector<double> h;
+  for (int p = 0; ok && p < np; ++p) {
+    const size_t n = (size_t)L->L->npts(p);
+    h.resize(n);
+    for (int list : {AMSS_LIST_STATE, AMSS_LIST_OLD})
+      for (int v = 0; ok && v < 24; ++v) {
+        HIPCHK(hipMemcpy(h.data(), L->L->field(p, list, v), n * sizeof(double), hipMemcpyDeviceToHost));
+        ok = put(F.f, h.data(), n);
+      }
+  }
+  if (!ok) return fail(AMSS_ERR_ARG, std::string("amss_level_checkpoint_write: short write to ") + path);
+  return 0;
+}
+
+int amss_level_checkpoint_read(amss_level* L, const char* path, double* time, int* movls, int* nbh, double* porg,
+                               int maxbh) {
+  if (!L || !path || !time || !movls || !nbh || maxbh < 0 || (maxbh > 0 && !porg))
+    return fail(AMSS_ERR_ARG, "amss_level_checkpoint_read: bad arguments");
+  File F;
+  F.f = std::fopen(path, "rb");
+  if (!F.f) return fail(AMSS_ERR_ARG, std::string("amss_level_checkpoint_read: cannot open ") + path);
+  const std::string bad = std::string("amss_level_checkpoint_read: ") + path + ": ";
+  int levels = 0, grids = 0;
+  double lt = 0.0;
+  if (!get(F.f, time, 1) || !get(F.f, &levels, 1) || !get(F.f, movls, 1) || !get(F.f, nbh, 1))
+    return fail(AMSS_ERR_ARG, bad + "short header");
+  if (levels != 1) return fail(AMSS_ERR_ARG, bad + "not a one-level checkpoint");
+  if (*nbh < 0 || *nbh > maxbh) return fail(AMSS_ERR_ARG, bad + "more punctures than maxbh");
+  const int np = L->L->npatch();
+  if (!get(F.f, &grids, 1) || !get(F.f, &lt, 1)) return fail(AMSS_ERR_ARG, bad + "short header");
+  if (grids != np) return fail(AMSS_ERR_ARG, bad + "patch count differs from the level's");
+  for (int p = 0; p < np; ++p) {
+    double bb[6], hd[3];
+    int sh[3];
+    if (!get(F.f, bb, 6) || !get(F.f, sh, 3) || !get(F.f, hd, 3)) return fail(AMSS_ERR_ARG, bad + "short patch table");
+    const Patch& P = L->L->patch(p);
+    for (int a = 0; a < 3; ++a)
+      if (sh[a] != P.shape[a]) return fail(AMSS_ERR_ARG, bad + "patch shape differs from the level's");
+    for (int a = 0; a < 6; ++a)
+      if (std::fabs(bb[a] - P.bbox[a]) > 1e-12 * std::max(1.0, std::fabs(P.bbox[a])))
+        return fail(AMSS_ERR_ARG, bad + "patch bbox differs from the level's");
+  }
+  for (int b = 0; b < *nbh; ++b)
+    if (!get(F.f, porg + 3 * b, 3)) return fail(AMSS_ERR_ARG, bad + "short puncture table");
+  HIPCHK(hipStreamSynchronize(L->L->stream()));
+  std::vector<double> h;
+  for (int p = 0; p < np; ++p) {
+    const size_t n = (size_t)L->L->npts(p);
+    h.resize(n);
+    for (int list : {AMSS_LIST_STATE, AMSS_LIST_OLD})
+      for (int v = 0; v < 24; ++v) {
+        if (!get(F.f, h.data(), n)) return fail(AMSS_ERR_ARG, bad + "short data");
+        HIPCHK(hipMemcpy(L->L->field(p, list, v), h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      }
+  }
   return 0;
 }
 

@@ -101,6 +101,7 @@ class BssnLevel {
   int set_tune(const BatchTune& t);
   const BatchTune& tune() const { return tune_; }
   int npatch() const { return (int)patches_.size(); }
+  const Patch& patch(int i) const { return patches_[i]; }
   long npts(int patch) const;
 
   // the reference's lists (bssn_class.C:412-535)
