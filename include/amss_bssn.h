@@ -108,6 +108,21 @@ void sommerfeld_routbam_(int *ex, double *X, double *Y, double *Z, double *xmin,
                          double *xmax, double *ymax, double *zmax, double *f_rhs, double *f0, double *velocity,
                          double *SoA, int *Symmetry);
 
+/* Replace f_prolong3 / f_restrict3 (prolongrestrict.h:26-37) with the
+ * Cell, ghost_width 3 routines of prolongrestrict_cell.f90 (prolong3
+ * :1915-2164, restrict3 :2330-2468): 6-point, 5th-order interpolation from a
+ * coarse array (bbox llbc..uubc, extc cells) to the fine cells of the region
+ * llbp..uubp of a 2:1 fine array (llbf..uubf, extf), and back onto the
+ * coarse cells of llbr..uubr.  SoA: the field's parity (mirrored halo below
+ * the first cell).  A region the reference rejects, or whose stencils would
+ * leave the source and its mirrored halo, writes nothing: prolong3_ returns
+ * 1 (restrict3_ sets amss_last_error; the reference stops there); grids that
+ * are not 2:1 or wei != 3: 2. */
+int prolong3_(int *wei, double *llbc, double *uubc, int *extc, double *func, double *llbf, double *uubf,
+              int *extf, double *funf, double *llbp, double *uubp, double *SoA, int *Symmetry);
+void restrict3_(int *wei, double *llbc, double *uubc, int *extc, double *func, double *llbf, double *uubf,
+                int *extf, double *funf, double *llbr, double *uubr, double *SoA, int *Symmetry);
+
 /* ---------------------------------------------------------------------------
  * 2. Device-resident API
  * ------------------------------------------------------------------------- */
@@ -124,6 +139,15 @@ int amss_rhs_bssn_dev(const int *ex, const double *h, const double *const *state
                       const double *const *matter, double *const *gam, double *const *ric, double *const *res,
                       int symmetry, const double *origin, double eps, int co, int gauge, int *flag,
                       void *stream);
+
+/* prolong3_ / restrict3_ on device arrays, ordered on stream (no
+ * synchronisation); same arguments and return codes. */
+int amss_prolong3_dev(const double *llbc, const double *uubc, const int *extc, const double *func,
+                      const double *llbf, const double *uubf, const int *extf, double *funf, const double *llbp,
+                      const double *uubp, const double *SoA, void *stream);
+int amss_restrict3_dev(const double *llbc, const double *uubc, const int *extc, double *func, const double *llbf,
+                       const double *uubf, const int *extf, const double *funf, const double *llbr,
+                       const double *uubr, const double *SoA, void *stream);
 
 /* enforce_ga on device arrays (12 pointers: dxx gxy gxz dyy gyz dzz Axx..Azz). */
 int amss_enforce_ga_dev(const int *ex, double *const *ga, void *stream);

@@ -28,6 +28,12 @@ SIGNATURES = {
     "l2normhelper_": ([_ip, _dp, _dp, _dp] + [_dp] * 6 + [_dp, _dp, _ip], None),
     "sommerfeld_rout_": ([_ip, _dp, _dp, _dp] + [_dp] * 6 + [_dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip], None),
     "sommerfeld_routbam_": ([_ip, _dp, _dp, _dp] + [_dp] * 6 + [_dp, _dp, _dp, _dp, _ip], None),
+    "prolong3_": ([_ip, _dp, _dp, _ip, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _ip], C.c_int),
+    "restrict3_": ([_ip, _dp, _dp, _ip, _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _ip], None),
+    "amss_prolong3_dev": ([_dp, _dp, _ip, C.c_void_p, _dp, _dp, _ip, C.c_void_p, _dp, _dp, _dp, C.c_void_p],
+                          C.c_int),
+    "amss_restrict3_dev": ([_dp, _dp, _ip, C.c_void_p, _dp, _dp, _ip, C.c_void_p, _dp, _dp, _dp, C.c_void_p],
+                           C.c_int),
     "amss_rhs_bssn_dev": ([_ip, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                            C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     "amss_enforce_ga_dev": ([_ip, C.c_void_p, C.c_void_p], C.c_int),

@@ -1,8 +1,8 @@
 """Calls of the library's Fortran-ABI replacements (compute_rhs_bssn_,
 rungekutta4_rout_, enforce_ga_, lowerboundset_, l2normhelper_,
-sommerfeld_rout_, sommerfeld_routbam_) on host numpy arrays, with the same
-Python signatures as the reference build's, so a caller switches by
-changing the library.
+sommerfeld_rout_, sommerfeld_routbam_, prolong3_, restrict3_) on host numpy
+arrays, with the same Python signatures as the reference build's, so a
+caller switches by changing the library.
 Arrays: float64, shape (nz, ny, nx), C-contiguous.
 """
 from __future__ import annotations
@@ -87,6 +87,24 @@ def sommerfeld_bam(f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
     sa = (C.c_double * 3)(*soa)
     lib().sommerfeld_routbam_(_ex(f0.shape), _p(X), _p(Y), _p(Z), *[_d(b) for b in bbox], _p(f_rhs), _p(f0),
                               _d(velocity), sa, _i(symmetry))
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def prolong3(func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
+    """prolong3_ into funf in place; returns the status (0, 1 bad region, 2 bad grids)."""
+    return lib().prolong3_(_i(3), _d3(llbc), _d3(uubc), _ex(func.shape), _p(func), _d3(llbf), _d3(uubf),
+                           _ex(funf.shape), _p(funf), _d3(llbp), _d3(uubp), _d3(soa), _i(symmetry))
+
+
+def restrict3(func, llbc, uubc, funf, llbf, uubf, llbr, uubr, soa, symmetry=0):
+    """restrict3_ into func in place (a bad region leaves func untouched and
+    sets the library's last error)."""
+    lib().restrict3_(_i(3), _d3(llbc), _d3(uubc), _ex(func.shape), _p(func), _d3(llbf), _d3(uubf),
+                     _ex(funf.shape), _p(funf), _d3(llbr), _d3(uubr), _d3(soa), _i(symmetry))
+    return 0
 
 
 def compute_rhs_dev(state, h, eps, co=1, symmetry=0, origin=(0.0, 0.0, 0.0), gauge=0, matter=None, stream=None):

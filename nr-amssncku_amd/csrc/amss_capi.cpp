@@ -15,6 +15,7 @@
 #include "../../include/amss_bssn.h"
 #include "amss_adjoint.h"
 #include "amss_host.h"
+#include "amss_xfer.h"
 
 namespace amss {
 
@@ -397,6 +398,68 @@ void sommerfeld_routbam_(int* ex, double* X, double* Y, double* Z, double* xmin,
                          double* SoA, int* Symmetry) {
   const double bb[6] = {*xmin, *ymin, *zmin, *xmax, *ymax, *zmax};
   sommer_bam_host(ex, X, Y, Z, bb, f_rhs, f0, *velocity, SoA, *Symmetry);
+}
+
+// ---- inter-level transfer (prolongrestrict_cell.f90) ---------------------------
+static int xfer_host(bool prolong, int wei, double* llbc, double* uubc, int* extc, double* func, double* llbf,
+                     double* uubf, int* extf, double* funf, double* ll, double* uu, double* SoA) {
+  const char* who = prolong ? "prolong3_" : "restrict3_";
+  if (wei != 3) return fail(AMSS_ERR_ARG, std::string(who) + ": only 3 dimensions");
+  XferPlan P;
+  if (int r = xfer_plan(llbc, uubc, extc, llbf, uubf, extf, ll, uu, prolong, P))
+    return fail(r == 2 ? AMSS_ERR_ARG : AMSS_ERR_NONFINITE,
+                std::string(who) + (r == 2 ? ": grids are not 2:1" : ": the region does not fit the arrays"));
+  const long nc = (long)extc[0] * extc[1] * extc[2], nf = (long)extf[0] * extf[1] * extf[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure((size_t)(nc + nf))) return r;
+  double *dc = g_shim.buf, *df = dc + nc;
+  HIPCHK(hipMemcpyAsync(dc, func, nc * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(df, funf, nf * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  if (prolong) launch_prolong3(dc, df, P, SoA, g_shim.s);
+  else launch_restrict3(df, dc, P, SoA, g_shim.s);
+  HIPCHK(hipGetLastError());
+  if (prolong) HIPCHK(hipMemcpyAsync(funf, df, nf * sizeof(double), hipMemcpyDeviceToHost, g_shim.s));
+  else HIPCHK(hipMemcpyAsync(func, dc, nc * sizeof(double), hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+int prolong3_(int* wei, double* llbc, double* uubc, int* extc, double* func, double* llbf, double* uubf, int* extf,
+              double* funf, double* llbp, double* uubp, double* SoA, int* Symmetry) {
+  (void)Symmetry;
+  return xfer_host(true, *wei, llbc, uubc, extc, func, llbf, uubf, extf, funf, llbp, uubp, SoA);
+}
+
+void restrict3_(int* wei, double* llbc, double* uubc, int* extc, double* func, double* llbf, double* uubf,
+                int* extf, double* funf, double* llbr, double* uubr, double* SoA, int* Symmetry) {
+  (void)Symmetry;
+  xfer_host(false, *wei, llbc, uubc, extc, func, llbf, uubf, extf, funf, llbr, uubr, SoA);
+}
+
+int amss_prolong3_dev(const double* llbc, const double* uubc, const int* extc, const double* func,
+                      const double* llbf, const double* uubf, const int* extf, double* funf, const double* llbp,
+                      const double* uubp, const double* SoA, void* stream) {
+  if (!llbc || !uubc || !extc || !func || !llbf || !uubf || !extf || !funf || !llbp || !uubp || !SoA)
+    return fail(AMSS_ERR_ARG, "amss_prolong3_dev: null argument");
+  XferPlan P;
+  if (int r = xfer_plan(llbc, uubc, extc, llbf, uubf, extf, llbp, uubp, true, P))
+    return fail(r == 2 ? AMSS_ERR_ARG : AMSS_ERR_NONFINITE, "amss_prolong3_dev: bad grids or region");
+  launch_prolong3(func, funf, P, SoA, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int amss_restrict3_dev(const double* llbc, const double* uubc, const int* extc, double* func, const double* llbf,
+                       const double* uubf, const int* extf, const double* funf, const double* llbr,
+                       const double* uubr, const double* SoA, void* stream) {
+  if (!llbc || !uubc || !extc || !func || !llbf || !uubf || !extf || !funf || !llbr || !uubr || !SoA)
+    return fail(AMSS_ERR_ARG, "amss_restrict3_dev: null argument");
+  XferPlan P;
+  if (int r = xfer_plan(llbc, uubc, extc, llbf, uubf, extf, llbr, uubr, false, P))
+    return fail(r == 2 ? AMSS_ERR_ARG : AMSS_ERR_NONFINITE, "amss_restrict3_dev: bad grids or region");
+  launch_restrict3(funf, func, P, SoA, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 // ---- device-resident API ----------------------------------------------------

@@ -1002,6 +1002,148 @@ void orc_sommerfeld_routbam(const int *ex, const double *X, const double *Y, con
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Inter-level transfer of cell-centred grids, 2:1 (prolongrestrict_cell.f90,
+ * the ghost_width 3 branch: prolong3 :1915-2164, restrict3 :2330-2468).
+ * Index bookkeeping in the reference's 1-based cell numbering from a common
+ * base point; the Fortran literal 0.4 is single precision. */
+#define F04 ((double)0.4f)
+
+typedef struct {
+    int lo[3], hi[3]; /* target loop bounds (1-based array indices) */
+    int lbf[3], lbc[3];
+} xfer_t;
+
+/* base, lbf, lbc and the target/source ranges of a transfer (:1948-1990,
+ * 2361-2405).  ll/uu: bounds of the wanted region; tgt_fine: the target is
+ * the fine array (prolong).  Returns 0, or 1 where the reference reports an
+ * error (prolong3 prints and returns, restrict3 stops). */
+static int xfer_setup(const double *llbc, const double *uubc, const int *extc, const double *llbf,
+                      const double *uubf, const int *extf, const double *ll, const double *uu, int tgt_fine,
+                      xfer_t *x)
+{
+    double CD[3], FD[3], base[3];
+    for (int a = 0; a < 3; a++) {
+        CD[a] = (uubc[a] - llbc[a]) / extc[a];
+        FD[a] = (uubf[a] - llbf[a]) / extf[a];
+        if (fabs(CD[a] - 2 * FD[a]) > 1.e-10) return 1;
+    }
+    for (int a = 0; a < 3; a++) {
+        if (llbc[a] <= llbf[a]) {
+            base[a] = llbc[a];
+        } else {
+            const int j = (int)((llbc[a] - llbf[a]) / FD[a] + F04);
+            base[a] = (j / 2 * 2 == j) ? llbf[a] : llbf[a] - CD[a] / 2;
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        x->lbf[a] = (int)((llbf[a] - base[a]) / FD[a] + F04) + 1;
+        x->lbc[a] = (int)((llbc[a] - base[a]) / CD[a] + F04) + 1;
+        const int lt = (int)((ll[a] - base[a]) / (tgt_fine ? FD[a] : CD[a]) + F04) + 1;
+        const int ut = (int)((uu[a] - base[a]) / (tgt_fine ? FD[a] : CD[a]) + F04);
+        const int ls = (int)((ll[a] - base[a]) / (tgt_fine ? CD[a] : FD[a]) + F04) + 1;
+        const int us = (int)((uu[a] - base[a]) / (tgt_fine ? CD[a] : FD[a]) + F04);
+        const int tl = tgt_fine ? x->lbf[a] : x->lbc[a], sl = tgt_fine ? x->lbc[a] : x->lbf[a];
+        const int next = tgt_fine ? extf[a] : extc[a], nsrc = tgt_fine ? extc[a] : extf[a];
+        x->lo[a] = lt - tl + 1;
+        x->hi[a] = ut - tl + 1;
+        const int smin = ls - sl + 1, smax = us - sl + 1;
+        if (x->lo[a] < 1 || smin < 1 || x->hi[a] > next || smax > nsrc - 2) return 1;
+    }
+    return 0;
+}
+
+/* the source value at 1-based index (i, j, k), with symmetry_bd's mirror
+ * below index 1 (fmisc.f90:877-901) */
+static inline double xf_at(const double *f, const int *ex, int i, int j, int k, const double *S)
+{
+    return fh(f, ex, i - 1, j - 1, k - 1, S, 0.0);
+}
+
+int orc_prolong3(int wei, const double *llbc, const double *uubc, const int *extc, const double *func,
+                 const double *llbf, const double *uubf, const int *extf, double *funf, const double *llbp,
+                 const double *uubp, const double *SoA, int symmetry)
+{
+    (void)symmetry;
+    if (wei != 3) return 2;
+    xfer_t x;
+    if (xfer_setup(llbc, uubc, extc, llbf, uubf, extf, llbp, uubp, 1, &x)) return 1;
+    /* the stencils must stay inside funcc(-2:extc); the reference only prints
+     * "error in prolong" (:2038) and reads on */
+    for (int a = 0; a < 3; a++) {
+        const int c0 = (x.lo[a] + x.lbf[a] - 1) / 2 - x.lbc[a] + 1, c1 = (x.hi[a] + x.lbf[a] - 1) / 2 - x.lbc[a] + 1;
+        if (c0 < 0 || c1 + 3 > extc[a]) return 1;
+    }
+    static const double C1 = 7.7e1 / 8.192e3, C2 = -6.93e2 / 8.192e3, C3 = 3.465e3 / 4.096e3;
+    static const double C6 = 6.3e1 / 8.192e3, C5 = -4.95e2 / 8.192e3, C4 = 1.155e3 / 4.096e3;
+    const double Wf[6] = {C1, C2, C3, C4, C5, C6}, Wr[6] = {C6, C5, C4, C3, C2, C1};
+    for (int k = x.lo[2]; k <= x.hi[2]; k++)
+        for (int j = x.lo[1]; j <= x.hi[1]; j++)
+            for (int i = x.lo[0]; i <= x.hi[0]; i++) {
+                const int t[3] = {i, j, k};
+                int c[3];
+                const double *w[3];
+                for (int a = 0; a < 3; a++) {
+                    c[a] = (t[a] + x.lbf[a] - 1) / 2 - x.lbc[a] + 1; /* :2033-2036 */
+                    const int g = t[a] + x.lbf[a] - 1;
+                    w[a] = (g / 2 * 2 == g) ? Wf : Wr;
+                }
+                double tmp2[6][6], tmp1[6];
+                for (int b = 0; b < 6; b++)
+                    for (int a = 0; a < 6; a++) {
+                        double s = w[2][0] * xf_at(func, extc, c[0] - 2 + a, c[1] - 2 + b, c[2] - 2, SoA);
+                        for (int q = 1; q < 6; q++)
+                            s = s + w[2][q] * xf_at(func, extc, c[0] - 2 + a, c[1] - 2 + b, c[2] - 2 + q, SoA);
+                        tmp2[a][b] = s;
+                    }
+                for (int a = 0; a < 6; a++) {
+                    double s = w[1][0] * tmp2[a][0];
+                    for (int b = 1; b < 6; b++) s = s + w[1][b] * tmp2[a][b];
+                    tmp1[a] = s;
+                }
+                double s = w[0][0] * tmp1[0];
+                for (int a = 1; a < 6; a++) s = s + w[0][a] * tmp1[a];
+                funf[IX(extf, i - 1, j - 1, k - 1)] = s;
+            }
+    return 0;
+}
+
+int orc_restrict3(int wei, const double *llbc, const double *uubc, const int *extc, double *func,
+                  const double *llbf, const double *uubf, const int *extf, const double *funf,
+                  const double *llbr, const double *uubr, const double *SoA, int symmetry)
+{
+    (void)symmetry;
+    if (wei != 3) return 2;
+    xfer_t x;
+    if (xfer_setup(llbc, uubc, extc, llbf, uubf, extf, llbr, uubr, 0, &x)) return 1;
+    /* the stencils must stay inside funff(-1:extf) (:2455) */
+    for (int a = 0; a < 3; a++) {
+        const int c0 = 2 * (x.lo[a] + x.lbc[a] - 1) - x.lbf[a], c1 = 2 * (x.hi[a] + x.lbc[a] - 1) - x.lbf[a];
+        if (c0 < 1 || c1 + 3 > extf[a]) return 1;
+    }
+    static const double C1 = 3.e0 / 2.56e2, C2 = -2.5e1 / 2.56e2, C3 = 7.5e1 / 1.28e2;
+    for (int k = x.lo[2]; k <= x.hi[2]; k++)
+        for (int j = x.lo[1]; j <= x.hi[1]; j++)
+            for (int i = x.lo[0]; i <= x.hi[0]; i++) {
+                const int t[3] = {i, j, k};
+                int c[3];
+                for (int a = 0; a < 3; a++) c[a] = 2 * (t[a] + x.lbc[a] - 1) - 1 - x.lbf[a] + 1; /* :2450-2453 */
+                double tmp2[6][6], tmp1[6];
+#define G(a, b, q) xf_at(funf, extf, c[0] - 2 + (a), c[1] - 2 + (b), c[2] - 2 + (q), SoA)
+                for (int b = 0; b < 6; b++)
+                    for (int a = 0; a < 6; a++)
+                        tmp2[a][b] = C1 * (G(a, b, 0) + G(a, b, 5)) + C2 * (G(a, b, 1) + G(a, b, 4)) +
+                                     C3 * (G(a, b, 2) + G(a, b, 3));
+#undef G
+                for (int a = 0; a < 6; a++)
+                    tmp1[a] = C1 * (tmp2[a][0] + tmp2[a][5]) + C2 * (tmp2[a][1] + tmp2[a][4]) +
+                              C3 * (tmp2[a][2] + tmp2[a][3]);
+                func[IX(extc, i - 1, j - 1, k - 1)] =
+                    C1 * (tmp1[0] + tmp1[5]) + C2 * (tmp1[1] + tmp1[4]) + C3 * (tmp1[2] + tmp1[3]);
+            }
+    return 0;
+}
+
 /* Propagation speed of each field in the Sommerfeld condition: sqrt(2) for
  * chi, trK and the lapse (the 1+log gauge speed), 1 for the rest
  * (bssn_class.C:390-410, var.C:30). */

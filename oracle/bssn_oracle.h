@@ -68,6 +68,17 @@ void orc_sommerfeld_routbam(const int *ex, const double *X, const double *Y, con
                             const double *bbox, double *f_rhs, const double *f0, double velocity,
                             const double *SoA, int symmetry);
 
+/* prolongrestrict_cell.f90 (Cell, ghost_width 3): 6-point 5th-order
+ * prolongation onto the fine region [llbp, uubp] and restriction onto the
+ * coarse region [llbr, uubr].  Return 0, 1 where the reference reports a bad
+ * region (and writes nothing), 2 for wei != 3. */
+int orc_prolong3(int wei, const double *llbc, const double *uubc, const int *extc, const double *func,
+                 const double *llbf, const double *uubf, const int *extf, double *funf, const double *llbp,
+                 const double *uubp, const double *SoA, int symmetry);
+int orc_restrict3(int wei, const double *llbc, const double *uubc, const int *extc, double *func,
+                  const double *llbf, const double *uubf, const int *extf, const double *funf,
+                  const double *llbr, const double *uubr, const double *SoA, int symmetry);
+
 /* One RK4 step of one Block in the order of bssn_class::Step
  * (bssn_class.C:2981-3901) for a single rank and a single patch.
  * U: 24 arrays, overwritten with the new state.  bc: 0 = copy the U0 value

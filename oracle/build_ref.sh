@@ -29,7 +29,7 @@ trap 'rm -rf "$TMP"' EXIT
 # fmisc.f90 minus fwritefile; the include of macrodef.fh is resolved with -I.
 awk '/^ *subroutine fwritefile\(/{skip=1} !skip{print} /^ *end subroutine fwritefile/{skip=0}' \
     "$REF/fmisc.f90" > "$TMP/fmisc_nofwrite.f90"
-SRCS="bssn_rhs diff_new lopsidediff kodiss enforce_algebra rungekutta4_rout sommerfeld_rout"
+SRCS="bssn_rhs diff_new lopsidediff kodiss enforce_algebra rungekutta4_rout sommerfeld_rout prolongrestrict_cell"
 build() {  # $1 = suffix, $2.. = flags
   local sfx=$1; shift
   local objs=()

@@ -140,6 +140,20 @@ class Orc:
         self.lib.orc_sommerfeld_copy(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, _ptr(f0), _ptr(f),
                                      C.c_int(symmetry))
 
+    def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
+        """prolong3 into funf in place; returns the status (0 ok, 1 bad region)."""
+        d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
+        return self.lib.orc_prolong3(C.c_int(3), d3(llbc), d3(uubc), _ex(func.shape), _ptr(func), d3(llbf),
+                                     d3(uubf), _ex(funf.shape), _ptr(funf), d3(llbp), d3(uubp), d3(soa),
+                                     C.c_int(symmetry))
+
+    def restrict3(self, func, llbc, uubc, funf, llbf, uubf, llbr, uubr, soa, symmetry=0):
+        """restrict3 into func in place; returns the status (0 ok, 1 bad region)."""
+        d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
+        return self.lib.orc_restrict3(C.c_int(3), d3(llbc), d3(uubc), _ex(func.shape), _ptr(func), d3(llbf),
+                                      d3(uubf), _ex(funf.shape), _ptr(funf), d3(llbr), d3(uubr), d3(soa),
+                                      C.c_int(symmetry))
+
     def sommerfeld_bam(self, f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
         """sommerfeld_routbam on f_rhs in place."""
         bb = (C.c_double * 6)(*bbox)
@@ -251,6 +265,20 @@ class Ref:
         self.lib.sommerfeld_rout_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), *[self._d(b) for b in bbox],
                                   self._d(0.0), _ptr(dummy), _ptr(dummy), _ptr(f0), _ptr(f), soa,
                                   self._i(symmetry), self._i(1))
+
+    def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
+        """prolong3_ (a subroutine: no status) into funf in place."""
+        d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
+        self.lib.prolong3_(self._i(3), d3(llbc), d3(uubc), _ex(func.shape), _ptr(func), d3(llbf), d3(uubf),
+                           _ex(funf.shape), _ptr(funf), d3(llbp), d3(uubp), d3(soa), self._i(symmetry))
+        return 0
+
+    def restrict3(self, func, llbc, uubc, funf, llbf, uubf, llbr, uubr, soa, symmetry=0):
+        """restrict3_ into func in place."""
+        d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
+        self.lib.restrict3_(self._i(3), d3(llbc), d3(uubc), _ex(func.shape), _ptr(func), d3(llbf), d3(uubf),
+                            _ex(funf.shape), _ptr(funf), d3(llbr), d3(uubr), d3(soa), self._i(symmetry))
+        return 0
 
     def sommerfeld_bam(self, f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
         """sommerfeld_routbam_ on f_rhs in place."""

@@ -30,3 +30,32 @@ def rel_err(a, b):
     m = float(np.max(np.abs(b)))
     d = float(np.max(np.abs(a - b)))
     return d / m if m > 0 else d
+
+
+def transfer_cases(seed, n):
+    """Random 2:1 cell-centred coarse/fine pairs and wanted regions for
+    prolong3 / restrict3: fine boxes offset from the coarse by even and odd
+    numbers of fine cells, on both sides of the coarse lower bound, regions
+    touching the arrays' lower edges (mirrored halo reads).  Yields dicts."""
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        CD = float(rng.choice([0.4, 0.5, 0.25]))
+        FD = CD / 2
+        extc = rng.integers(10, 17, 3)
+        llbc = np.array([-CD * extc[a] / 2 + CD * rng.integers(-1, 2) for a in range(3)])
+        extf = rng.integers(8, 15, 3)
+        llbf = llbc + FD * rng.integers(-2, 12, 3)
+        func = rng.standard_normal(tuple(extc[::-1]))
+        funf = rng.standard_normal(tuple(extf[::-1]))
+        i0 = rng.integers(0, extf - 1)
+        i1 = np.minimum(i0 + rng.integers(1, 8, 3), extf)
+        soa = tuple(rng.choice([-1.0, 1.0], 3))
+        c = dict(llbc=llbc, uubc=llbc + CD * extc, llbf=llbf, uubf=llbf + FD * extf, func=func, funf=funf,
+                 llbp=llbf + FD * i0, uubp=llbf + FD * i1, soa=soa, llbr=None, uubr=None)
+        lo = np.ceil((llbf + 2 * FD - llbc) / CD - 1e-9).astype(int)
+        hi = np.floor((c["uubf"] - 2 * FD - llbc) / CD + 1e-9).astype(int)
+        if np.all(hi > lo):
+            j0 = np.array([rng.integers(a, b) for a, b in zip(lo, hi)])
+            j1 = np.array([rng.integers(a + 1, b + 1) for a, b in zip(j0, hi)])
+            c["llbr"], c["uubr"] = llbc + CD * j0, llbc + CD * j1
+        yield c

@@ -76,6 +76,24 @@ def test_level_symmetry_needs_a_reference_boundary_mode():
         A.Level([p], bc=A.BC_NONE, symmetry=3)
 
 
+def test_transfer_arguments_checked_before_any_gpu_work():
+    """prolong3_ / restrict3_: grids that are not 2:1 return 2, a region the
+    reference rejects returns 1, both before the library touches a GPU."""
+    import numpy as np
+    import amssbssn
+    from amssbssn import fortran_abi as fa
+    func, funf = np.zeros((12, 12, 12)), np.ones((10, 10, 10))
+    llbc, uubc = (-2.4,) * 3, (2.4,) * 3           # h = 0.4
+    llbf, uubf = (-1.0,) * 3, (1.5,) * 3           # h = 0.25: not 2:1
+    assert fa.prolong3(func, llbc, uubc, funf, llbf, uubf, llbf, uubf, (1, 1, 1)) == 2
+    assert "2:1" in amssbssn.last_error()
+    llbf, uubf = (-1.0,) * 3, (1.0,) * 3           # h = 0.2
+    assert fa.prolong3(func, llbc, uubc, funf, llbf, uubf, (-3.0,) * 3, uubf, (1, 1, 1)) == 1
+    fa.restrict3(func, llbc, uubc, funf, llbf, uubf, llbc, uubc, (1, 1, 1))
+    assert "does not fit" in amssbssn.last_error()
+    assert np.all(func == 0) and np.all(funf == 1)
+
+
 def test_config_geometry():
     import amssbssn as A
     c0 = A.config0()

@@ -192,6 +192,41 @@ def test_sommerfeld_bam_exact_on_quadratics(orc, sym):
     assert np.all(out[~layer] == 7.0)
 
 
+def _poly5(x, y, z):
+    return (0.3 + x - 0.7 * x ** 5 + 0.2 * x ** 3 * y ** 2 + y ** 4 - 0.5 * y ** 5 * z + 1.1 * z ** 5
+            - x * y * z + 0.05 * (x * y * z) ** 2)
+
+
+def test_prolong3_restrict3_exact_on_degree5(orc):
+    """6-point Lagrange interpolation (prolong: fine cell centres at 1/4 and
+    3/4 of a coarse cell; restrict: coarse centres midway between fine ones)
+    is exact for polynomials of degree <= 5 in each variable; regions kept
+    off the arrays' lower edges (no mirrored reads)."""
+    CD, FD = 0.5, 0.25
+    extc, extf = np.array([14, 12, 13]), np.array([12, 14, 10])
+    llbc = np.array([-3.5, -3.0, -3.25])
+    llbf = llbc + FD * np.array([6, 7, 8])  # even and odd offsets
+    cen = lambda lo, n, d: [lo[a] + (np.arange(n[a]) + 0.5) * d for a in range(3)]  # noqa: E731
+    Xc, Xf = cen(llbc, extc, CD), cen(llbf, extf, FD)
+    zc, yc, xc = np.meshgrid(Xc[2], Xc[1], Xc[0], indexing="ij")
+    zf, yf, xf = np.meshgrid(Xf[2], Xf[1], Xf[0], indexing="ij")
+    func, funf = _poly5(xc, yc, zc), _poly5(xf, yf, zf)
+    # prolong onto the fine interior
+    out = np.full(funf.shape, 9.0)
+    assert orc.prolong3(func, llbc, llbc + CD * extc, out, llbf, llbf + FD * extf, llbf + FD * 2,
+                        llbf + FD * (extf - 2), (1, 1, 1)) == 0
+    s = np.s_[2:-2, 2:-2, 2:-2]
+    assert np.max(np.abs(out[s] - funf[s])) < 1e-11 * np.max(np.abs(funf[s]))
+    assert np.all(out[0] == 9.0)
+    # restrict onto coarse cells covered by the fine data with a 3-cell margin
+    out = np.full(func.shape, 9.0)
+    lo, hi = llbf + 3 * FD + np.array([0.0, 0.25, 0.0]), llbf + FD * (extf - 3) - np.array([0.0, 0.25, 0.0])
+    assert orc.restrict3(out, llbc, llbc + CD * extc, funf, llbf, llbf + FD * extf, lo, hi, (1, 1, 1)) == 0
+    m = out != 9.0
+    assert m.sum() >= 12
+    assert np.max(np.abs(out[m] - func[m])) < 1e-11 * np.max(np.abs(func))
+
+
 def test_philox_known_answers(orc):
     import amssbssn.grid as G
     kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),

@@ -133,6 +133,32 @@ def test_step_with_symmetry_planes_bitwise(orc, ref, sym, bc):
         assert np.array_equal(a, b)
 
 
+def test_prolong3_restrict3_bitwise(orc, ref):
+    """prolongrestrict_cell.f90 (Cell, ghost width 3) on random 2:1 pairs and
+    regions; the reference is called only on regions the oracle accepts
+    (restrict3 stops the process on a bad one)."""
+    from helpers import transfer_cases
+    np_, nr = 0, 0
+    for c in transfer_cases(5, 150):
+        a = c["funf"].copy()
+        if orc.prolong3(c["func"], c["llbc"], c["uubc"], a, c["llbf"], c["uubf"], c["llbp"], c["uubp"],
+                        c["soa"]) == 0:
+            b = c["funf"].copy()
+            ref.prolong3(c["func"], c["llbc"], c["uubc"], b, c["llbf"], c["uubf"], c["llbp"], c["uubp"], c["soa"])
+            assert np.array_equal(a, b) and not np.array_equal(a, c["funf"])
+            np_ += 1
+        if c["llbr"] is not None:
+            a = c["func"].copy()
+            if orc.restrict3(a, c["llbc"], c["uubc"], c["funf"], c["llbf"], c["uubf"], c["llbr"], c["uubr"],
+                             c["soa"]) == 0:
+                b = c["func"].copy()
+                ref.restrict3(b, c["llbc"], c["uubc"], c["funf"], c["llbf"], c["uubf"], c["llbr"], c["uubr"],
+                              c["soa"])
+                assert np.array_equal(a, b) and not np.array_equal(a, c["func"])
+                nr += 1
+    assert np_ > 80 and nr > 80, (np_, nr)
+
+
 def _bam_cases():
     """(X, Y, Z, bbox, symmetry): a box whose faces all lie on bbox and
     straddle the origin, boxes starting at the symmetry planes, and a box
