@@ -96,6 +96,63 @@ def batching(steps, warmup):
     return out
 
 
+def transfer_leg(reps=20):
+    """Inter-level transfer (prolong3 / restrict3, 6-point 5th order, 2:1
+    cell-centred): a 76^3 coarse array (h 0.5 over [-19,19]^3) prolonged
+    onto the whole of a 134^3 fine array (h 0.25 over [-16.75,16.75]^3), and
+    the fine data restricted back onto the 62^3 coarse cells of
+    [-15.5,15.5]^3.  Device arrays,
+    kernels timed with events on the launch stream.  Algorithmic bytes: 8 B
+    written + 1 B read (an eighth of a coarse value) per fine cell; 8 B
+    written + 64 B read per restricted coarse cell."""
+    import ctypes as C
+    import torch
+    import amssbssn as A
+    L = A.lib()
+    dev = torch.device("cuda:0")
+    CD, FD = 0.5, 0.25
+    extc, extf = (76, 76, 76), (134, 134, 134)
+    llbc = [-CD * 38] * 3
+    llbf = [-FD * 67] * 3
+    uubc = [a + CD * n for a, n in zip(llbc, extc)]
+    uubf = [a + FD * n for a, n in zip(llbf, extf)]
+    d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
+    i3 = lambda v: (C.c_int * 3)(*v)  # noqa: E731
+    g = torch.Generator(device="cpu").manual_seed(3)
+    func = torch.randn(extc[::-1], generator=g, dtype=torch.float64).to(dev)
+    funf = torch.zeros(extf[::-1], dtype=torch.float64, device=dev)
+    s = torch.cuda.current_stream(dev)
+    llbr, uubr = [-15.5] * 3, [15.5] * 3
+
+    def prolong():
+        return L.amss_prolong3_dev(d3(llbc), d3(uubc), i3(extc), C.c_void_p(func.data_ptr()), d3(llbf),
+                                   d3(uubf), i3(extf), C.c_void_p(funf.data_ptr()), d3(llbf), d3(uubf),
+                                   d3((1.0, 1.0, 1.0)), C.c_void_p(s.cuda_stream))
+
+    def restrict():
+        return L.amss_restrict3_dev(d3(llbc), d3(uubc), i3(extc), C.c_void_p(func.data_ptr()), d3(llbf),
+                                    d3(uubf), i3(extf), C.c_void_p(funf.data_ptr()), d3(llbr), d3(uubr),
+                                    d3((1.0, 1.0, 1.0)), C.c_void_p(s.cuda_stream))
+
+    out = dict(workload="76^3 coarse (h 0.5) -> whole 134^3 fine (h 0.25), then restrict onto 62^3 coarse cells")
+    nf = extf[0] * extf[1] * extf[2]
+    nr = int(round((uubr[0] - llbr[0]) / CD)) ** 3
+    for name, fn, pts, bpp in (("prolong", prolong, nf, 9.0), ("restrict", restrict, nr, 72.0)):
+        assert fn() == 0
+        s.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = dict(points=pts, ms=round(ms, 4), mpoints_s=round(pts / ms / 1e3, 1),
+                         achieved_gbs=round(pts * bpp / ms / 1e6, 1),
+                         frac_hbm=round(pts * bpp / ms / 1e6 / 8000.0, 4), bytes_per_point=bpp)
+    return out
+
+
 def adjoint_leg(steps, checkpoint=20):
     """The 64^3 differentiation workload (config-1 data over [-16,16]^3,
     parameters: masses, x positions, 16^3 spline coefficients on A_xx): J and
@@ -152,6 +209,8 @@ def main():
                     help="steps of the 64^3 gradient run (0 skips the adjoint/forward measurement)")
     ap.add_argument("--batched-adjoint-steps", type=int, default=20,
                     help="steps of the 16-patch batched gradient run (0 skips it; the full workload is 400)")
+    ap.add_argument("--transfer-reps", type=int, default=20,
+                    help="timed repetitions of the inter-level transfer leg (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -226,6 +285,8 @@ def main():
     lv.close()
     if args.batch_steps > 0:
         out["batching"] = batching(args.batch_steps, 2)
+    if args.transfer_reps > 0:
+        out["transfer"] = transfer_leg(args.transfer_reps)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if args.batched_adjoint_steps > 0:

@@ -110,6 +110,7 @@ struct EdgeEnv {
   __device__ EdgeEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, int i_, int j_, int k_,
                      const int* n_, const int* lo_, long sy_, long sz_)
       : F(F_), M(M_), C(C_), i(i_), j(j_), k(k_), sy(sy_), sz(sz_) {
+    #pragma unroll
     for (int a = 0; a < 3; ++a) {
       n[a] = n_[a];
       lo[a] = lo_[a];
@@ -141,6 +142,7 @@ struct EdgeEnv {
     return axis == 0 ? rd(v, i + m, j, k, off) : (axis == 1 ? rd(v, i, j + m, k, off) : rd(v, i, j, k + m, off));
   }
   __device__ void grad(int v, double d[3]) const {
+    #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (ord == 2)
         d[a] = C.d1[a] * ((ax(v, a, -2) - ax(v, a, 2)) + 8.0 * (ax(v, a, 1) - ax(v, a, -1)));
@@ -151,19 +153,24 @@ struct EdgeEnv {
     }
   }
   __device__ void hess(int v, double d[6]) const {
+    #pragma unroll
     for (int s = 0; s < 6; ++s) d[s] = 0.0;
     if (ord == 0) return;
     const double c = at3(v, 0, 0, 0);
     if (ord == 2) {
+      #pragma unroll
       for (int a = 0; a < 3; ++a)
         d[sidx(a, a)] = C.d2[a] * (16.0 * (ax(v, a, -1) + ax(v, a, 1)) - (ax(v, a, -2) + ax(v, a, 2)) - 30.0 * c);
       const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+      #pragma unroll
       for (int q = 0; q < 3; ++q) {
         double acc = 0.0;
         const double w[5] = {1.0, -8.0, 0.0, 8.0, -1.0};
+        #pragma unroll
         for (int bb = -2; bb <= 2; ++bb) {
           if (bb == 0) continue;
           double r = 0.0;
+          #pragma unroll
           for (int aa = -2; aa <= 2; ++aa) {
             if (aa == 0) continue;
             int o[3] = {0, 0, 0};
@@ -176,9 +183,11 @@ struct EdgeEnv {
         d[sidx(pa[q], pb[q])] = C.dm[q] * acc;
       }
     } else {
+      #pragma unroll
       for (int a = 0; a < 3; ++a)
         d[sidx(a, a)] = C.d2lo[a] * (ax(v, a, -1) + ax(v, a, 1) - 2.0 * c);
       const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+      #pragma unroll
       for (int q = 0; q < 3; ++q) {
         int o[3] = {0, 0, 0};
         auto f = [&](int a, int b) {
@@ -224,6 +233,7 @@ struct EdgeEnv {
     if (!kook) return 0.0;
     const double c20 = 20.0 * at3(v, 0, 0, 0);
     double r = 0.0;
+    #pragma unroll
     for (int a = 0; a < 3; ++a)
       r += C.ko[a] * ((ax(v, a, -3) + ax(v, a, 3)) - 6.0 * (ax(v, a, -2) + ax(v, a, 2)) +
                       15.0 * (ax(v, a, -1) + ax(v, a, 1)) - c20);
@@ -385,9 +395,9 @@ __device__ __forceinline__ double sommer_rhs(const SommerGeom& G, const int n[3]
 // field's RHS is replaced by the boundary condition's before the update, as
 // the reference applies sommerfeld_routbam to the stage's RHS before
 // rungekutta4_rout (bssn_class.C:3106-3117, 3476-3484).
-template <class Env>
+template <class Env, class Inner>
 struct SommerRk4Sink {
-  Rk4Sink S;
+  Inner S;
   const Env& E;
   const SommerGeom& G;
   bool on;

@@ -27,14 +27,51 @@ __device__ __forceinline__ bool interior_index(const Box& box, int& i, int& j, i
   k = box.lo[2] + blockIdx.z;
   return i < box.hi[0] && j < box.hi[1];
 }
+// The points outside the box [lo, hi) of an array, enumerated as six
+// disjoint slabs (z faces whole, then y faces over the middle z range, then
+// x faces over the middle y and z ranges), so that every lane of a wave has a
+// point: walking the whole array instead leaves an x-face row's three points
+// alone in a wave.  An empty box makes the shell the whole array.
+struct Shell {
+  int n[3], L[3], H[3];
+  long cnt[6];
+  __host__ __device__ explicit Shell(const Box& b) {
+    for (int a = 0; a < 3; ++a) {
+      n[a] = b.n[a];
+      L[a] = b.lo[a] < 0 ? 0 : (b.lo[a] > n[a] ? n[a] : b.lo[a]);
+      H[a] = b.hi[a] < L[a] ? L[a] : (b.hi[a] > n[a] ? n[a] : b.hi[a]);
+    }
+    const long mk = H[2] - L[2], mj = H[1] - L[1];
+    cnt[0] = (long)n[0] * n[1] * L[2];
+    cnt[1] = (long)n[0] * n[1] * (n[2] - H[2]);
+    cnt[2] = (long)n[0] * L[1] * mk;
+    cnt[3] = (long)n[0] * (n[1] - H[1]) * mk;
+    cnt[4] = (long)L[0] * mj * mk;
+    cnt[5] = (long)(n[0] - H[0]) * mj * mk;
+  }
+  __host__ __device__ long size() const { return cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4] + cnt[5]; }
+  // point t of the shell (t < size())
+  __device__ void point(long t, int& i, int& j, int& k) const {
+    int s = 0;
+    while (s < 5 && t >= cnt[s]) t -= cnt[s++];
+    // slab s spans [x0, x0 + ex) x [y0, y0 + ey) x [z0, ...)
+    const int ex = s < 4 ? n[0] : (s == 4 ? L[0] : n[0] - H[0]);
+    const int ey = s < 2 ? n[1] : (s == 2 ? L[1] : (s == 3 ? n[1] - H[1] : H[1] - L[1]));
+    const int x0 = s == 5 ? H[0] : 0;
+    const int y0 = s < 3 ? 0 : (s == 3 ? H[1] : L[1]);
+    const int z0 = s == 0 ? 0 : (s == 1 ? H[2] : L[2]);
+    i = x0 + (int)(t % ex);
+    j = y0 + (int)((t / ex) % ey);
+    k = z0 + (int)(t / ((long)ex * ey));
+  }
+};
+
 __device__ __forceinline__ bool edge_index(const Box& box, int& i, int& j, int& k) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long N = (long)box.n[0] * box.n[1] * box.n[2];
-  if (t >= N) return false;
-  i = t % box.n[0];
-  j = (t / box.n[0]) % box.n[1];
-  k = t / ((long)box.n[0] * box.n[1]);
-  return !(i >= box.lo[0] && i < box.hi[0] && j >= box.lo[1] && j < box.hi[1] && k >= box.lo[2] && k < box.hi[2]);
+  const Shell sh(box);
+  if (t >= sh.size()) return false;
+  sh.point(t, i, j, k);
+  return true;
 }
 
 // ---- plain RHS (the compute_rhs_bssn outputs) ------------------------------
@@ -110,9 +147,15 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
   if (X.bad) *flag = 1;
 }
 
-// sg: the Sommerfeld faces of the array (none when sg.faces == 0)
-template <int GAUGE, bool CO>
-__global__ void __launch_bounds__(256) rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
+// sg: the Sommerfeld faces of the array (none when sg.faces == 0).  The
+// sub-step is a template argument, as in the interior kernels (a runtime
+// stage costs registers and spills).
+#ifndef AMSS_EDGE_WAVES
+#define AMSS_EDGE_WAVES 2
+#endif
+template <int GAUGE, int STAGE, bool CO>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AMSS_EDGE_WAVES, AMSS_EDGE_WAVES)))
+rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
                                                        AuxOut aux, Box box, Coef C, Rk4Param R, int* flag,
                                                        SommerGeom sg) {
   int i, j, k;
@@ -120,7 +163,8 @@ __global__ void __launch_bounds__(256) rk4_edge_kernel(FieldsIn in, FieldsIn u0,
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
   EdgeEnv E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
-  SommerRk4Sink<EdgeEnv> S{{u0, acc, out, aux, p, R.stage, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
+  SommerRk4Sink<EdgeEnv, Rk4SinkT<STAGE>> S{
+      {u0, acc, out, aux, p, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
   bssn_rhs_point<GAUGE, CO, false, false>(E, S);
   S.finish();
 }
@@ -135,13 +179,13 @@ inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); 
 template <int G>
 void launch_rhs_gauge(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux,
                       const Box& box, const Coef& C, int co, int* flag, bool edge, hipStream_t s) {
-  const long N = (long)box.n[0] * box.n[1] * box.n[2];
+  edge = edge && Shell(box).size() > 0;
   if (co == 0) {
     if (!box_empty(box)) rhs_interior_kernel<G, true><<<interior_grid(box), kRhsThreads, 0, s>>>(in, mat, out, aux, box, C, flag);
-    if (edge) rhs_edge_kernel<G, true><<<blocks_for(N, 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
+    if (edge) rhs_edge_kernel<G, true><<<blocks_for(Shell(box).size(), 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
   } else {
     if (!box_empty(box)) rhs_interior_kernel<G, false><<<interior_grid(box), kRhsThreads, 0, s>>>(in, mat, out, aux, box, C, flag);
-    if (edge) rhs_edge_kernel<G, false><<<blocks_for(N, 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
+    if (edge) rhs_edge_kernel<G, false><<<blocks_for(Shell(box).size(), 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
   }
 }
 
@@ -190,12 +234,19 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
       default: rk4_interior_kernel<G, 3, false><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag); break;
     }
   }
-  if (edge) {
-    const long N = (long)box.n[0] * box.n[1] * box.n[2];
-    if (want_res && R.stage == 0)
-      rk4_edge_kernel<G, true><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag, sg);
-    else
-      rk4_edge_kernel<G, false><<<blocks_for(N, 256), 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag, sg);
+  if (edge && Shell(box).size() > 0) {
+    const unsigned gr = blocks_for(Shell(box).size(), 256);
+#define AMSS_EDGE(ST, CO) rk4_edge_kernel<G, ST, CO><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag, sg)
+    switch (R.stage) {
+      case 0:
+        if (want_res) AMSS_EDGE(0, true);
+        else AMSS_EDGE(0, false);
+        break;
+      case 1: AMSS_EDGE(1, false); break;
+      case 2: AMSS_EDGE(2, false); break;
+      default: AMSS_EDGE(3, false); break;
+    }
+#undef AMSS_EDGE
   }
 }
 
