@@ -1,9 +1,7 @@
 // amss_xfer.hip -- inter-level transfer between 2:1 cell-centred grids: the
 // 6-point, 5th-order prolongation (coarse -> fine) and restriction (fine ->
 // coarse) of prolongrestrict_cell.f90 (Cell, ghost_width 3; prolong3
-// :1915-2164, restrict3 :2330-2468).  One thread per target point; the
-// 6x6x6 source block is contracted along z, then y, then x, with the
-// reference's weights and summation order.
+// :1915-2164, restrict3 :2330-2468), on LDS-staged tiles of target cells.
 #include "amss_xfer.h"
 
 #include <cmath>
@@ -79,58 +77,131 @@ struct XferSrc {
   }
 };
 
+// Both transfers work on tiles of target cells: the source block a tile
+// needs is staged once in LDS and the 6x6x6 contraction is factorized (along
+// z for every (y, x) of the block, then along y, then along x), which is
+// the reference's summation order.  Cells past the end of the target range
+// in a partial tile are computed from unloaded LDS and not stored; every LDS
+// index stays inside the fixed block sizes (shown at each size below).
+constexpr double kProW[6] = {77.0 / 8192.0, -693.0 / 8192.0, 3465.0 / 4096.0,
+                             1155.0 / 4096.0, -495.0 / 8192.0, 63.0 / 8192.0};
+constexpr double kResW[3] = {3.0 / 256.0, -25.0 / 256.0, 75.0 / 128.0};
+
+// prolong: fine tile 16 x 8 x 8.  Fine cell p reads coarse cells
+// c(p) - 2 .. c(p) + 3, c(p) = (p + lbf - 1) / 2 - lbc + 1; over a tile c
+// advances by at most (F - 1 + 1) / 2 = F / 2, so a block of F / 2 + 6 coarse
+// cells per axis covers it (index c(p) - c(p0) + q <= F / 2 + 5).
+constexpr int kPFX = 16, kPFY = 8, kPFZ = 8;
+constexpr int kPCX = kPFX / 2 + 6, kPCY = kPFY / 2 + 6, kPCZ = kPFZ / 2 + 6;
+
 __global__ void __launch_bounds__(256) prolong3_kernel(XferSrc src, double* __restrict__ funf, XferPlan P) {
-  const int e0 = P.hi[0] - P.lo[0] + 1, e1 = P.hi[1] - P.lo[1] + 1, e2 = P.hi[2] - P.lo[2] + 1;
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)e0 * e1 * e2) return;
-  const int p[3] = {P.lo[0] + (int)(t % e0), P.lo[1] + (int)((t / e0) % e1), P.lo[2] + (int)(t / ((long)e0 * e1))};
-  // Lagrange weights at a quarter (even fine cell) or three quarters (odd)
-  // of the coarse cell c toward c + 1, over coarse cells c-2 .. c+3
-  constexpr double W[6] = {77.0 / 8192.0, -693.0 / 8192.0, 3465.0 / 4096.0,
-                           1155.0 / 4096.0, -495.0 / 8192.0, 63.0 / 8192.0};
-  int c[3];
-  bool rev[3];
+  __shared__ double B[kPCZ][kPCY][kPCX];
+  __shared__ double T1[kPFZ][kPCY][kPCX];
+  __shared__ double T2[kPFZ][kPFY][kPCX];
+  const int F[3] = {kPFX, kPFY, kPFZ};
+  int nt[3], p0[3], pe[3], c0[3], par0[3];
+  for (int a = 0; a < 3; ++a) nt[a] = (P.hi[a] - P.lo[a] + F[a]) / F[a];
+  const int b = blockIdx.x;
+  const int tb[3] = {b % nt[0], (b / nt[0]) % nt[1], b / (nt[0] * nt[1])};
   for (int a = 0; a < 3; ++a) {
-    const int g = p[a] + P.lbf[a] - 1;
-    c[a] = g / 2 - P.lbc[a] + 1;
-    rev[a] = (g % 2) != 0;
+    p0[a] = P.lo[a] + tb[a] * F[a];
+    pe[a] = min(p0[a] + F[a] - 1, P.hi[a]);
+    c0[a] = (p0[a] + P.lbf[a] - 1) / 2 - P.lbc[a] + 1;
+    par0[a] = p0[a] + P.lbf[a] - 1;  // fine global index of p0 (its parity picks the weights)
   }
-  double row[6];
-  for (int a = 0; a < 6; ++a) {
-    double col = 0.0;
-    for (int b = 0; b < 6; ++b) {
-      double s = 0.0;
-      for (int q = 0; q < 6; ++q) s += W[rev[2] ? 5 - q : q] * src.at(c[0] - 2 + a, c[1] - 2 + b, c[2] - 2 + q);
-      col += W[rev[1] ? 5 - b : b] * s;
-    }
-    row[a] = col;
+  const int tid = threadIdx.x;
+  // the coarse cells the tile's valid fine cells read, mirrored below 1
+  int nc[3];
+  for (int a = 0; a < 3; ++a) nc[a] = ((pe[a] + P.lbf[a] - 1) / 2 - P.lbc[a] + 1) + 3 - (c0[a] - 2) + 1;
+  for (int e = tid; e < nc[0] * nc[1] * nc[2]; e += 256) {
+    const int x = e % nc[0], y = (e / nc[0]) % nc[1], z = e / (nc[0] * nc[1]);
+    B[z][y][x] = src.at(c0[0] - 2 + x, c0[1] - 2 + y, c0[2] - 2 + z);
   }
-  double r = 0.0;
-  for (int a = 0; a < 6; ++a) r += W[rev[0] ? 5 - a : a] * row[a];
-  funf[(p[0] - 1) + (long)P.ext_f[0] * ((p[1] - 1) + (long)P.ext_f[1] * (p[2] - 1))] = r;
+  __syncthreads();
+  // offset of fine cell d of the tile in the block, and its weight order
+  auto off = [&](int a, int d) { return (par0[a] + d) / 2 - par0[a] / 2; };
+  auto rev = [&](int a, int d) { return ((par0[a] + d) & 1) != 0; };
+  for (int e = tid; e < kPFZ * kPCY * kPCX; e += 256) {
+    const int x = e % kPCX, y = (e / kPCX) % kPCY, fz = e / (kPCX * kPCY);
+    const int o = off(2, fz);
+    const bool r = rev(2, fz);
+    double s = 0.0;
+    for (int q = 0; q < 6; ++q) s += kProW[r ? 5 - q : q] * B[o + q][y][x];
+    T1[fz][y][x] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < kPFZ * kPFY * kPCX; e += 256) {
+    const int x = e % kPCX, fy = (e / kPCX) % kPFY, fz = e / (kPCX * kPFY);
+    const int o = off(1, fy);
+    const bool r = rev(1, fy);
+    double s = 0.0;
+    for (int q = 0; q < 6; ++q) s += kProW[r ? 5 - q : q] * T1[fz][o + q][x];
+    T2[fz][fy][x] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < kPFZ * kPFY * kPFX; e += 256) {
+    const int fx = e % kPFX, fy = (e / kPFX) % kPFY, fz = e / (kPFX * kPFY);
+    const int o = off(0, fx);
+    const bool r = rev(0, fx);
+    double s = 0.0;
+    for (int q = 0; q < 6; ++q) s += kProW[r ? 5 - q : q] * T2[fz][fy][o + q];
+    const int i = p0[0] + fx, j = p0[1] + fy, k = p0[2] + fz;
+    if (i <= pe[0] && j <= pe[1] && k <= pe[2])
+      funf[(i - 1) + (long)P.ext_f[0] * ((j - 1) + (long)P.ext_f[1] * (k - 1))] = s;
+  }
 }
 
+// restrict: coarse tile 8 x 8 x 4.  Coarse cell p reads fine cells
+// f(p) - 2 .. f(p) + 3, f(p) = 2 (p + lbc - 1) - lbf; over a tile f advances
+// by 2 per cell, so a block of 2 C + 4 fine cells per axis covers it
+// (index 2 d + q <= 2 C + 3).
+constexpr int kRCX = 8, kRCY = 8, kRCZ = 4;
+constexpr int kRFX = 2 * kRCX + 4, kRFY = 2 * kRCY + 4, kRFZ = 2 * kRCZ + 4;
+
 __global__ void __launch_bounds__(256) restrict3_kernel(XferSrc src, double* __restrict__ func, XferPlan P) {
-  const int e0 = P.hi[0] - P.lo[0] + 1, e1 = P.hi[1] - P.lo[1] + 1, e2 = P.hi[2] - P.lo[2] + 1;
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)e0 * e1 * e2) return;
-  const int p[3] = {P.lo[0] + (int)(t % e0), P.lo[1] + (int)((t / e0) % e1), P.lo[2] + (int)(t / ((long)e0 * e1))};
-  // midpoint weights, symmetric: pairs (0,5), (1,4), (2,3) of fine cells c-2 .. c+3
-  constexpr double W[3] = {3.0 / 256.0, -25.0 / 256.0, 75.0 / 128.0};
-  int c[3];
-  for (int a = 0; a < 3; ++a) c[a] = 2 * (p[a] + P.lbc[a] - 1) - P.lbf[a];
-  double row[6];
-  for (int a = 0; a < 6; ++a) {
-    double cols[6];
-    for (int b = 0; b < 6; ++b) {
-      double v[6];
-      for (int q = 0; q < 6; ++q) v[q] = src.at(c[0] - 2 + a, c[1] - 2 + b, c[2] - 2 + q);
-      cols[b] = W[0] * (v[0] + v[5]) + W[1] * (v[1] + v[4]) + W[2] * (v[2] + v[3]);
-    }
-    row[a] = W[0] * (cols[0] + cols[5]) + W[1] * (cols[1] + cols[4]) + W[2] * (cols[2] + cols[3]);
+  __shared__ double B[kRFZ][kRFY][kRFX];
+  __shared__ double T1[kRCZ][kRFY][kRFX];
+  __shared__ double T2[kRCZ][kRCY][kRFX];
+  const int Ct[3] = {kRCX, kRCY, kRCZ};
+  int nt[3], p0[3], pe[3], f0[3];
+  for (int a = 0; a < 3; ++a) nt[a] = (P.hi[a] - P.lo[a] + Ct[a]) / Ct[a];
+  const int b = blockIdx.x;
+  const int tb[3] = {b % nt[0], (b / nt[0]) % nt[1], b / (nt[0] * nt[1])};
+  for (int a = 0; a < 3; ++a) {
+    p0[a] = P.lo[a] + tb[a] * Ct[a];
+    pe[a] = min(p0[a] + Ct[a] - 1, P.hi[a]);
+    f0[a] = 2 * (p0[a] + P.lbc[a] - 1) - P.lbf[a];
   }
-  func[(p[0] - 1) + (long)P.ext_c[0] * ((p[1] - 1) + (long)P.ext_c[1] * (p[2] - 1))] =
-      W[0] * (row[0] + row[5]) + W[1] * (row[1] + row[4]) + W[2] * (row[2] + row[3]);
+  const int tid = threadIdx.x;
+  int nf[3];
+  for (int a = 0; a < 3; ++a) nf[a] = 2 * (pe[a] - p0[a]) + 6;
+  for (int e = tid; e < nf[0] * nf[1] * nf[2]; e += 256) {
+    const int x = e % nf[0], y = (e / nf[0]) % nf[1], z = e / (nf[0] * nf[1]);
+    B[z][y][x] = src.at(f0[0] - 2 + x, f0[1] - 2 + y, f0[2] - 2 + z);
+  }
+  __syncthreads();
+  for (int e = tid; e < kRCZ * kRFY * kRFX; e += 256) {
+    const int x = e % kRFX, y = (e / kRFX) % kRFY, cz = e / (kRFX * kRFY);
+    const double* v = &B[2 * cz][y][x];
+    constexpr int S = kRFY * kRFX;
+    T1[cz][y][x] = kResW[0] * (v[0] + v[5 * S]) + kResW[1] * (v[S] + v[4 * S]) + kResW[2] * (v[2 * S] + v[3 * S]);
+  }
+  __syncthreads();
+  for (int e = tid; e < kRCZ * kRCY * kRFX; e += 256) {
+    const int x = e % kRFX, cy = (e / kRFX) % kRCY, cz = e / (kRFX * kRCY);
+    const double* v = &T1[cz][2 * cy][x];
+    constexpr int S = kRFX;
+    T2[cz][cy][x] = kResW[0] * (v[0] + v[5 * S]) + kResW[1] * (v[S] + v[4 * S]) + kResW[2] * (v[2 * S] + v[3 * S]);
+  }
+  __syncthreads();
+  {
+    const int cx = tid % kRCX, cy = (tid / kRCX) % kRCY, cz = tid / (kRCX * kRCY);
+    const double* v = &T2[cz][cy][2 * cx];
+    const double r = kResW[0] * (v[0] + v[5]) + kResW[1] * (v[1] + v[4]) + kResW[2] * (v[2] + v[3]);
+    const int i = p0[0] + cx, j = p0[1] + cy, k = p0[2] + cz;
+    if (i <= pe[0] && j <= pe[1] && k <= pe[2])
+      func[(i - 1) + (long)P.ext_c[0] * ((j - 1) + (long)P.ext_c[1] * (k - 1))] = r;
+  }
 }
 
 // target points of a plan; 0 when any axis is empty (nothing is launched)
@@ -140,16 +211,23 @@ static long plan_points(const XferPlan& P) {
   return n;
 }
 
+// tiles of size T covering the target range of a plan
+static unsigned plan_tiles(const XferPlan& P, const int T[3]) {
+  long n = 1;
+  for (int a = 0; a < 3; ++a) n *= (P.hi[a] - P.lo[a] + T[a]) / T[a];
+  return (unsigned)n;
+}
+
 void launch_prolong3(const double* func, double* funf, const XferPlan& P, const double S[3], hipStream_t s) {
-  const long n = plan_points(P);
+  const int T[3] = {kPFX, kPFY, kPFZ};
   XferSrc src{func, P.ext_c[0], P.ext_c[1], {S[0], S[1], S[2]}};
-  if (n > 0) prolong3_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, funf, P);
+  if (plan_points(P) > 0) prolong3_kernel<<<plan_tiles(P, T), 256, 0, s>>>(src, funf, P);
 }
 
 void launch_restrict3(const double* funf, double* func, const XferPlan& P, const double S[3], hipStream_t s) {
-  const long n = plan_points(P);
+  const int T[3] = {kRCX, kRCY, kRCZ};
   XferSrc src{funf, P.ext_f[0], P.ext_f[1], {S[0], S[1], S[2]}};
-  if (n > 0) restrict3_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, func, P);
+  if (plan_points(P) > 0) restrict3_kernel<<<plan_tiles(P, T), 256, 0, s>>>(src, func, P);
 }
 
 }  // namespace amss
