@@ -183,7 +183,7 @@ def uneven_level(npatch: int, lo: int, hi: int, seed: int, half_width: float, gw
 
 
 def config2(steps: int = 50) -> Config:
-    """37 uneven patches (multiples of 8 in [24,96], seed 1) in [-24,24]^3, C1's data."""
+    """37 uneven patches (multiples of 8 in [24,96], seed 1) in [-24,24]^3, the two-puncture data of config1()."""
     patches, _ = uneven_level(37, 24, 96, seed=1, half_width=24.0)
     h = patches[0].h[0]
     return Config("37 uneven patches in [-24,24]^3", patches, dt=0.25 * h, steps=steps,

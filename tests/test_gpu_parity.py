@@ -3,8 +3,8 @@ reference Fortran, itself pinned to the reference build bit for bit).
 
 All calls go through the C ABI of libamssbssn.so.  Metric: per field,
 max |gpu - oracle| / max |oracle| (absolute when the oracle field is zero).
-Tolerances: 1e-12 per field for one RHS call and one RK4 step (the north-star
-one-step bar), 1e-9 after many steps.  The GPU evaluates the same algebra in
+Tolerances: 1e-12 per field for one RHS call and one RK4 step, 1e-9 after
+many steps.  The GPU evaluates the same algebra in
 a different operation order (tensor-index form, FMA contraction), so
 agreement is to rounding, not bitwise.
 """
@@ -384,7 +384,7 @@ def test_c0_one_step_vs_reference_vectors():
     worst = _vs_gold(lv.get_state(0), g, 1e-12)
     assert abs(lv.ham_l2() / float(g["ham_l2"]) - 1) < 1e-12
     lv.close()
-    print(f"C0 one step vs reference: worst {worst:.2e}")
+    print(f"single puncture 48^3, one step vs reference: worst {worst:.2e}")
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -398,7 +398,7 @@ def test_c1_32cubed_100_steps_vs_reference_vectors(variant):
     worst = _vs_gold(lv.get_state(0), g, 1e-9)
     assert abs(lv.ham_l2() / float(g["ham_l2"][-1]) - 1) < 1e-9
     lv.close()
-    print(f"C1 32^3, 100 steps vs reference: worst {worst:.2e}")
+    print(f"two punctures 32^3, 100 steps vs reference: worst {worst:.2e}")
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -411,7 +411,7 @@ def test_c1_128cubed_100_steps_vs_reference_vectors(variant):
     assert lv.step(cfg.dt, 100) == 0
     worst = _vs_gold(lv.get_state(0), g, 1e-9)
     lv.close()
-    print(f"C1 128^3, 100 steps vs reference: worst {worst:.2e}")
+    print(f"two punctures 128^3, 100 steps vs reference: worst {worst:.2e}")
 
 
 @pytest.mark.parametrize("variant", [0, 1])
