@@ -154,7 +154,7 @@ def transfer_leg(reps=20):
 
 
 def adjoint_leg(steps, checkpoint=20):
-    """The 64^3 differentiation workload (config-1 data over [-16,16]^3,
+    """The 64^3 differentiation workload (two-puncture data over [-16,16]^3,
     parameters: masses, x positions, 16^3 spline coefficients on A_xx): J and
     dJ/dtheta through the checkpointed step adjoint, the wall time of the
     backward pass over the forward pass, and a central difference of dJ/dm1."""
