@@ -201,6 +201,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128, help="interior points per axis")
     ap.add_argument("--mode", choices=["plain", "batched", "graph"], default="plain")
+    ap.add_argument("--bc", choices=["frozen", "copy", "none", "sommerfeld"], default="frozen",
+                    help="outer boundary of the 128^3 step: frozen ghost zones (the benchmark workload), or the "
+                         "reference's one-layer copy, none, or level-0 Sommerfeld (boundary kernel on the shell)")
     ap.add_argument("--variant", type=int, choices=[0, 1], default=None,
                     help="interior kernel: 0 direct stencil reads, 1 LDS-staged tiles (default: the level's)")
     ap.add_argument("--batch-steps", type=int, default=10,
@@ -229,7 +232,9 @@ def main():
 
     cfg = A.config1(n=args.n)
     mode = {"plain": A.MODE_PLAIN, "batched": A.MODE_BATCHED, "graph": A.MODE_GRAPH}[args.mode]
-    lv = A.Level.from_config(cfg, gauge=0)
+    bc = {"frozen": A.BC_FROZEN_GHOSTS, "copy": A.BC_COPY_LAYER, "none": A.BC_NONE,
+          "sommerfeld": A.BC_SOMMERFELD}[args.bc]
+    lv = A.Level.from_config(cfg, gauge=0, bc=bc)
     if args.variant is not None:
         lv.set_variant(args.variant)
     variant = lv.variant()
@@ -260,7 +265,7 @@ def main():
     ach_tf = flops / ktime / 1e12
     ai = perf.FLOPS_PER_PT_RHS / (sum(perf.BYTES_PER_PT_STAGE) / 4)
     ridge = perf.PEAK_FP64_TFLOPS * 1e12 / (perf.PEAK_HBM_GBS * 1e9)
-    traffic = traffic_from_profiles() if (variant == 1 and args.n == 128) else None
+    traffic = traffic_from_profiles() if (variant == 1 and args.n == 128 and args.bc == "frozen") else None
     roof = dict(bound="hbm" if ai < ridge else "fp64", achieved=round(ach_gbs, 1), peak=perf.PEAK_HBM_GBS,
                 unit="GB/s", frac=round(ach_gbs / perf.PEAK_HBM_GBS, 4), traffic=traffic,
                 kernel=("rk4_lds_kernel" if variant == 1 else "rk4_interior_kernel") +
@@ -280,7 +285,7 @@ def main():
                data="synthetic: analytic two-puncture (Brill-Lindquist) initial data, Philox A_ij noise",
                config=dict(workload=f"two-puncture BSSN RK4 step, one {args.n}^3 patch + 3 ghosts over [-16,16]^3",
                            interior_points=pts, rhs_evals_per_step=4, gauge=0, ko_eps=cfg.eps, dt=cfg.dt,
-                           mode=args.mode, variant=variant, stage_kernel_ms=[round(m / max(n, 1), 4) for m, n in zip(ms, nl)]),
+                           mode=args.mode, boundary=args.bc, variant=variant, stage_kernel_ms=[round(m / max(n, 1), 4) for m, n in zip(ms, nl)]),
                roofline=roof)
     lv.close()
     if args.batch_steps > 0:
