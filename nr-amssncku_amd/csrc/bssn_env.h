@@ -95,8 +95,13 @@ struct InteriorEnv {
 // axis gets zero derivatives and no advection, the upwind stencil falls back
 // per direction, KO needs +-3 room on every axis, and the halo index -1
 // mirrors index 0 with the field's parity (symmetry_bd, fmisc.f90:877-901).
+// ORD: the order of the centred stencils at the point (2, 1 or 0), fixed per
+// launch by the shell the point lies in, so that each launch compiles only
+// its own stencils.
+template <int ORD>
 struct EdgeEnv {
   static constexpr bool kFence = true;
+  static constexpr int ord = ORD;
   const FieldsIn& F;
   const FieldsIn& M;
   const Coef& C;
@@ -105,7 +110,6 @@ struct EdgeEnv {
   int lo[3];  // lowest readable index per axis (0, or -3 on a symmetry plane)
   long sy, sz;
   bool last, kook;
-  int ord;
 
   __device__ EdgeEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, int i_, int j_, int k_,
                      const int* n_, const int* lo_, long sy_, long sz_)
@@ -116,14 +120,13 @@ struct EdgeEnv {
       lo[a] = lo_[a];
     }
     last = (i == n[0] - 1) || (j == n[1] - 1) || (k == n[2] - 1);
-    // the joint three-axis room test of fderivs/fdderivs/kodis
-    // (diff_new.f90:996-1100, 1397-1524, kodiss.f90:162-212)
+    // the joint three-axis room test of kodis (kodiss.f90:162-212); the
+    // stencil order's test (diff_new.f90:996-1100, 1397-1524) chose ORD
     auto rm = [&](int r) {
       return i - r >= lo[0] && i + r <= n[0] - 1 && j - r >= lo[1] && j + r <= n[1] - 1 && k - r >= lo[2] &&
              k + r <= n[2] - 1;
     };
-    ord = last ? 0 : (rm(2) ? 2 : (rm(1) ? 1 : 0));
-    kook = rm(3);
+    kook = ORD == 2 && rm(3);
   }
   __device__ double rd(int v, int a, int b, int c, double off) const {
     double s = 1.0;

@@ -27,48 +27,83 @@ __device__ __forceinline__ bool interior_index(const Box& box, int& i, int& j, i
   k = box.lo[2] + blockIdx.z;
   return i < box.hi[0] && j < box.hi[1];
 }
-// The points outside the box [lo, hi) of an array, enumerated as six
-// disjoint slabs (z faces whole, then y faces over the middle z range, then
-// x faces over the middle y and z ranges), so that every lane of a wave has a
-// point: walking the whole array instead leaves an x-face row's three points
-// alone in a wave.  An empty box makes the shell the whole array.
+// The points of an outer box [OL, OH) outside an inner box [IL, IH) (clamped
+// into it), enumerated as six disjoint slabs (z faces whole, then y faces
+// over the inner z range, then x faces over the inner y and z ranges), so
+// that every lane of a wave has a point: walking the whole array instead
+// leaves an x-face row's points alone in a wave.  An inner box empty along
+// any axis holds no point, and the slabs then cover the whole outer box.
 struct Shell {
-  int n[3], L[3], H[3];
+  int OL[3], OH[3], IL[3], IH[3];
   long cnt[6];
-  __host__ __device__ explicit Shell(const Box& b) {
+  __host__ __device__ Shell(const int ol[3], const int oh[3], const int il[3], const int ih[3]) {
+    bool empty = false;
     for (int a = 0; a < 3; ++a) {
-      n[a] = b.n[a];
-      L[a] = b.lo[a] < 0 ? 0 : (b.lo[a] > n[a] ? n[a] : b.lo[a]);
-      H[a] = b.hi[a] < L[a] ? L[a] : (b.hi[a] > n[a] ? n[a] : b.hi[a]);
+      OL[a] = ol[a];
+      OH[a] = oh[a] < ol[a] ? ol[a] : oh[a];
+      IL[a] = il[a] < OL[a] ? OL[a] : (il[a] > OH[a] ? OH[a] : il[a]);
+      IH[a] = ih[a] < IL[a] ? IL[a] : (ih[a] > OH[a] ? OH[a] : ih[a]);
+      empty = empty || IH[a] == IL[a];
     }
-    const long mk = H[2] - L[2], mj = H[1] - L[1];
-    cnt[0] = (long)n[0] * n[1] * L[2];
-    cnt[1] = (long)n[0] * n[1] * (n[2] - H[2]);
-    cnt[2] = (long)n[0] * L[1] * mk;
-    cnt[3] = (long)n[0] * (n[1] - H[1]) * mk;
-    cnt[4] = (long)L[0] * mj * mk;
-    cnt[5] = (long)(n[0] - H[0]) * mj * mk;
+    if (empty)
+      for (int a = 0; a < 3; ++a) IH[a] = IL[a];
+    const long ex = OH[0] - OL[0], ey = OH[1] - OL[1];
+    const long mk = IH[2] - IL[2], mj = IH[1] - IL[1];
+    cnt[0] = ex * ey * (IL[2] - OL[2]);
+    cnt[1] = ex * ey * (OH[2] - IH[2]);
+    cnt[2] = ex * (IL[1] - OL[1]) * mk;
+    cnt[3] = ex * (OH[1] - IH[1]) * mk;
+    cnt[4] = (long)(IL[0] - OL[0]) * mj * mk;
+    cnt[5] = (long)(OH[0] - IH[0]) * mj * mk;
   }
   __host__ __device__ long size() const { return cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4] + cnt[5]; }
   // point t of the shell (t < size())
   __device__ void point(long t, int& i, int& j, int& k) const {
     int s = 0;
     while (s < 5 && t >= cnt[s]) t -= cnt[s++];
-    // slab s spans [x0, x0 + ex) x [y0, y0 + ey) x [z0, ...)
-    const int ex = s < 4 ? n[0] : (s == 4 ? L[0] : n[0] - H[0]);
-    const int ey = s < 2 ? n[1] : (s == 2 ? L[1] : (s == 3 ? n[1] - H[1] : H[1] - L[1]));
-    const int x0 = s == 5 ? H[0] : 0;
-    const int y0 = s < 3 ? 0 : (s == 3 ? H[1] : L[1]);
-    const int z0 = s == 0 ? 0 : (s == 1 ? H[2] : L[2]);
+    const int x0 = s < 4 ? OL[0] : (s == 4 ? OL[0] : IH[0]);
+    const int ex = s < 4 ? OH[0] - OL[0] : (s == 4 ? IL[0] - OL[0] : OH[0] - IH[0]);
+    const int y0 = s < 2 ? OL[1] : (s == 2 ? OL[1] : (s == 3 ? IH[1] : IL[1]));
+    const int ey = s < 2 ? OH[1] - OL[1] : (s == 2 ? IL[1] - OL[1] : (s == 3 ? OH[1] - IH[1] : IH[1] - IL[1]));
+    const int z0 = s == 0 ? OL[2] : (s == 1 ? IH[2] : IL[2]);
     i = x0 + (int)(t % ex);
     j = y0 + (int)((t / ex) % ey);
     k = z0 + (int)(t / ((long)ex * ey));
   }
 };
 
-__device__ __forceinline__ bool edge_index(const Box& box, int& i, int& j, int& k) {
+// The boundary points of an array outside the interior box, split by the
+// order of their centred stencils (the joint three-axis room test of
+// fderivs/fdderivs, diff_new.f90:996-1100, with the lowest readable index
+// mlo): order 2 on [max(0, mlo + 2), n - 2), order 1 on the next layer out to
+// [max(0, mlo + 1), n - 1), order 0 (and the last index) on the rest.
+struct EdgeShells {
+  Shell s[3];  // by stencil order 0, 1, 2
+};
+inline EdgeShells edge_shells(const Box& b) {
+  int z[3] = {0, 0, 0}, L1[3], H1[3], L2[3], H2[3];
+  for (int a = 0; a < 3; ++a) {
+    L1[a] = b.mlo[a] + 1 > 0 ? b.mlo[a] + 1 : 0;
+    H1[a] = b.n[a] - 1;
+    L2[a] = b.mlo[a] + 2 > 0 ? b.mlo[a] + 2 : 0;
+    H2[a] = b.n[a] - 2;
+    if (H2[a] < L2[a]) H2[a] = L2[a];
+    if (H1[a] < L1[a]) H1[a] = L1[a];
+  }
+  int blo[3], bhi[3];
+  bool empty = false;
+  for (int a = 0; a < 3; ++a) {
+    blo[a] = b.lo[a];
+    bhi[a] = b.hi[a];
+    empty = empty || b.hi[a] <= b.lo[a];
+  }
+  if (empty)  // no interior launch: the order-2 shell takes its box whole
+    for (int a = 0; a < 3; ++a) bhi[a] = blo[a] = L2[a];
+  return EdgeShells{{Shell(z, b.n, L1, H1), Shell(L1, H1, L2, H2), Shell(L2, H2, blo, bhi)}};
+}
+
+__device__ __forceinline__ bool edge_index(const Shell& sh, int& i, int& j, int& k) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const Shell sh(box);
   if (t >= sh.size()) return false;
   sh.point(t, i, j, k);
   return true;
@@ -87,14 +122,14 @@ AMSS_POINT_KERNEL rhs_interior_kernel(FieldsIn in, FieldsIn mat, FieldsOut out, 
   bssn_rhs_point<GAUGE, CO, true, true>(E, S);
 }
 
-template <int GAUGE, bool CO>
+template <int GAUGE, bool CO, int ORD>
 __global__ void __launch_bounds__(256) rhs_edge_kernel(FieldsIn in, FieldsIn mat, FieldsOut out, AuxOut aux,
-                                                       Box box, Coef C, int* flag) {
+                                                       Box box, Shell sh, Coef C, int* flag) {
   int i, j, k;
-  if (!edge_index(box, i, j, k)) return;
+  if (!edge_index(sh, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
-  EdgeEnv E(in, mat, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
+  EdgeEnv<ORD> E(in, mat, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
   RhsSink S{out, aux, p};
   bssn_rhs_point<GAUGE, CO, true, true>(E, S);
 }
@@ -153,17 +188,16 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
 #ifndef AMSS_EDGE_WAVES
 #define AMSS_EDGE_WAVES 2
 #endif
-template <int GAUGE, int STAGE, bool CO>
+template <int GAUGE, int STAGE, bool CO, int ORD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AMSS_EDGE_WAVES, AMSS_EDGE_WAVES)))
-rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
-                                                       AuxOut aux, Box box, Coef C, Rk4Param R, int* flag,
-                                                       SommerGeom sg) {
+rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box, Shell sh, Coef C,
+                Rk4Param R, int* flag, SommerGeom sg) {
   int i, j, k;
-  if (!edge_index(box, i, j, k)) return;
+  if (!edge_index(sh, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
-  EdgeEnv E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
-  SommerRk4Sink<EdgeEnv, Rk4SinkT<STAGE>> S{
+  EdgeEnv<ORD> E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
+  SommerRk4Sink<EdgeEnv<ORD>, Rk4SinkT<STAGE>> S{
       {u0, acc, out, aux, p, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
   bssn_rhs_point<GAUGE, CO, false, false>(E, S);
   S.finish();
@@ -179,14 +213,23 @@ inline unsigned blocks_for(long n, int t) { return (unsigned)((n + t - 1) / t); 
 template <int G>
 void launch_rhs_gauge(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out, const AuxOut& aux,
                       const Box& box, const Coef& C, int co, int* flag, bool edge, hipStream_t s) {
-  edge = edge && Shell(box).size() > 0;
   if (co == 0) {
     if (!box_empty(box)) rhs_interior_kernel<G, true><<<interior_grid(box), kRhsThreads, 0, s>>>(in, mat, out, aux, box, C, flag);
-    if (edge) rhs_edge_kernel<G, true><<<blocks_for(Shell(box).size(), 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
   } else {
     if (!box_empty(box)) rhs_interior_kernel<G, false><<<interior_grid(box), kRhsThreads, 0, s>>>(in, mat, out, aux, box, C, flag);
-    if (edge) rhs_edge_kernel<G, false><<<blocks_for(Shell(box).size(), 256), 256, 0, s>>>(in, mat, out, aux, box, C, flag);
   }
+  if (!edge) return;
+  const EdgeShells es = edge_shells(box);
+#define AMSS_REDGE(O)                                                                                       \
+  if (es.s[O].size() > 0) {                                                                                 \
+    const unsigned gr = blocks_for(es.s[O].size(), 256);                                                    \
+    if (co == 0) rhs_edge_kernel<G, true, O><<<gr, 256, 0, s>>>(in, mat, out, aux, box, es.s[O], C, flag);  \
+    else rhs_edge_kernel<G, false, O><<<gr, 256, 0, s>>>(in, mat, out, aux, box, es.s[O], C, flag);         \
+  }
+  AMSS_REDGE(0)
+  AMSS_REDGE(1)
+  AMSS_REDGE(2)
+#undef AMSS_REDGE
 }
 
 template <int G>
@@ -234,19 +277,29 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
       default: rk4_interior_kernel<G, 3, false><<<gr, kRhsThreads, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag); break;
     }
   }
-  if (edge && Shell(box).size() > 0) {
-    const unsigned gr = blocks_for(Shell(box).size(), 256);
-#define AMSS_EDGE(ST, CO) rk4_edge_kernel<G, ST, CO><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, C, R, flag, sg)
-    switch (R.stage) {
-      case 0:
-        if (want_res) AMSS_EDGE(0, true);
-        else AMSS_EDGE(0, false);
-        break;
-      case 1: AMSS_EDGE(1, false); break;
-      case 2: AMSS_EDGE(2, false); break;
-      default: AMSS_EDGE(3, false); break;
-    }
+  if (edge) {
+    const EdgeShells es = edge_shells(box);
+    for (int o = 0; o < 3; ++o) {
+      const Shell& sh = es.s[o];
+      if (sh.size() <= 0) continue;
+      const unsigned gr = blocks_for(sh.size(), 256);
+#define AMSS_EDGE(ST, CO)                                                                                      \
+  do {                                                                                                         \
+    if (o == 0) rk4_edge_kernel<G, ST, CO, 0><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, C, R, flag, sg); \
+    else if (o == 1) rk4_edge_kernel<G, ST, CO, 1><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, C, R, flag, sg); \
+    else rk4_edge_kernel<G, ST, CO, 2><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, C, R, flag, sg);       \
+  } while (0)
+      switch (R.stage) {
+        case 0:
+          if (want_res) AMSS_EDGE(0, true);
+          else AMSS_EDGE(0, false);
+          break;
+        case 1: AMSS_EDGE(1, false); break;
+        case 2: AMSS_EDGE(2, false); break;
+        default: AMSS_EDGE(3, false); break;
+      }
 #undef AMSS_EDGE
+    }
   }
 }
 
