@@ -42,7 +42,8 @@ __global__ void __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_e
 rhs_k_lds_kernel(FieldsIn in, FieldsOut out, Box box, Coef C) {
   using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
   __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
-  const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
+  const int tx = threadIdx.x % kLdsTX, ty = tile_row((threadIdx.x / kLdsTX) % kLdsTY, kLdsTY),
+            tz = threadIdx.x / (kLdsTX * kLdsTY);
   const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
   const int b = xcd_spread(blockIdx.x, gridDim.x);
   const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;
@@ -122,7 +123,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 adj_feature_lds_kernel(FieldsIn in, FieldsIn lam, double* feat, long fs, Box box, Coef C) {
   using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
   __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
-  const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
+  const int tx = threadIdx.x % kLdsTX, ty = tile_row((threadIdx.x / kLdsTX) % kLdsTY, kLdsTY),
+            tz = threadIdx.x / (kLdsTX * kLdsTY);
   const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
   const int b = xcd_spread(blockIdx.x, gridDim.x);
   const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;

@@ -13,7 +13,7 @@ template <int GAUGE, int STAGE, bool CO, int TX, int TY, int TZ>
 __device__ __forceinline__ void tile_body(const PatchDesc* __restrict__ P, const TileRef& r, const Roles& m,
                                           const Coef& C, const Rk4Param& R, int* flag, lds_double* b0, int tid) {
   using Tl = LdsTile<TX, TY, TZ>;
-  const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
+  const int tx = tid % TX, ty = tile_row((tid / TX) % TY, TY), tz = tid / (TX * TY);
   const int ltv = (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3));
   const PatchDesc d = P[r.patch];
   const long p = (r.i0 + tx) + (long)(r.j0 + ty) * d.sy + (long)(r.k0 + tz) * d.sz;

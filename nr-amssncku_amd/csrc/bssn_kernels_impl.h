@@ -162,7 +162,8 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
                                  Coef C, Rk4Param R, int* flag) {
   using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
   __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
-  const int tx = threadIdx.x % kLdsTX, ty = (threadIdx.x / kLdsTX) % kLdsTY, tz = threadIdx.x / (kLdsTX * kLdsTY);
+  const int tx = threadIdx.x % kLdsTX, ty = tile_row((threadIdx.x / kLdsTX) % kLdsTY, kLdsTY),
+            tz = threadIdx.x / (kLdsTX * kLdsTY);
   const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
   const int b = xcd_spread(blockIdx.x, gridDim.x);
   const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;

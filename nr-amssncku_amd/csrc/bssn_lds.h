@@ -38,6 +38,16 @@ __host__ __device__ constexpr int group_extra(int v) {
 // every stencil offset a 64-bit flat address).
 typedef __attribute__((address_space(3))) double lds_double;
 
+#ifndef AMSS_ROWPERM
+#define AMSS_ROWPERM 1
+#endif
+// Tile row of a thread's row index r (of TY): with 8 rows a 16-lane LDS
+// access group can hold rows y and y + 4, whose 14-double pitch puts them
+// 112 dwords apart (16 mod 32 banks).
+__host__ __device__ constexpr int tile_row(int r, int TY) {
+  return (AMSS_ROWPERM && TY == 8) ? ((r & 1) ? 4 + (r >> 1) : (r >> 1)) : r;
+}
+
 template <int TX, int TY, int TZ>
 struct LdsTile {
   static constexpr int LX = TX + 6, LY = TY + 6, LZ = TZ + 6;
