@@ -200,7 +200,15 @@ rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut a
   EdgeEnv<ORD> E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
   SommerRk4Sink<EdgeEnv<ORD>, Rk4SinkT<STAGE>> S{
       {u0, acc, out, aux, p, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
-  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  // On a Sommerfeld face every field's RHS is replaced before the update, so
+  // unless the residuals are wanted the RHS algebra there is skipped (face
+  // points lie on the outermost layer: stencil order 0).
+  if (ORD == 0 && !CO && S.on) {
+#pragma unroll
+    for (int v = 0; v < NFIELD; ++v) S.put(v, 0.0);
+  } else {
+    bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  }
   S.finish();
 }
 
