@@ -235,6 +235,7 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
     }
     faces_.push_back(fc);
     SommerGeom g;
+    if (bc == AMSS_BC_COPY_LAYER) g.copy = fc;
     if (bc == AMSS_BC_SOMMERFELD) {
       g.faces = fc;
       for (int a = 0; a < 3; ++a) {

@@ -349,14 +349,22 @@ struct SommerGeom {
   double h[3] = {1.0, 1.0, 1.0};
   int faces = 0;           // bit a: the upper face of axis a is physical; bit 3 + a: the lower one
   int lo[3] = {0, 0, 0};   // lowest readable index: -1 below a symmetry plane (one mirrored layer)
+  // faces (same bits) whose outermost layer the one-layer copy overwrites
+  // right after the update (sommerfeld_rout's copy branch): its points need
+  // neither the RHS nor the update
+  int copy = 0;
 };
 
-__host__ __device__ __forceinline__ bool sommer_point(const SommerGeom& G, const int n[3], int i, int j, int k) {
+// point (i, j, k) lies on the outermost layer of a face in the bit mask
+__host__ __device__ __forceinline__ bool on_faces(int mask, const int n[3], int i, int j, int k) {
   const int p[3] = {i, j, k};
   bool on = false;
   for (int a = 0; a < 3; ++a)
-    on = on || (((G.faces >> a) & 1) && p[a] == n[a] - 1) || (((G.faces >> (3 + a)) & 1) && p[a] == 0);
+    on = on || (((mask >> a) & 1) && p[a] == n[a] - 1) || (((mask >> (3 + a)) & 1) && p[a] == 0);
   return on;
+}
+__host__ __device__ __forceinline__ bool sommer_point(const SommerGeom& G, const int n[3], int i, int j, int k) {
+  return on_faces(G.faces, n, i, j, k);
 }
 
 // Propagation speed: sqrt(2) for chi, trK and the lapse (the 1+log gauge

@@ -197,6 +197,7 @@ rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut a
   if (!edge_index(sh, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
+  if (ORD == 0 && !CO && on_faces(sg.copy, box.n, i, j, k)) return;
   EdgeEnv<ORD> E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
   SommerRk4Sink<EdgeEnv<ORD>, Rk4SinkT<STAGE>> S{
       {u0, acc, out, aux, p, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
