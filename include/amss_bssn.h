@@ -54,7 +54,8 @@ int amss_set_gauge(int gauge);
  * first 21 state arrays is NaN.  Symmetry 1 (equatorial: z) and 2 (octant:
  * x, y, z) mirror the stencils across a lower face whose first coordinate is
  * within one spacing of 0 (diff_new.f90:975-980), with each field's parity;
- * other values return 2. */
+ * other values return 2, and so does an axis of fewer than 5 points without a
+ * symmetry plane (the upwind stencil would read past it). */
 int compute_rhs_bssn_(int *ex, double *T, double *X, double *Y, double *Z,
                       double *chi, double *trK, double *dxx, double *gxy, double *gxz, double *dyy,
                       double *gyz, double *dzz, double *Axx, double *Axy, double *Axz, double *Ayy,

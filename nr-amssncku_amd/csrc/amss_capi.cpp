@@ -77,6 +77,16 @@ static int set_mirrors(Box& b, int symmetry, const double* x0, const double* h, 
   return 0;
 }
 
+// The upwind fallbacks for a negative shift read up to 2 points past the
+// upper end of an axis shorter than 5 points (the reference reads past its
+// arrays there too); an axis on a symmetry plane never does.
+static int check_upwind_room(const Box& b, const char* who) {
+  for (int a = 0; a < 3; ++a)
+    if (b.mlo[a] == 0 && b.n[a] < 5)
+      return fail(AMSS_ERR_ARG, std::string(who) + ": an axis without a symmetry plane needs 5 points");
+  return 0;
+}
+
 // ---- a grow-only device workspace + stream for the host-pointer entry points
 struct HostShim {
   std::mutex mu;
@@ -120,6 +130,7 @@ static int rhs_host(const int* ex, double* X, double* Y, double* Z, double* cons
   const double x0[3] = {X[0], Y[0], Z[0]};
   Box box = make_box(ex, 3, 3);
   if (int r = set_mirrors(box, symmetry, x0, h, "compute_rhs_bssn_")) return r;
+  if (int r = check_upwind_room(box, "compute_rhs_bssn_")) return r;
   const long n = (long)ex[0] * ex[1] * ex[2];
   if (sum_is_nan(st, n)) return AMSS_ERR_NONFINITE;
   std::lock_guard<std::mutex> lk(g_shim.mu);
@@ -478,6 +489,7 @@ int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state
   Box box = make_box(ex, 3, 3);
   if (symmetry != 0)
     if (int r = set_mirrors(box, symmetry, origin, h, "amss_rhs_bssn_dev")) return r;
+  if (int r = check_upwind_room(box, "amss_rhs_bssn_dev")) return r;
   const long n = (long)ex[0] * ex[1] * ex[2];
   if (n >= (1L << 29)) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: more than 2^29 points per block");
   hipStream_t s = (hipStream_t)stream;

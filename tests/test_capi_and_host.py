@@ -63,6 +63,11 @@ def test_rhs_symmetry_arguments_rejected_before_any_gpu_work():
     with pytest.raises(RuntimeError, match="symmetry axis needs 3 points"):
         fa.compute_rhs(st2, X, X, Z, 0.02, symmetry=1)
     assert "3 points" in amssbssn.last_error()
+    # without a plane an axis needs 5 points: the upwind stencil reaches 2 past a shorter one
+    Z4 = -1.0 + np.arange(4) * 0.3
+    st4 = [np.zeros((4, 6, 6)) for _ in range(24)]
+    with pytest.raises(RuntimeError, match="needs 5 points"):
+        fa.compute_rhs(st4, X, X, Z4, 0.02)
 
 
 def test_level_symmetry_needs_a_reference_boundary_mode():
