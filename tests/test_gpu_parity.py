@@ -249,7 +249,7 @@ def test_sommerfeld_boundary_steps_match_oracle(orc, gauge):
 
 @pytest.mark.parametrize("sym", [1, 2])
 @pytest.mark.parametrize("bc", [1, 2, 3])
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_level_with_symmetry_planes_matches_oracle(orc, sym, bc, mode):
     """A level whose patch starts at the symmetry planes (equatorial: z;
     octant: x, y, z), two steps, per-patch and batched: mirrored boundary
