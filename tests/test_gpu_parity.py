@@ -112,6 +112,25 @@ def test_rhs_symmetry_planes_match_oracle(fa, orc, sym, gauge, co):
     assert not np.array_equal(z["rhs"][0][0], o["rhs"][0][0])
 
 
+@pytest.mark.parametrize("shape,sym", [((5, 6, 23), 0), ((9, 5, 7), 0), ((5, 5, 5), 0), ((6, 7, 6), 0),
+                                       ((3, 7, 5), 2), ((4, 4, 6), 2), ((8, 6, 3), 1)])
+def test_rhs_ragged_and_minimal_shapes_match_oracle(fa, orc, shape, sym):
+    """Thin or empty interiors (the boundary kernels' stencil-order shells
+    then cover most or all points), the smallest axes the entry point accepts
+    (5 points, 3 on a symmetry plane), odd row pitches."""
+    nx, ny, nz = shape
+    X = _plane_grid(nx, 0.3, sym > 1)
+    Y = _plane_grid(ny, 0.27, sym > 1)
+    Z = _plane_grid(nz, 0.33, sym > 0)
+    st = random_state((nz, ny, nx), seed=200 + nx + 7 * ny + 31 * nz + sym)
+    fa.set_gauge(0)
+    for co in (0, 1):
+        g = fa.compute_rhs(st, X, Y, Z, 0.02, co=co, symmetry=sym)
+        o = orc.compute_rhs(st, X, Y, Z, 0.02, co=co, symmetry=sym, gauge=0)
+        assert g["gont"] == 0 and o["gont"] == 0
+        _cmp_rhs(g, o, co, f"shape {shape} symmetry {sym} co {co}")
+
+
 @pytest.mark.parametrize("faces", [(False, False, True), (True, False, True), (False, True, False)])
 def test_rhs_octant_with_some_faces_off_the_plane(fa, orc, faces):
     """Symmetry 2 mirrors only the axes whose first coordinate is within one
