@@ -153,6 +153,26 @@ def transfer_leg(reps=20):
     return out
 
 
+def boundary_leg(steps=5, warmup=2):
+    """The 128^3 two-puncture step under each outer boundary: frozen ghost
+    zones (the headline workload: no boundary kernel), and the reference's
+    one-layer copy, no boundary, and level-0 Sommerfeld (the 3-deep shell of
+    308,104 points through the boundary kernels with the reference's edge
+    stencils).  Wall ms per RK4 step, one launch per kind of work (mode 0)."""
+    import amssbssn as A
+    cfg = A.config1(n=128)
+    out = dict(workload="two-puncture 128^3 + 3 ghosts, GAUGE 0, plain mode", steps=steps)
+    for name, bc in (("frozen", A.BC_FROZEN_GHOSTS), ("copy", A.BC_COPY_LAYER), ("none", A.BC_NONE),
+                     ("sommerfeld", A.BC_SOMMERFELD)):
+        lv = A.Level.from_config(cfg, gauge=0, bc=bc)
+        assert lv.step(cfg.dt, warmup) == 0
+        t0 = time.perf_counter()
+        assert lv.step(cfg.dt, steps) == 0
+        out[name + "_ms_per_step"] = round((time.perf_counter() - t0) * 1e3 / steps, 3)
+        lv.close()
+    return out
+
+
 def adjoint_leg(steps, checkpoint=20):
     """The 64^3 differentiation workload (two-puncture data over [-16,16]^3,
     parameters: masses, x positions, 16^3 spline coefficients on A_xx): J and
@@ -214,6 +234,8 @@ def main():
                     help="steps of the 16-patch batched gradient run (0 skips it; the full workload is 400)")
     ap.add_argument("--transfer-reps", type=int, default=20,
                     help="timed repetitions of the inter-level transfer leg (0: skip)")
+    ap.add_argument("--boundary-steps", type=int, default=5,
+                    help="timed steps of the boundary-mode leg (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -292,6 +314,8 @@ def main():
         out["batching"] = batching(args.batch_steps, 2)
     if args.transfer_reps > 0:
         out["transfer"] = transfer_leg(args.transfer_reps)
+    if args.boundary_steps > 0:
+        out["boundary_modes"] = boundary_leg(args.boundary_steps)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
     if args.batched_adjoint_steps > 0:
