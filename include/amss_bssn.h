@@ -91,10 +91,16 @@ void lowerboundset_(int *ex, double *chi, double *tinny);
 void l2normhelper_(int *ex, double *X, double *Y, double *Z, double *xmin, double *ymin, double *zmin,
                    double *xmax, double *ymax, double *zmax, double *f, double *f_out, int *gw);
 
-/* Replaces f_sommerfeld_rout (sommerfeld_rout.h:26-29; sommerfeld_rout.f90:9-221)
- * for precor == 1 (copy f0 onto the outermost layer of every physical face).
- * precor == 0 (the interpolating Sommerfeld predictor) is not provided: the
- * call leaves f untouched and amss_last_error() says so. */
+/* Replaces f_sommerfeld_rout (sommerfeld_rout.h:26-29; sommerfeld_rout.f90:9-221).
+ * precor == 1: copy f0 onto the outermost layer of every physical face.
+ * Any other precor (the interpolating predictor of SommerType 1): each point
+ * of those layers becomes the outgoing wave (1 - r) f0(x - r x) with
+ * r = (Lap0 + 1) sqrt(1 + chi0) dT / |x|, f0 interpolated by 6-point
+ * polynomials along z, y, x over a 6^3 block kept inside the array (mirrored
+ * with SoA below a symmetry plane).  Where the array is too small for a
+ * block (the reference's decide3d error) those points are left unchanged.
+ * Errors (that one, Symmetry outside 0..2, a HIP failure) are written to
+ * stderr and to amss_last_error(). */
 void sommerfeld_rout_(int *ex, double *X, double *Y, double *Z, double *xmin, double *ymin, double *zmin,
                       double *xmax, double *ymax, double *zmax, double *dT, double *chi0, double *Lap0,
                       double *f0, double *f, double *SoA, int *Symmetry, int *precor);
@@ -203,12 +209,31 @@ int amss_level_copy_list(amss_level *L, int patch, int list, double *const *ptrs
 /* Advance nsteps RK4 steps of size dT (bssn_class::Step order).
  * mode 0: one launch per patch and sub-step; mode 1: batched work queue over
  * all patches; mode 2: batched, replayed from a captured hipGraph.
- * want_ham: compute the Hamiltonian residual in the last step's predictor.
+ * want_ham: 0 no constraint residuals; 1 the residuals (Cons_Ham, Cons_P*,
+ * Cons_G*) in the last step's predictor only; 2 in every step's predictor,
+ * as the reference's Step does (co = 0 there, bssn_class.C:3086-3092), and
+ * the seven L2 norms of every step recorded (Constraint_Out,
+ * bssn_class.C:7475-7481; read them with amss_level_constraint_series).
  * Synchronises at the end; returns 1 if any RHS saw a NaN input. */
 int amss_level_step(amss_level *L, double dT, int nsteps, int mode, int want_ham);
-/* sqrt(sum H^2 dV) over the patches' interiors (ghost zones excluded), from
- * the last predictor that computed residuals (Parallel.C:4214-4243). */
+/* The L2 norm of Cons_Ham over all patches, from the last predictor that
+ * computed residuals (Parallel::L2Norm, Parallel.C:4214-4243); the same as
+ * amss_level_constraint_norms(L, -1, v) then v[0]. */
 int amss_level_ham_l2(amss_level *L, double *out);
+/* The seven constraint norms (Ham, Px, Py, Pz, Gx, Gy, Gz: Parallel::L2Norm
+ * of Cons_Ham .. Cons_Gz, bssn_class.C:7475-7481) of one patch, or of all
+ * patches together (patch -1: the square root of the summed squares), from
+ * the last predictor that computed residuals.  The points summed are those of
+ * l2normhelper (fmisc.f90:1263-1309): a level's Block always reaches its
+ * patch's bbox, so under the reference's boundary modes (copy layer, none,
+ * Sommerfeld) every point of the Block counts; with frozen ghost zones, whose
+ * residuals are never evaluated, the points [gw, n - gw) do.  The reference's
+ * Constraint_Out reports the first patch of a level (patch 0). */
+int amss_level_constraint_norms(amss_level *L, int patch, double *out7);
+/* The norms recorded by the last amss_level_step with want_ham 2: out[7 s + c]
+ * for step s < min(maxsteps, *nsteps), norm c as above; *nsteps receives the
+ * number of steps that call recorded. */
+int amss_level_constraint_series(amss_level *L, int patch, double *out, int maxsteps, int *nsteps);
 /* The stream the level's work is ordered on (a hipStream_t). */
 void *amss_level_stream(amss_level *L);
 

@@ -44,6 +44,8 @@ SIGNATURES = {
     "amss_level_copy": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int], C.c_int),
     "amss_level_step": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int], C.c_int),
     "amss_level_ham_l2": ([C.c_void_p, _dp], C.c_int),
+    "amss_level_constraint_norms": ([C.c_void_p, C.c_int, _dp], C.c_int),
+    "amss_level_constraint_series": ([C.c_void_p, C.c_int, _dp, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "amss_level_stream": ([C.c_void_p], C.c_void_p),
     "amss_level_write_bin": ([C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double], C.c_int),
     "amss_level_checkpoint_write": ([C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, _dp, _dp], C.c_int),

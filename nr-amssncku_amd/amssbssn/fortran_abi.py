@@ -82,6 +82,13 @@ def sommerfeld_copy(f0, f, X, Y, Z, bbox, symmetry=0):
                            _p(dummy), _p(f0), _p(f), soa, _i(symmetry), _i(1))
 
 
+def sommerfeld_interp(f0, f, chi0, lap0, X, Y, Z, bbox, dT, soa, symmetry=0):
+    """sommerfeld_rout_ with precor 0 (the interpolating predictor) on f in place."""
+    sa = (C.c_double * 3)(*soa)
+    lib().sommerfeld_rout_(_ex(f.shape), _p(X), _p(Y), _p(Z), *[_d(b) for b in bbox], _d(dT), _p(chi0),
+                           _p(lap0), _p(f0), _p(f), sa, _i(symmetry), _i(0))
+
+
 def sommerfeld_bam(f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
     """sommerfeld_routbam_ on f_rhs in place."""
     sa = (C.c_double * 3)(*soa)

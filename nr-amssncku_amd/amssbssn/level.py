@@ -15,6 +15,7 @@ from .grid import NF, Config
 BC_FROZEN_GHOSTS, BC_COPY_LAYER, BC_NONE, BC_SOMMERFELD = 0, 1, 2, 3
 LIST_STATE, LIST_OLD, LIST_PRE, LIST_COR, LIST_RHS, LIST_RES = range(6)
 MODE_PLAIN, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
+RES_NONE, RES_LAST_STEP, RES_EVERY_STEP = 0, 1, 2
 
 
 class Level:
@@ -74,8 +75,11 @@ class Level:
         return lib().amss_level_field(self.h, patch, list_, v)
 
     # -- evolution -------------------------------------------------------------
-    def step(self, dt: float, nsteps: int = 1, mode: int = MODE_PLAIN, want_ham: bool = False) -> int:
-        rc = lib().amss_level_step(self.h, dt, nsteps, mode, 1 if want_ham else 0)
+    def step(self, dt: float, nsteps: int = 1, mode: int = MODE_PLAIN, want_ham=False) -> int:
+        """want_ham: False/0 no residuals; True/1 residuals in the last step's
+        predictor; RES_EVERY_STEP (2) in every predictor, as the reference's
+        Step does, with the seven constraint norms recorded per step."""
+        rc = lib().amss_level_step(self.h, dt, nsteps, mode, int(want_ham))
         if rc not in (0, 1):
             check(rc, "amss_level_step")
         return rc
@@ -84,6 +88,21 @@ class Level:
         out = C.c_double(0.0)
         check(lib().amss_level_ham_l2(self.h, C.byref(out)), "amss_level_ham_l2")
         return out.value
+
+    def constraint_norms(self, patch: int = -1) -> np.ndarray:
+        """The 7 L2 norms (Ham, Px, Py, Pz, Gx, Gy, Gz) of the last residuals."""
+        out = (C.c_double * 7)()
+        check(lib().amss_level_constraint_norms(self.h, patch, out), "amss_level_constraint_norms")
+        return np.array(out[:])
+
+    def constraint_series(self, patch: int = -1) -> np.ndarray:
+        """(steps, 7) norms recorded by the last step() with RES_EVERY_STEP."""
+        n = C.c_int(0)
+        check(lib().amss_level_constraint_series(self.h, patch, None, 0, C.byref(n)), "amss_level_constraint_series")
+        out = (C.c_double * max(7 * n.value, 1))()
+        check(lib().amss_level_constraint_series(self.h, patch, out, n.value, C.byref(n)),
+              "amss_level_constraint_series")
+        return np.array(out[:7 * n.value]).reshape(n.value, 7)
 
     def stream(self) -> int:
         return lib().amss_level_stream(self.h)

@@ -12,11 +12,15 @@ kernel per interior point and RK4 sub-step: stage 0 reads the state and
 writes the accumulator and the stage output (72 FP64 fields); stages 1-2
 read the stage input, base state and accumulator and write accumulator and
 output (120); stage 3 reads input, base and accumulator and writes the new
-state (96).  Halo re-reads are not compulsory and not counted.
+state (96).  Halo re-reads are not compulsory and not counted.  A
+predictor that forms the constraint residuals (the reference's co = 0 call)
+also writes the 7 residual fields: BYTES_PER_PT_STAGE_RES.
 """
 FLOPS_PER_PT_RHS = 7596
 FLOPS_PER_PT_RHS_CO0 = 8492
 BYTES_PER_PT_STAGE = (72 * 8, 120 * 8, 120 * 8, 96 * 8)
+BYTES_PER_PT_STAGE_RES = (79 * 8, 120 * 8, 120 * 8, 96 * 8)
+FLOPS_PER_PT_STAGE_RES = (FLOPS_PER_PT_RHS_CO0, FLOPS_PER_PT_RHS, FLOPS_PER_PT_RHS, FLOPS_PER_PT_RHS)
 
 # MI355X published peaks
 PEAK_HBM_GBS = 8000.0        # HBM3E, GB/s

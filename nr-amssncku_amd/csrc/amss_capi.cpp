@@ -346,19 +346,56 @@ static int sommerfeld_host(int* ex, double* X, double* Y, double* Z, const doubl
   return 0;
 }
 
+// The interpolating predictor branch (precor != 1, sommerfeld_rout.f90:141-217).
+static int sommer_interp_host(int* ex, double* X, double* Y, double* Z, const double* bb, double dT, double* chi0,
+                              double* Lap0, double* f0, double* f, const double* SoA, int symmetry) {
+  if (!ex || ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "sommerfeld_rout_: shape below 2");
+  if (symmetry < 0 || symmetry > 2) return fail(AMSS_ERR_ARG, "sommerfeld_rout_: Symmetry must be 0, 1 or 2");
+  bool on[6];
+  copy_faces(ex, X, Y, Z, bb, symmetry, on);
+  int faces = 0;
+  for (int q = 0; q < 6; ++q)
+    if (on[q]) faces |= 1 << q;
+  const long n = (long)ex[0] * ex[1] * ex[2];
+  const long nc = (long)ex[0] + ex[1] + ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure(4 * (size_t)n + nc)) return r;
+  double *dchi = g_shim.buf, *dlap = dchi + n, *d0 = dlap + n, *d1 = d0 + n, *dx = d1 + n;
+  const size_t b = n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(dchi, chi0, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dlap, Lap0, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(d0, f0, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(d1, f, b, hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dx, X, ex[0] * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dx + ex[0], Y, ex[1] * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dx + ex[0] + ex[1], Z, ex[2] * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemsetAsync(g_shim.flag, 0, sizeof(int), g_shim.s));
+  Shape sh;
+  for (int a = 0; a < 3; ++a) sh.n[a] = ex[a];
+  sh.sy = ex[0];
+  sh.sz = (long)ex[0] * ex[1];
+  launch_sommer_interp(dx, dx + ex[0], dx + ex[0] + ex[1], dchi, dlap, d0, d1, sh, faces, dT, SoA, symmetry,
+                       g_shim.flag, g_shim.s);
+  HIPCHK(hipGetLastError());
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(f, d1, b, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipMemcpyAsync(&bad, g_shim.flag, sizeof(int), hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  if (bad)
+    return fail(AMSS_ERR_ARG, "sommerfeld_rout_: an interpolation block does not fit in the array (decide3d); "
+                              "those face points are left unchanged");
+  return 0;
+}
+
 void sommerfeld_rout_(int* ex, double* X, double* Y, double* Z, double* xmin, double* ymin, double* zmin,
                       double* xmax, double* ymax, double* zmax, double* dT, double* chi0, double* Lap0, double* f0,
                       double* f, double* SoA, int* Symmetry, int* precor) {
-  (void)dT;
-  (void)chi0;
-  (void)Lap0;
-  (void)SoA;
-  if (*precor != 1) {
-    fail(AMSS_ERR_UNSUPPORTED, "sommerfeld_rout_: only the precor == 1 copy branch is provided");
-    return;
-  }
   const double bb[6] = {*xmin, *ymin, *zmin, *xmax, *ymax, *zmax};
-  sommerfeld_host(ex, X, Y, Z, bb, f0, f, *Symmetry);
+  // the reference's test is precor == CORRECTSTEP (1): anything else interpolates
+  const int r = *precor == 1 ? sommerfeld_host(ex, X, Y, Z, bb, f0, f, *Symmetry)
+                             : sommer_interp_host(ex, X, Y, Z, bb, *dT, chi0, Lap0, f0, f, SoA, *Symmetry);
+  // a void drop-in: make a failure visible to a caller that never reads amss_last_error
+  if (r) std::fprintf(stderr, "libamssbssn: %s\n", g_err.c_str());
 }
 
 // Faces and mirrored layer of sommerfeld_routbam (sommerfeld_rout.f90:255-325):
@@ -601,7 +638,7 @@ int amss_level_copy_list(amss_level* L, int patch, int list, double* const* ptrs
 
 int amss_level_step(amss_level* L, double dT, int nsteps, int mode, int want_ham) {
   if (!L) return fail(AMSS_ERR_ARG, "amss_level_step: null level");
-  const int r = L->L->step(dT, nsteps, mode, want_ham != 0);
+  const int r = L->L->step(dT, nsteps, mode, want_ham);
   if (r) return fail(r, L->L->err());
   return 0;
 }
@@ -747,7 +784,24 @@ int amss_level_checkpoint_read(amss_level* L, const char* path, double* time, in
 
 int amss_level_ham_l2(amss_level* L, double* out) {
   if (!L || !out) return fail(AMSS_ERR_ARG, "amss_level_ham_l2: null argument");
-  return L->L->ham_l2(out);
+  const int r = L->L->ham_l2(out);
+  if (r) return fail(r, L->L->err());
+  return 0;
+}
+
+int amss_level_constraint_norms(amss_level* L, int patch, double* out7) {
+  if (!L || !out7) return fail(AMSS_ERR_ARG, "amss_level_constraint_norms: null argument");
+  const int r = L->L->constraint_norms(patch, out7);
+  if (r) return fail(r, L->L->err());
+  return 0;
+}
+
+int amss_level_constraint_series(amss_level* L, int patch, double* out, int maxsteps, int* nsteps) {
+  if (!L || !nsteps || (maxsteps > 0 && !out) || maxsteps < 0)
+    return fail(AMSS_ERR_ARG, "amss_level_constraint_series: bad argument");
+  const int r = L->L->constraint_series(patch, out, maxsteps, nsteps);
+  if (r) return fail(r, L->L->err());
+  return 0;
 }
 
 void* amss_level_stream(amss_level* L) { return L ? (void*)L->L->stream() : nullptr; }

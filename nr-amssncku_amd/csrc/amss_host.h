@@ -86,8 +86,16 @@ class BssnLevel {
   const std::string& err() const { return err_; }
 
   double* field(int patch, int list, int v);
-  int step(double dT, int nsteps, int mode, bool want_ham);
+  // want_res: 0 none; 1 residuals in the last step's predictor; 2 in every
+  // predictor (the reference's co = 0), with the seven constraint norms of
+  // every patch recorded per step
+  int step(double dT, int nsteps, int mode, int want_res);
   int ham_l2(double* out);
+  // the seven constraint L2 norms (Ham, Px, Py, Pz, Gx, Gy, Gz) of one patch,
+  // or of all patches together (patch -1), from the last residuals
+  int constraint_norms(int patch, double* out7);
+  // the per-step norms recorded by the last step() with want_res 2
+  int constraint_series(int patch, double* out, int maxsteps, int* nsteps);
   hipStream_t stream() const { return stream_; }
   // HIP-event timing of the fused RHS+RK4 interior launches (per stage)
   int set_timing(bool on);
@@ -115,7 +123,9 @@ class BssnLevel {
  private:
   int Step(double dT, bool want_res);  // one RK4 step, one launch per patch and sub-step
   int StepBatched(double dT, bool want_res);
-  int StepGraph(double dT);
+  int StepGraph(double dT, bool want_res);
+  void build_l2_jobs();
+  int norms_of(const double* sums, int patch, double* out7) const;
   void rotate_lists();  // the end-of-step list rotation (bssn_class.C:3845-3859)
   int slot_of(MyList<var>* l) const;
   int time_event_begin(int stage, Ev** ev);
@@ -138,9 +148,18 @@ class BssnLevel {
   double eps_, chitiny_;
   hipStream_t stream_ = nullptr;
   int* dflag_ = nullptr;
-  double* dpart_ = nullptr;
-  int nparts_ = 4096;
   bool ham_valid_ = false;
+  // constraint norms: one L2 job per (patch, residual), their partials, the
+  // last evaluation and the per-step series (device, then host copy)
+  std::vector<L2Job> hl2_;
+  L2Job* dl2_ = nullptr;
+  double* dl2part_ = nullptr;
+  int l2maxparts_ = 1;
+  double* dl2last_ = nullptr;
+  double* dseries_ = nullptr;
+  int series_cap_ = 0;
+  std::vector<double> series_;
+  int series_steps_ = 0;
   bool timing_ = false;
   int variant_ = 1;
   std::vector<Ev> ev_;       // event pairs of the current step() call
@@ -164,7 +183,7 @@ class BssnLevel {
   BatchTune tune_{0, 0, 1};
   bool tuned_ = false;
   bool capturing_ = false;
-  struct Graph { int key; double dT; hipGraphExec_t exec; };
+  struct Graph { int key; double dT; bool res; hipGraphExec_t exec; };
   std::vector<Graph> graphs_;
   double ms_[4] = {0, 0, 0, 0};
   long nl_[4] = {0, 0, 0, 0};

@@ -248,7 +248,6 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
   }
   LCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   LCHK(hipMalloc(&dflag_, sizeof(int)));
-  LCHK(hipMalloc(&dpart_, nparts_ * sizeof(double)));
   build_ghost_segments();
   int dev = 0;
   hipDeviceProp_t prop;
@@ -256,6 +255,7 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
       prop.multiProcessorCount > 0)
     ncu_ = prop.multiProcessorCount;
   build_batch_tables();
+  build_l2_jobs();
 }
 
 BssnLevel::~BssnLevel() {
@@ -277,12 +277,13 @@ BssnLevel::~BssnLevel() {
   }
   clear_graphs();
   if (dflag_) (void)hipFree(dflag_);
-  if (dpart_) (void)hipFree(dpart_);
   if (dpd_) (void)hipFree(dpd_);
   for (TileRef* t : dtiles_)
     if (t) (void)hipFree(t);
   if (dgbox_) (void)hipFree(dgbox_);
   if (dgwork_) (void)hipFree(dgwork_);
+  for (void* p : {(void*)dl2_, (void*)dl2part_, (void*)dl2last_, (void*)dseries_})
+    if (p) (void)hipFree(p);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -696,11 +697,11 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
 // One batched step replayed from a hipGraph.  The list slots return to the
 // same assignment every 3 steps, so at most 3 graphs (keyed by the slot
 // assignment at the start of the step, and dT) cover a run.
-int BssnLevel::StepGraph(double dT) {
+int BssnLevel::StepGraph(double dT, bool want_res) {
   const int key = (((slot_of(StateList) * 5 + slot_of(OldStateList)) * 5 + slot_of(SynchList_pre)) * 5 +
                    slot_of(SynchList_cor)) * 5 + slot_of(RHSList);
   for (const Graph& g : graphs_)
-    if (g.key == key && g.dT == dT) {
+    if (g.key == key && g.dT == dT && g.res == want_res) {
       RCHK(hipGraphLaunch(g.exec, stream_));
       // the pre/cor swaps of sub-steps 1 and 2 cancel; only the rotation is left
       rotate_lists();
@@ -709,7 +710,7 @@ int BssnLevel::StepGraph(double dT) {
   hipGraph_t graph = nullptr;
   RCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   capturing_ = true;
-  const int r = StepBatched(dT, false);
+  const int r = StepBatched(dT, want_res);
   capturing_ = false;
   const hipError_t e = hipStreamEndCapture(stream_, &graph);
   if (r) {
@@ -720,7 +721,7 @@ int BssnLevel::StepGraph(double dT) {
     err_ = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
     return AMSS_ERR_HIP;
   }
-  Graph g{key, dT, nullptr};
+  Graph g{key, dT, want_res, nullptr};
   const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) {
@@ -811,7 +812,7 @@ int BssnLevel::autotune(double* ms, int nms) {
   return 0;
 }
 
-int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
+int BssnLevel::step(double dT, int nsteps, int mode, int want_res) {
   if (mode < 0 || mode > 2) {
     err_ = "mode must be 0 (per patch), 1 (batched) or 2 (batched, hipGraph)";
     return AMSS_ERR_ARG;
@@ -820,23 +821,45 @@ int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
     err_ = "negative step count";
     return AMSS_ERR_ARG;
   }
+  if (want_res < 0 || want_res > 2) {
+    err_ = "want_ham must be 0, 1 (last step) or 2 (every step)";
+    return AMSS_ERR_ARG;
+  }
   if (mode > 0 && !tuned_)
     if (int r = autotune(nullptr, 0)) return r;
+  const int njobs = (int)hl2_.size();
+  if (want_res == 2 && nsteps > series_cap_) {
+    RCHK(hipStreamSynchronize(stream_));
+    if (dseries_) RCHK(hipFree(dseries_));
+    dseries_ = nullptr;
+    series_cap_ = 0;
+    RCHK(hipMalloc(&dseries_, (size_t)nsteps * njobs * sizeof(double)));
+    series_cap_ = nsteps;
+  }
   RCHK(hipMemsetAsync(dflag_, 0, sizeof(int), stream_));
   ev_used_ = 0;
   for (int n = 0; n < nsteps; ++n) {
-    const bool res = want_ham && n == nsteps - 1;
+    const bool res = want_res == 2 || (want_res == 1 && n == nsteps - 1);
     int r = 0;
     if (mode == 0)
       r = Step(dT, res);
-    else if (mode == 1 || res)
+    else if (mode == 1)
       r = StepBatched(dT, res);
     else
-      r = StepGraph(dT);
+      r = StepGraph(dT, res);
     if (r) return r;
+    // the predictor's residuals of this step -> its norms (Constraint_Out)
+    if (want_res == 2) launch_l2_jobs(dl2_, njobs, l2maxparts_, dl2part_, dseries_ + (size_t)n * njobs, stream_);
   }
   int flag = 0;
   RCHK(hipMemcpyAsync(&flag, dflag_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  if (want_res == 2) {
+    series_.assign((size_t)nsteps * njobs, 0.0);
+    if (nsteps > 0)
+      RCHK(hipMemcpyAsync(series_.data(), dseries_, series_.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+    series_steps_ = nsteps;
+  }
   RCHK(hipStreamSynchronize(stream_));
   for (size_t i = 0; i < ev_used_; ++i) {
     float ms = 0.f;
@@ -845,7 +868,11 @@ int BssnLevel::step(double dT, int nsteps, int mode, bool want_ham) {
     nl_[ev_[i].stage] += 1;
   }
   ev_used_ = 0;
-  if (want_ham) ham_valid_ = true;
+  if (want_res && nsteps > 0) ham_valid_ = true;
+  if (hipGetLastError() != hipSuccess) {
+    err_ = "kernel launch failed";
+    return AMSS_ERR_HIP;
+  }
   if (flag) {
     err_ = "NaN found in a right-hand-side input";
     return AMSS_ERR_NONFINITE;
@@ -879,30 +906,88 @@ int BssnLevel::kernel_ms(double* ms4, long* n4) {
   return 0;
 }
 
-int BssnLevel::ham_l2(double* out) {
+// One L2 job per (patch, residual) in Constraint_Out's order (Ham, Px, Py,
+// Pz, Gx, Gy, Gz; bssn_class.C:7475-7481) over the points l2normhelper sums
+// (fmisc.f90:1283-1301): the ghost layers of a Block count where the Block
+// reaches its patch's bbox, which a level's Block always does, so under the
+// reference's boundary modes every point counts.  With frozen ghost zones
+// (whose residuals are never evaluated) only [gw, n - gw) counts.
+void BssnLevel::build_l2_jobs() {
+  hl2_.clear();
+  int part = 0;
+  l2maxparts_ = 1;
+  for (Block* b : blocks_) {
+    const int w = bc_ == AMSS_BC_FROZEN_GHOSTS ? gw_ : 0;
+    L2Job J;
+    long N = 1;
+    for (int a = 0; a < 3; ++a) {
+      J.lo[a] = w;
+      J.hi[a] = std::max(w, b->shape[a] - w);
+      N *= J.hi[a] - J.lo[a];
+    }
+    J.sy = b->shape[0];
+    J.sz = (long)b->shape[0] * b->shape[1];
+    J.dv = b->getdX(0) * b->getdX(1) * b->getdX(2);
+    J.nparts = (int)std::min<long>(512, std::max<long>(1, (N + 4095) / 4096));
+    l2maxparts_ = std::max(l2maxparts_, J.nparts);
+    MyList<var>* c = ConsList;
+    for (int r = 0; r < 7; ++r, c = c->next) {
+      J.f = b->fgfs[c->data->sgfn];
+      J.part0 = part;
+      part += J.nparts;
+      hl2_.push_back(J);
+    }
+  }
+  LCHK(hipMalloc(&dl2_, hl2_.size() * sizeof(L2Job)));
+  LCHK(hipMemcpy(dl2_, hl2_.data(), hl2_.size() * sizeof(L2Job), hipMemcpyHostToDevice));
+  LCHK(hipMalloc(&dl2part_, (size_t)std::max(part, 1) * sizeof(double)));
+  LCHK(hipMalloc(&dl2last_, hl2_.size() * sizeof(double)));
+}
+
+// sqrt of the per-patch sums (one patch, or all patches added: -1)
+int BssnLevel::norms_of(const double* sums, int patch, double* out7) const {
+  const int np = (int)blocks_.size();
+  if (patch < -1 || patch >= np) return AMSS_ERR_ARG;
+  for (int r = 0; r < 7; ++r) {
+    double t = 0.0;
+    for (int p = (patch < 0 ? 0 : patch); p < (patch < 0 ? np : patch + 1); ++p) t += sums[p * 7 + r];
+    out7[r] = std::sqrt(t);
+  }
+  return 0;
+}
+
+int BssnLevel::constraint_norms(int patch, double* out7) {
   if (!ham_valid_) {
-    err_ = "no Hamiltonian residual computed yet (step with want_ham)";
+    err_ = "no constraint residuals computed yet (step with want_ham)";
     return AMSS_ERR_ARG;
   }
-  double s = 0.0;
-  for (Block* b : blocks_) {
-    Box box;
-    for (int a = 0; a < 3; ++a) {
-      box.n[a] = b->shape[a];
-      box.lo[a] = gw_;
-      box.hi[a] = b->shape[a] - gw_;
-    }
-    box.sy = b->shape[0];
-    box.sz = (long)b->shape[0] * b->shape[1];
-    const int nb = launch_sumsq(b->fgfs[ConsList->data->sgfn], box, dpart_, nparts_, stream_);
-    std::vector<double> part(nb);
-    RCHK(hipMemcpyAsync(part.data(), dpart_, nb * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    RCHK(hipStreamSynchronize(stream_));
-    double t = 0.0;
-    for (double x : part) t += x;
-    s += t * b->getdX(0) * b->getdX(1) * b->getdX(2);
+  if (patch < -1 || patch >= (int)blocks_.size()) {
+    err_ = "patch index out of range";
+    return AMSS_ERR_ARG;
   }
-  *out = std::sqrt(s);
+  std::vector<double> sums(hl2_.size());
+  launch_l2_jobs(dl2_, (int)hl2_.size(), l2maxparts_, dl2part_, dl2last_, stream_);
+  RCHK(hipMemcpyAsync(sums.data(), dl2last_, sums.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  RCHK(hipStreamSynchronize(stream_));
+  return norms_of(sums.data(), patch, out7);
+}
+
+int BssnLevel::constraint_series(int patch, double* out, int maxsteps, int* nsteps) {
+  if (patch < -1 || patch >= (int)blocks_.size()) {
+    err_ = "patch index out of range";
+    return AMSS_ERR_ARG;
+  }
+  const int n = std::min(maxsteps, series_steps_);
+  const size_t nj = hl2_.size();
+  for (int s = 0; s < n; ++s) norms_of(series_.data() + s * nj, patch, out + 7 * s);
+  *nsteps = series_steps_;
+  return 0;
+}
+
+int BssnLevel::ham_l2(double* out) {
+  double v[7];
+  if (int r = constraint_norms(-1, v)) return r;
+  *out = v[0];
   return 0;
 }
 

@@ -64,6 +64,13 @@ void launch_rk4_g2(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc,
 // (parity S per axis for the mirrored layer), velocity vel.
 void launch_sommer_bam(const double* f0, double* f_rhs, const Shape& n, const SommerGeom& sg, double vel,
                        const double S[3], hipStream_t s);
+// sommerfeld_rout(precor = 0) on one field: f on the outermost layer of the
+// faces in the mask (bits as SommerGeom::faces) from f0, the base state's
+// chi0 and lap0 and the coordinate arrays X, Y, Z (device); *bad is set when
+// a point's interpolation block cannot be placed inside the array.
+void launch_sommer_interp(const double* X, const double* Y, const double* Z, const double* chi0, const double* lap0,
+                          const double* f0, double* f, const Shape& n, int faces, double dT, const double S[3],
+                          int symmetry, int* bad, hipStream_t s);
 void launch_ghost_fill(const FieldsIn& u0, const FieldsOut& out, const Box& box, const Faces& w, int stage,
                        double tinny, int* flag, hipStream_t s);
 void launch_seg_copy(const FieldsIn& src, const FieldsOut& dst, const SegDesc& d, hipStream_t s);
@@ -72,5 +79,19 @@ void launch_lowerbound(double* chi, long n, double tinny, hipStream_t s);
 void launch_rk4_update(const double* f0, double* f1, double* frhs, long n, double dT, int stage, hipStream_t s);
 void launch_copy_box(const double* f0, double* f, const Box& box, hipStream_t s);
 int launch_sumsq(const double* f, const Box& box, double* part, int maxblocks, hipStream_t s);
+
+// One L2 reduction: dv times the sum of squares of f over the index box
+// [lo, hi) (strides sy, sz), reduced through nparts workgroup partials at
+// part[part0 ..] and then summed in a fixed order (deterministic results).
+struct L2Job {
+  const double* f;
+  int lo[3], hi[3];
+  long sy, sz;
+  double dv;
+  int part0, nparts;
+};
+// out[j] = job j's sum for every job (jobs: device table of njobs; maxparts:
+// the largest nparts).  Stream-ordered, no synchronisation.
+void launch_l2_jobs(const L2Job* jobs, int njobs, int maxparts, double* part, double* out, hipStream_t s);
 
 }  // namespace amss

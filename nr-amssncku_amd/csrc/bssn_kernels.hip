@@ -149,6 +149,39 @@ __global__ void __launch_bounds__(256) sumsq_kernel(const double* __restrict__ f
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Partial sums of squares of the L2 jobs: workgroup x of job y covers every
+// (nparts * 256)-th point of the job's box, starting at x * 256 + lane.
+__global__ void __launch_bounds__(256) l2_parts_kernel(const L2Job* __restrict__ jobs, double* __restrict__ part) {
+  const L2Job& J = jobs[blockIdx.y];
+  if ((int)blockIdx.x >= J.nparts) return;  // the whole workgroup
+  __shared__ double red[256];
+  const int nx = J.hi[0] - J.lo[0], ny = J.hi[1] - J.lo[1], nz = J.hi[2] - J.lo[2];
+  const long N = (long)nx * ny * nz;
+  double s = 0.0;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < N; t += (long)J.nparts * 256) {
+    const int i = J.lo[0] + (int)(t % nx), j = J.lo[1] + (int)((t / nx) % ny), k = J.lo[2] + (int)(t / ((long)nx * ny));
+    const double v = J.f[i + j * J.sy + k * J.sz];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[J.part0 + blockIdx.x] = red[0];
+}
+
+__global__ void l2_final_kernel(const L2Job* __restrict__ jobs, int njobs, const double* __restrict__ part,
+                                double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  const L2Job& J = jobs[j];
+  double s = 0.0;
+  for (int b = 0; b < J.nparts; ++b) s += part[J.part0 + b];
+  out[j] = s * J.dv;
+}
+
 // --------------------------------------------------------------------------
 // Host-side launchers (stream-ordered, no synchronisation).
 
@@ -187,6 +220,131 @@ __global__ void __launch_bounds__(256) sommer_bam_kernel(const double* __restric
     return f0[q[0] + q[1] * n.sy + q[2] * n.sz] * sgn;
   };
   frhs[t] = sommer_rhs(sg, n.n, p, vel, rd);
+}
+
+// Neville's tableau over the samples y[0..5] at nodes 0..5, evaluated at x
+// (polint, fmisc.f90:1121-1179): start from the sample nearest x, then add
+// the corrections c or d that keep the path centred on it.
+__device__ __forceinline__ double neville6(const double y[6], double x) {
+  double c[6], d[6], ho[6];
+  int ns = 0;
+  double dif = fabs(x);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    c[m] = d[m] = y[m];
+    ho[m] = (double)m - x;
+    const double t = fabs(x - (double)m);
+    if (t < dif) {
+      ns = m;
+      dif = t;
+    }
+  }
+  double v = y[ns];
+  // ns counts the tableau entries left of the path (1-based index minus one)
+#pragma unroll
+  for (int m = 1; m < 6; ++m) {
+#pragma unroll
+    for (int i = 0; i < 6 - m; ++i) {
+      const double w = (c[i + 1] - d[i]) / (ho[i] - ho[i + m]);
+      d[i] = ho[i + m] * w;
+      c[i] = ho[i] * w;
+    }
+    double dy;
+    if (2 * ns < 6 - m) {
+      dy = c[ns];
+    } else {
+      dy = d[ns - 1];
+      ns -= 1;
+    }
+    v = v + dy;
+  }
+  return v;
+}
+
+// sommerfeld_rout with precor == 0 (sommerfeld_rout.f90:141-217) on one
+// field: each point of the face layers takes the outgoing wave
+// (1 - r) f0(x - r x), r = alpha sqrt(chi) dT / |x|, f0 interpolated over the
+// 6^3 points around x - r x (shifted inside the array where it would leave
+// it, mirrored with the parity S below a symmetry plane: decide3d,
+// fmisc.f90:798-874), Neville along z, then y, then x (polin3).  A thread per
+// array point; a point whose block would still leave the array is left alone
+// and raises *bad.
+__global__ void __launch_bounds__(256) sommer_interp_kernel(const double* __restrict__ X, const double* __restrict__ Y,
+                                                            const double* __restrict__ Z,
+                                                            const double* __restrict__ chi0,
+                                                            const double* __restrict__ lap0,
+                                                            const double* __restrict__ f0, double* __restrict__ f,
+                                                            Shape n, int faces, double dT, double S0, double S1,
+                                                            double S2, int symmetry, int* __restrict__ bad) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long N = (long)n.n[0] * n.n[1] * n.n[2];
+  if (t >= N) return;
+  const int p[3] = {(int)(t % n.n[0]), (int)((t / n.n[0]) % n.n[1]), (int)(t / ((long)n.n[0] * n.n[1]))};
+  if (!on_faces(faces, n.n, p[0], p[1], p[2])) return;
+  const double xc[3] = {X[p[0]], Y[p[1]], Z[p[2]]};
+  const double h[3] = {X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0]};
+  const double r = (lap0[t] + 1.0) * sqrt(1.0 + chi0[t]) * dT / sqrt(xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
+  // lo[a]: first array index (0-based) of the 6-point block; u[a]: the
+  // target's position inside it (block index units)
+  int lo[3];
+  double u[3];
+  for (int a = 0; a < 3; ++a) {
+    const double cx = r * xc[a] / h[a];
+    const double ti = trunc(cx);
+    lo[a] = p[a] - (int)ti - (cx > 0.0 ? 3 : 2);
+    u[a] = ti - cx + (cx > 0.0 ? 3.0 : 2.0);
+    // a block reaching below the array moves up unless the axis mirrors there
+    const bool mirrors = a == 2 ? symmetry != 0 : symmetry == 2;
+    if (!mirrors && lo[a] < 0) {
+      u[a] += lo[a];
+      lo[a] = 0;
+    }
+  }
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] + 5 > n.n[a] - 1) {
+      const int sh = lo[a] + 5 - (n.n[a] - 1);
+      u[a] += sh;
+      lo[a] -= sh;
+    }
+  for (int a = 0; a < 3; ++a)  // decide3d's range checks (mirrored reads included)
+    if (lo[a] + 5 > n.n[a] - 1 || -lo[a] - 1 > n.n[a] - 1) {
+      *bad = 1;
+      return;
+    }
+  const double S[3] = {S0, S1, S2};
+  double ym[6];
+#pragma unroll 1
+  for (int a = 0; a < 6; ++a) {
+    double yz[6];
+#pragma unroll 1
+    for (int b = 0; b < 6; ++b) {
+      double line[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        int q[3] = {lo[0] + a, lo[1] + b, lo[2] + c};
+        double sgn = 1.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          if (q[m] < 0) {
+            q[m] = -q[m] - 1;
+            sgn *= S[m];
+          }
+        line[c] = f0[q[0] + q[1] * n.sy + q[2] * n.sz] * sgn;
+      }
+      yz[b] = neville6(line, u[2]);
+    }
+    ym[a] = neville6(yz, u[1]);
+  }
+  f[t] = neville6(ym, u[0]) * (1.0 - r);
+}
+
+void launch_sommer_interp(const double* X, const double* Y, const double* Z, const double* chi0, const double* lap0,
+                          const double* f0, double* f, const Shape& n, int faces, double dT, const double S[3],
+                          int symmetry, int* bad, hipStream_t s) {
+  const long N = (long)n.n[0] * n.n[1] * n.n[2];
+  if (N > 0 && faces != 0)
+    sommer_interp_kernel<<<blocks_for(N, 256), 256, 0, s>>>(X, Y, Z, chi0, lap0, f0, f, n, faces, dT, S[0], S[1],
+                                                            S[2], symmetry, bad);
 }
 
 void launch_sommer_bam(const double* f0, double* f_rhs, const Shape& n, const SommerGeom& sg, double vel,
@@ -232,6 +390,12 @@ int launch_sumsq(const double* f, const Box& box, double* part, int maxblocks, h
   if (nb < 1) nb = 1;
   sumsq_kernel<<<nb, 256, 0, s>>>(f, box, part);
   return nb;
+}
+
+void launch_l2_jobs(const L2Job* jobs, int njobs, int maxparts, double* part, double* out, hipStream_t s) {
+  if (njobs <= 0) return;
+  l2_parts_kernel<<<dim3((unsigned)maxparts, (unsigned)njobs), 256, 0, s>>>(jobs, part);
+  l2_final_kernel<<<(njobs + 63) / 64, 64, 0, s>>>(jobs, njobs, part, out);
 }
 
 }  // namespace amss

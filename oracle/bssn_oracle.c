@@ -938,6 +938,131 @@ void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const 
     }
 }
 
+/* polint (fmisc.f90:1121-1179): Neville's tableau on xa(1..n), ya(1..n)
+ * (1-based arrays of n + 1 entries), starting from the sample nearest x. */
+static double neville(const double *xa, const double *ya, double x, int n)
+{
+    double c[8], d[8], den[8], ho[8];
+    for (int i = 1; i <= n; i++) { c[i] = ya[i]; d[i] = ya[i]; ho[i] = xa[i] - x; }
+    int ns = 1;
+    double dif = fabs(x - xa[1]);
+    for (int m = 1; m <= n; m++) {
+        const double dift = fabs(x - xa[m]);
+        if (dift < dif) { ns = m; dif = dift; }
+    }
+    double y = ya[ns], dy;
+    ns = ns - 1;
+    for (int m = 1; m <= n - 1; m++) {
+        for (int i = 1; i <= n - m; i++) den[i] = ho[i] - ho[i + m];
+        for (int i = 1; i <= n - m; i++) den[i] = (c[i + 1] - d[i]) / den[i];
+        for (int i = 1; i <= n - m; i++) {
+            d[i] = ho[i + m] * den[i];
+            c[i] = ho[i] * den[i];
+        }
+        if (2 * ns < n - m) dy = c[ns + 1];
+        else { dy = d[ns]; ns = ns - 1; }
+        y = y + dy;
+    }
+    return y;
+}
+
+/* sommerfeld_rout.f90:141-217 at one face point (1-based i, j, k as in the
+ * Fortran); returns 0, or 1 when decide3d rejects the block. */
+static int sinterp_point(const int *ex, const double *X, const double *Y, const double *Z, double dX, double dY,
+                         double dZ, double dT, const double *chi0, const double *Lap0, const double *f0, double *f,
+                         const double *SoA, int symmetry, int i, int j, int k)
+{
+    enum { ORDN = 6 };
+    const size_t p = IX(ex, i - 1, j - 1, k - 1);
+    double r = (Lap0[p] + 1.0) * sqrt(1.0 + chi0[p]) * dT /
+               sqrt(X[i - 1] * X[i - 1] + Y[j - 1] * Y[j - 1] + Z[k - 1] * Z[k - 1]);
+    const double fac = 1.0 - r;
+    double cx[3] = {r * X[i - 1] / dX, r * Y[j - 1] / dY, r * Z[k - 1] / dZ};
+    const int idx[3] = {i, j, k};
+    int cxB[3], cxT[3];
+    for (int m = 0; m < 3; m++) {
+        if (cx[m] > 0.0) cxB[m] = (int)((double)idx[m] - trunc(cx[m]) - ORDN / 2);
+        else cxB[m] = (int)((double)idx[m] - trunc(cx[m]) - ORDN / 2 + 1);
+    }
+    for (int m = 0; m < 3; m++) {
+        if (cx[m] > 0.0) cx[m] = trunc(cx[m]) - cx[m] + ORDN / 2;
+        else cx[m] = trunc(cx[m]) - cx[m] + ORDN / 2 - 1;
+        cxT[m] = cxB[m] + ORDN - 1;
+    }
+    if (symmetry == 0 && cxB[2] < 1) { cx[2] = cx[2] + (cxB[2] - 1); cxT[2] = cxT[2] - (cxB[2] - 1); cxB[2] = 1; }
+    if (symmetry < 2 && cxB[1] < 1) { cx[1] = cx[1] + (cxB[1] - 1); cxT[1] = cxT[1] - (cxB[1] - 1); cxB[1] = 1; }
+    if (symmetry < 2 && cxB[0] < 1) { cx[0] = cx[0] + (cxB[0] - 1); cxT[0] = cxT[0] - (cxB[0] - 1); cxB[0] = 1; }
+    for (int m = 0; m < 3; m++)
+        if (cxT[m] > ex[m]) {
+            cx[m] = cx[m] + (cxT[m] - ex[m]);
+            cxB[m] = cxB[m] - (cxT[m] - ex[m]);
+            cxT[m] = ex[m];
+        }
+    /* decide3d: indices >= 1 read f0, indices <= 0 read f0(1 - i) * SoA */
+    for (int m = 0; m < 3; m++) {
+        const int fmin1 = cxB[m] > 1 ? cxB[m] : 1, fmax1 = cxT[m];
+        const int fmin2 = cxB[m], fmax2 = cxT[m] < 0 ? cxT[m] : 0;
+        if (fmin1 <= fmax1 && (fmin1 < 1 || fmax1 > ex[m])) return 1;
+        if (fmin2 <= fmax2 && (1 - fmax2 < 1 || 1 - fmin2 > ex[m])) return 1;
+    }
+    double ya[ORDN][ORDN][ORDN]; /* [x][y][z], local index = global - cxB */
+    for (int c = 0; c < ORDN; c++)
+        for (int b = 0; b < ORDN; b++)
+            for (int a = 0; a < ORDN; a++) {
+                const int g[3] = {cxB[0] + a, cxB[1] + b, cxB[2] + c};
+                double s = 1.0;
+                int q[3];
+                for (int m = 0; m < 3; m++) {
+                    if (g[m] >= 1) q[m] = g[m];
+                    else { q[m] = 1 - g[m]; s *= SoA[m]; }
+                }
+                const double v = f0[IX(ex, q[0] - 1, q[1] - 1, q[2] - 1)];
+                ya[a][b][c] = (s == 1.0) ? v : v * s;
+            }
+    /* polin3 (fmisc.f90:1221-1260): z, then y, then x */
+    double xa[ORDN + 1], line[ORDN + 1], ym[ORDN + 1], yat[ORDN];
+    for (int a = 1; a <= ORDN; a++) xa[a] = (double)(a - 1);
+    for (int a = 0; a < ORDN; a++) {
+        for (int b = 0; b < ORDN; b++) {
+            for (int c = 0; c < ORDN; c++) line[c + 1] = ya[a][b][c];
+            yat[b] = neville(xa, line, cx[2], ORDN);
+        }
+        for (int b = 0; b < ORDN; b++) line[b + 1] = yat[b];
+        ym[a + 1] = neville(xa, line, cx[1], ORDN);
+    }
+    r = neville(xa, ym, cx[0], ORDN);
+    f[p] = r * fac;
+    return 0;
+}
+
+int orc_sommerfeld_interp(const int *ex, const double *X, const double *Y, const double *Z, const double *bbox,
+                          double dT, const double *chi0, const double *Lap0, const double *f0, double *f,
+                          const double *SoA, int symmetry)
+{
+    const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+    /* the six faces' layers, 1-based inclusive (sommerfeld_rout.f90:58-119) */
+    int lay[6][6], on[6] = {0, 0, 0, 0, 0, 0};
+    const int n0 = ex[0], n1 = ex[1], n2 = ex[2];
+    if (fabs(X[n0 - 1] - bbox[3]) < dX) { on[0] = 1; int t[6] = {n0, 1, 1, n0, n1, n2}; memcpy(lay[0], t, sizeof t); }
+    if (fabs(Y[n1 - 1] - bbox[4]) < dY) { on[1] = 1; int t[6] = {1, n1, 1, n0, n1, n2}; memcpy(lay[1], t, sizeof t); }
+    if (fabs(Z[n2 - 1] - bbox[5]) < dZ) { on[2] = 1; int t[6] = {1, 1, n2, n0, n1, n2}; memcpy(lay[2], t, sizeof t); }
+    if (fabs(X[0] - bbox[0]) < dX && !(symmetry == 2 && fabs(bbox[0]) < dX / 2)) {
+        on[3] = 1; int t[6] = {1, 1, 1, 1, n1, n2}; memcpy(lay[3], t, sizeof t); }
+    if (fabs(Y[0] - bbox[1]) < dY && !(symmetry == 2 && fabs(bbox[1]) < dY / 2)) {
+        on[4] = 1; int t[6] = {1, 1, 1, n0, 1, n2}; memcpy(lay[4], t, sizeof t); }
+    if (fabs(Z[0] - bbox[2]) < dZ && !(symmetry > 0 && fabs(bbox[2]) < dZ / 2)) {
+        on[5] = 1; int t[6] = {1, 1, 1, n0, n1, 1}; memcpy(lay[5], t, sizeof t); }
+    int bad = 0;
+    for (int gp = 0; gp < 6; gp++) {
+        if (!on[gp]) continue;
+        for (int k = lay[gp][2]; k <= lay[gp][5]; k++)
+            for (int j = lay[gp][1]; j <= lay[gp][4]; j++)
+                for (int i = lay[gp][0]; i <= lay[gp][3]; i++)
+                    bad += sinterp_point(ex, X, Y, Z, dX, dY, dZ, dT, chi0, Lap0, f0, f, SoA, symmetry, i, j, k);
+    }
+    return bad;
+}
+
 /* One axis of sommerfeld_routbam's derivative (sommerfeld_rout.f90:383-436):
  * second-order differences taken from the side the outgoing wave arrives
  * from (w = v x_a / r > 0: backward), centred or reversed where the array

@@ -62,6 +62,17 @@ double orc_l2normhelper(const int *ex, const double *X, const double *Y, const d
 /* sommerfeld_rout.f90:9-221, precor == 1 (copy) branch */
 void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const double *Z,
                          const double *bbox, const double *f0, double *f, int symmetry);
+/* sommerfeld_rout.f90:9-121, 141-217: precor == 0 (the interpolating
+ * predictor): on the outermost layer of each physical face of bbox,
+ * f = (1 - r) f0(x - r x), r = alpha sqrt(chi) dT / |x| (alpha, chi from
+ * Lap0, chi0), f0 interpolated by 6-point Neville polynomials (polin3,
+ * fmisc.f90:1121-1260) over a 6^3 block (decide3d, fmisc.f90:798-874:
+ * mirrored with SoA below index 1).  Returns the number of points whose
+ * block decide3d rejects (left unchanged; the reference prints an error and
+ * interpolates undefined data there). */
+int orc_sommerfeld_interp(const int *ex, const double *X, const double *Y, const double *Z, const double *bbox,
+                          double dT, const double *chi0, const double *Lap0, const double *f0, double *f,
+                          const double *SoA, int symmetry);
 /* sommerfeld_rout.f90:223-449 (sommerfeld_routbam): on the outermost layer
  * of each physical face of bbox, f_rhs = -velocity (x.grad f0 + f0) / r. */
 void orc_sommerfeld_routbam(const int *ex, const double *X, const double *Y, const double *Z,

@@ -52,7 +52,7 @@ class SlabStepper:
     def close(self):
         self.pool.shutdown()
 
-    def _rhs(self, src, dst, co):
+    def _rhs(self, src, dst, co, res=None):
         def job(ab):
             k0, k1 = ab
             s0, s1 = k0 - 3, k1 + 3
@@ -61,8 +61,26 @@ class SlabStepper:
                                     gauge=self.gauge)
             for v in range(NF):
                 dst[v][k0:k1] = o["rhs"][v][3:3 + (k1 - k0)]
+            if res is not None:
+                for r in range(7):
+                    res[r][k0:k1] = o["res"][r][3:3 + (k1 - k0)]
             return o["gont"]
         return max(self.pool.map(job, self.owned))
+
+    def residual_norms(self, U):
+        """The 7 constraint L2 norms (Ham, Px..Pz, Gx..Gz) of the predictor of
+        a step starting from U (residuals of the enforced U, co = 0) over the
+        owned points [gw, n - gw)^3 (frozen ghost zones)."""
+        shape = U[0].shape
+        V = [u.copy() for u in U]
+        self._planes(lambda ab: _enforce_planes(self.be, V, ab))
+        k = [np.zeros(shape) for _ in range(NF)]
+        res = [np.zeros(shape) for _ in range(7)]
+        self._rhs(V, k, 0, res)
+        gw = self.gw
+        dv = (self.X[1] - self.X[0]) * (self.Y[1] - self.Y[0]) * (self.Z[1] - self.Z[0])
+        s = np.s_[gw:shape[0] - gw, gw:shape[1] - gw, gw:shape[2] - gw]
+        return [float(np.sqrt(np.sum(r[s] ** 2) * dv)) for r in res]
 
     def _planes(self, fn):
         list(self.pool.map(fn, self.planes))
@@ -74,12 +92,13 @@ class SlabStepper:
         cor = [np.zeros(shape) for _ in range(NF)]
         acc = [np.zeros(shape) for _ in range(NF)]
         gw, dt, tinny = self.gw, self.dt, self.tinny
-        interior = (slice(gw, shape[0] - gw), slice(gw, shape[1] - gw), slice(gw, shape[2] - gw))
 
         def enforce(F):
             self._planes(lambda ab: _enforce_planes(self.be, F, ab))
 
         def rk_bc(out, k, stage):
+            # RK4 update, then the frozen-ghost rule (every point within gw of
+            # a face keeps U0) and the chi bound, plane range by plane range
             def job(ab):
                 a, b = ab
                 for v in range(NF):
@@ -91,13 +110,17 @@ class SlabStepper:
                     self.be.rk4(f0, f1, r, dt, stage)
                     if stage > 0:
                         out[v][a:b] = f1
+                    for z in range(a, b):  # the ghost shell of each plane
+                        o, u = out[v][z], U[v][z]
+                        if z < gw or z >= shape[0] - gw:
+                            o[...] = u
+                        else:
+                            o[:gw] = u[:gw]
+                            o[shape[1] - gw:] = u[shape[1] - gw:]
+                            o[:, :gw] = u[:, :gw]
+                            o[:, shape[2] - gw:] = u[:, shape[2] - gw:]
+                self.be.lowerboundset(out[0][a:b], tinny)
             self._planes(job)
-            # frozen ghost zones (every point within gw of a face keeps U0)
-            for v in range(NF):
-                keep = out[v][interior].copy()
-                out[v][...] = U[v]
-                out[v][interior] = keep
-            self.be.lowerboundset(out[0], tinny)
 
         enforce(U)
         k = [np.zeros(shape) for _ in range(NF)]
@@ -120,8 +143,10 @@ def _enforce_planes(be, F, ab):
     be.enforce_ga(views)
 
 
-def time_reference_steps(cfg, nsteps=2, nthreads=None, prefer_reference=True):
-    """Seconds per RK4 step of the CPU baseline on cfg's first patch, and what ran."""
+def time_reference_steps(cfg, nsteps=5, nthreads=None, prefer_reference=True):
+    """Median seconds per RK4 step of the CPU baseline on cfg's first patch
+    (after one warm-up step), what ran, the threads used and every step's
+    time."""
     if prefer_reference and O.have_ref():
         be, kind, gauge = O.Ref("_o3") if os.path.exists(os.path.join(O.HERE, "_ref", "libamssref_o3.so")) \
             else O.Ref(), "reference", None
@@ -131,9 +156,10 @@ def time_reference_steps(cfg, nsteps=2, nthreads=None, prefer_reference=True):
     stp = SlabStepper(be, geom, cfg.dt, cfg.eps, cfg.chitiny, gauge=gauge, nthreads=nthreads, gw=geom.gw)
     U = cfg.initial_state(0)
     stp.step(U)  # warm-up
-    t0 = time.perf_counter()
+    times = []
     for _ in range(nsteps):
+        t0 = time.perf_counter()
         stp.step(U)
-    dt = (time.perf_counter() - t0) / nsteps
+        times.append(time.perf_counter() - t0)
     stp.close()
-    return dt, kind, stp.nthreads
+    return float(np.median(times)), kind, stp.nthreads, times

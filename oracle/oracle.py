@@ -140,6 +140,14 @@ class Orc:
         self.lib.orc_sommerfeld_copy(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, _ptr(f0), _ptr(f),
                                      C.c_int(symmetry))
 
+    def sommerfeld_interp(self, f0, f, chi0, lap0, X, Y, Z, bbox, dT, soa, symmetry=0):
+        """sommerfeld_rout with precor == 0 on f in place; returns the number of
+        face points whose interpolation block decide3d rejects."""
+        bb = (C.c_double * 6)(*bbox)
+        sa = (C.c_double * 3)(*soa)
+        return self.lib.orc_sommerfeld_interp(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, C.c_double(dT),
+                                              _ptr(chi0), _ptr(lap0), _ptr(f0), _ptr(f), sa, C.c_int(symmetry))
+
     def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
         """prolong3 into funf in place; returns the status (0 ok, 1 bad region)."""
         d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
@@ -265,6 +273,14 @@ class Ref:
         self.lib.sommerfeld_rout_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), *[self._d(b) for b in bbox],
                                   self._d(0.0), _ptr(dummy), _ptr(dummy), _ptr(f0), _ptr(f), soa,
                                   self._i(symmetry), self._i(1))
+
+    def sommerfeld_interp(self, f0, f, chi0, lap0, X, Y, Z, bbox, dT, soa, symmetry=0):
+        """sommerfeld_rout with precor == 0 (the interpolating predictor)."""
+        sa = (C.c_double * 3)(*soa)
+        self.lib.sommerfeld_rout_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), *[self._d(b) for b in bbox],
+                                  self._d(dT), _ptr(chi0), _ptr(lap0), _ptr(f0), _ptr(f), sa,
+                                  self._i(symmetry), self._i(0))
+        return 0
 
     def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
         """prolong3_ (a subroutine: no status) into funf in place."""

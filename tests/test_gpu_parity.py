@@ -440,3 +440,24 @@ def test_odd_row_pitch_matches_oracle(orc, variant):
     import amssbssn as A
     cfg = A.config1(n=17)
     _level_vs_oracle(orc, cfg, 0, 0, 2, 1e-12, variant)
+
+
+@pytest.mark.parametrize("dT", [0.05, 0.6])
+def test_sommerfeld_interpolating_predictor_abi_matches_oracle(fa, orc, dT):
+    """sommerfeld_rout_ with precor 0 (SommerType 1's predictor) against the
+    oracle: the same face points written, values to 1e-13 relative (same
+    Neville order; the GPU contracts multiply-adds)."""
+    from test_oracle_reference import _bam_cases, PARS
+    rng = np.random.default_rng(41)
+    for X, Y, Z, bbox, sym in _bam_cases():
+        shape = (len(Z), len(Y), len(X))
+        f0 = rng.standard_normal(shape)
+        chi0 = -0.3 + 0.05 * rng.standard_normal(shape)
+        lap0 = -0.2 + 0.05 * rng.standard_normal(shape)
+        start = rng.standard_normal(shape)
+        for par in PARS[:3]:
+            g, o = start.copy(), start.copy()
+            fa.sommerfeld_interp(f0, g, chi0, lap0, X, Y, Z, bbox, dT, par, sym)
+            assert orc.sommerfeld_interp(f0, o, chi0, lap0, X, Y, Z, bbox, dT, par, sym) == 0
+            assert np.array_equal(g != start, o != start), (sym, par)
+            assert np.max(np.abs(g - o)) <= 1e-13 * np.max(np.abs(o)), (sym, par)

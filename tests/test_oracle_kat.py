@@ -237,3 +237,51 @@ def test_philox_known_answers(orc):
         assert tuple(orc.philox(ctr, key)) == want
         got = G.philox4x32_10(*[np.array([c], np.uint64) for c in ctr], *key)
         assert tuple(int(w[0]) for w in got) == want
+
+
+@pytest.mark.parametrize("sym,soa", [(0, (1, 1, 1)), (1, (1, 1, 1)), (1, (1, 1, -1)), (2, (-1, 1, -1))])
+def test_sommerfeld_interpolating_predictor_exact_on_polynomials(orc, sym, soa):
+    """sommerfeld_rout (precor 0) sets each face point to the outgoing wave
+    (1 - r) f0(x - r x), r = alpha sqrt(chi) dT / |x|; its 6-point
+    interpolation is exact for polynomials of degree <= 5 per axis, also when
+    the block is mirrored across a symmetry plane (f0 with the parity SoA)."""
+    h = (0.3, 0.27, 0.33)
+    n = (11, 12, 13)
+    if sym == 0:
+        X, Y, Z, bbox = grid(*n, h=h, origin=(-1.6, -1.5, -2.0))
+    else:  # the lower faces on the planes (z for sym 1; x, y, z for sym 2)
+        org = (0.0 if sym == 2 else -1.6, 0.0 if sym == 2 else -1.5, 0.0)
+        X, Y, Z, bbox = grid(*n, h=h, origin=org)
+    x, y, z = mesh(X, Y, Z)
+    sx, sy, sz = soa
+
+    def par(t, s):  # degree <= 5, parity s under t -> -t
+        return t ** 3 - t if s == -1 else t ** 4 - 2 * t ** 2 + 0.7
+
+    def P(x, y, z):  # degree <= 5 per axis, parity SoA on the mirrored axes
+        if sym == 0:
+            return x ** 5 - 2 * x ** 3 * y ** 2 + 0.5 * y ** 4 - 0.3 * y * z + z ** 5 - x * y * z + 1.0
+        if sym == 1:
+            return (x ** 5 - x * y ** 3 + 0.4 * y ** 2 + 1.0) * par(z, sz)
+        return par(x, sx) * par(y, sy) * par(z, sz)
+
+    f0 = P(x, y, z)
+    rng = np.random.default_rng(31)
+    chi0 = -0.3 + 0.05 * rng.standard_normal(f0.shape)
+    lap0 = -0.2 + 0.05 * rng.standard_normal(f0.shape)
+    dT = 0.4
+    f = np.full(f0.shape, 1e30)
+    assert orc.sommerfeld_interp(f0, f, chi0, lap0, X, Y, Z, bbox, dT, soa, sym) == 0
+    r = (lap0 + 1) * np.sqrt(1 + chi0) * dT / np.sqrt(x * x + y * y + z * z)
+    want = (1 - r) * P((1 - r) * x, (1 - r) * y, (1 - r) * z)
+    written = f != 1e30
+    face = np.zeros(f0.shape, bool)
+    face[-1], face[:, -1], face[:, :, -1] = True, True, True
+    if sym < 2:
+        face[:, :, 0] = face[:, 0] = True
+    if sym == 0:
+        face[0] = True
+    assert np.array_equal(written, face)
+    scale = np.max(np.abs(want[face]))
+    assert np.max(np.abs(f[face] - want[face])) < 1e-11 * scale
+    assert np.max(r[face]) * np.max(np.abs(np.stack([x, y, z])[:, face]) / 0.27) > 1.0  # moves > 1 cell

@@ -207,3 +207,24 @@ def test_step_bitwise(orc, ref, bc):
         assert np.array_equal(h1, h2)
     for a, b in zip(U1, U2):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dT", [0.05, 0.6])
+def test_sommerfeld_interpolating_predictor_bitwise(orc, ref, dT):
+    """sommerfeld_rout with precor == 0: the outgoing-wave value interpolated
+    from f0 at the point moved inwards by alpha sqrt(chi) dT along the ray
+    (6^3 Neville block, clamped at the array's edges, mirrored below a
+    symmetry plane), on every face layer; small and large displacements."""
+    rng = np.random.default_rng(21)
+    for X, Y, Z, bbox, sym in _bam_cases():
+        shape = (len(Z), len(Y), len(X))
+        f0 = rng.standard_normal(shape)
+        chi0 = -0.3 + 0.05 * rng.standard_normal(shape)
+        lap0 = -0.2 + 0.05 * rng.standard_normal(shape)
+        start = rng.standard_normal(shape)
+        for par in PARS[:3]:
+            a, b = start.copy(), start.copy()
+            assert orc.sommerfeld_interp(f0, a, chi0, lap0, X, Y, Z, bbox, dT, par, sym) == 0
+            ref.sommerfeld_interp(f0, b, chi0, lap0, X, Y, Z, bbox, dT, par, sym)
+            assert np.array_equal(a, b), (sym, par)
+            assert not np.array_equal(a, start)
