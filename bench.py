@@ -166,16 +166,17 @@ def boundary_leg(steps=5, warmup=2):
     zones (the headline workload: no boundary kernel), and the reference's
     one-layer copy, no boundary, and level-0 Sommerfeld (the 3-deep shell of
     308,104 points through the boundary kernels with the reference's edge
-    stencils).  Wall ms per RK4 step, one launch per kind of work (mode 0)."""
+    stencils).  Wall ms per RK4 step with the predictor's residuals, one
+    launch per kind of work (mode 0)."""
     import amssbssn as A
     cfg = A.config1(n=128)
     out = dict(workload="two-puncture 128^3 + 3 ghosts, GAUGE 0, plain mode", steps=steps)
     for name, bc in (("frozen", A.BC_FROZEN_GHOSTS), ("copy", A.BC_COPY_LAYER), ("none", A.BC_NONE),
                      ("sommerfeld", A.BC_SOMMERFELD)):
         lv = A.Level.from_config(cfg, gauge=0, bc=bc)
-        assert lv.step(cfg.dt, warmup) == 0
+        assert lv.step(cfg.dt, warmup, want_ham=A.RES_EVERY_STEP) == 0
         t0 = time.perf_counter()
-        assert lv.step(cfg.dt, steps) == 0
+        assert lv.step(cfg.dt, steps, want_ham=A.RES_EVERY_STEP) == 0
         out[name + "_ms_per_step"] = round((time.perf_counter() - t0) * 1e3 / steps, 3)
         lv.close()
     return out

@@ -237,6 +237,38 @@ int amss_level_constraint_series(amss_level *L, int patch, double *out, int maxs
 /* The stream the level's work is ordered on (a hipStream_t). */
 void *amss_level_stream(amss_level *L);
 
+/* ---- a hierarchy of refinement levels ------------------------------------ */
+/* Replaces bssn_class::RecursiveStep with RestrictProlong (bssn_class.C:2323-2403,
+ * 5724-5904; Parallel::Restrict, OutBdLow2Hi, prepare_inter_time_level,
+ * Parallel.C:4329-4441, 4487-4640) for a hierarchy of nlev levels on one GPU.
+ * npatch[l]: patches of level l; shapes / bboxes: all patches, level after
+ * level, as amss_level_create.  Level 0 is a level of boundary mode bc0 and
+ * ghost width 3; level l > 0 has half the spacing of level l - 1, and each of
+ * its patch arrays holds the reference's buffer zone of 6 cells
+ * (buffer_width, macrodef.h:88) on every face but a lower face on a symmetry
+ * plane; such a patch must sit on the coarse cells and its buffer zone inside
+ * the coarse patches less their own buffer zones and 3 ghost cells.  A fine
+ * level's Step copies the base state onto the outermost layer of its faces
+ * (sommerfeld_rout with cor, bssn_class.C:3124-3135), i.e. boundary mode
+ * AMSS_BC_COPY_LAYER.  Returns NULL on error. */
+typedef struct amss_amr amss_amr;
+amss_amr *amss_amr_create(int nlev, const int *npatch, const int *shapes, const double *bboxes, int gauge,
+                          double eps, double chitiny, int bc0, int symmetry);
+void amss_amr_destroy(amss_amr *A);
+/* Level lev of the hierarchy as a level handle (copy in / out, fields,
+ * constraint norms, timing); owned by the hierarchy: amss_level_destroy on
+ * it does nothing. */
+amss_level *amss_amr_level(amss_amr *A, int lev);
+/* nsteps coarse steps of size dT: level l takes 2^l steps of dT / 2^l, each
+ * followed by the restriction of its state onto level l - 1 and the
+ * prolongation of level l - 1 into its buffer zone, with level l - 1
+ * interpolated in time to the intermediate time (average of two time levels
+ * on the first coarse step, average2 of three after it).  mode / want_ham:
+ * as amss_level_step, for every level step. */
+int amss_amr_step(amss_amr *A, double dT, int nsteps, int mode, int want_ham);
+/* The hierarchy's physical time (the finest level's, PhysTime). */
+double amss_amr_time(amss_amr *A);
+
 /* ---- on-disk formats of the reference ------------------------------------ */
 /* One grid function of one patch in the layout of Parallel::writefile
  * (Parallel.C:1049-1071, read by plot_binary_data.py:37-43): time (double),

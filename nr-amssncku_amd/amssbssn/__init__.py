@@ -9,3 +9,4 @@ from .grid import (FIELDS, NF, Config, PatchGeom, config0, config1, config2, cub
 from .level import (BC_COPY_LAYER, BC_FROZEN_GHOSTS, BC_NONE, BC_SOMMERFELD, LIST_COR, LIST_OLD, LIST_PRE,  # noqa: F401
                     LIST_RES, LIST_RHS, LIST_STATE, MODE_BATCHED, MODE_GRAPH, MODE_PLAIN, RES_EVERY_STEP,
                     RES_LAST_STEP, RES_NONE, Level)
+from .amr import Hierarchy, nested_box  # noqa: F401,E402

@@ -47,6 +47,11 @@ SIGNATURES = {
     "amss_level_constraint_norms": ([C.c_void_p, C.c_int, _dp], C.c_int),
     "amss_level_constraint_series": ([C.c_void_p, C.c_int, _dp, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "amss_level_stream": ([C.c_void_p], C.c_void_p),
+    "amss_amr_create": ([C.c_int, _ip, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int], C.c_void_p),
+    "amss_amr_destroy": ([C.c_void_p], None),
+    "amss_amr_level": ([C.c_void_p, C.c_int], C.c_void_p),
+    "amss_amr_step": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int], C.c_int),
+    "amss_amr_time": ([C.c_void_p], C.c_double),
     "amss_level_write_bin": ([C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double], C.c_int),
     "amss_level_checkpoint_write": ([C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, _dp, _dp], C.c_int),
     "amss_level_checkpoint_read": ([C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp,

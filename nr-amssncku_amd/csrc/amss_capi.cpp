@@ -14,6 +14,7 @@
 
 #include "../../include/amss_bssn.h"
 #include "amss_adjoint.h"
+#include "amss_amr.h"
 #include "amss_host.h"
 #include "amss_xfer.h"
 
@@ -580,6 +581,7 @@ int amss_enforce_ga_dev(const int* ex, double* const* ga, void* stream) {
 // ---- levels ----------------------------------------------------------------
 struct amss_level {
   BssnLevel* L;
+  bool owned = true;  // false: a level of a hierarchy (amss_amr_level), freed with it
 };
 
 amss_level* amss_level_create(int npatch, const int* shapes, const double* bboxes, int gauge, double eps,
@@ -598,10 +600,57 @@ amss_level* amss_level_create(int npatch, const int* shapes, const double* bboxe
 }
 
 void amss_level_destroy(amss_level* L) {
-  if (!L) return;
+  if (!L || !L->owned) return;
   delete L->L;
   delete L;
 }
+
+// ---- level hierarchies (Berger-Oliger sub-cycling) --------------------------
+struct amss_amr {
+  AmrHierarchy* H;
+  std::vector<amss_level> levels;  // borrowed handles of the hierarchy's levels
+};
+
+amss_amr* amss_amr_create(int nlev, const int* npatch, const int* shapes, const double* bboxes, int gauge,
+                          double eps, double chitiny, int bc0, int symmetry) {
+  if (nlev < 1 || !npatch || !shapes || !bboxes) {
+    fail(AMSS_ERR_ARG, "amss_amr_create: bad arguments");
+    return nullptr;
+  }
+  auto* H = new AmrHierarchy(nlev, npatch, shapes, bboxes, gauge, eps, chitiny, bc0, symmetry);
+  if (!H->ok()) {
+    fail(AMSS_ERR_ARG, "amss_amr_create: " + H->err());
+    delete H;
+    return nullptr;
+  }
+  auto* A = new amss_amr;
+  A->H = H;
+  for (int l = 0; l < nlev; ++l) A->levels.push_back(amss_level{H->level(l), false});
+  return A;
+}
+
+void amss_amr_destroy(amss_amr* A) {
+  if (!A) return;
+  delete A->H;
+  delete A;
+}
+
+amss_level* amss_amr_level(amss_amr* A, int lev) {
+  if (!A || lev < 0 || lev >= (int)A->levels.size()) {
+    fail(AMSS_ERR_ARG, "amss_amr_level: no such level");
+    return nullptr;
+  }
+  return &A->levels[lev];
+}
+
+int amss_amr_step(amss_amr* A, double dT, int nsteps, int mode, int want_ham) {
+  if (!A || nsteps < 0) return fail(AMSS_ERR_ARG, "amss_amr_step: bad arguments");
+  const int r = A->H->step(dT, nsteps, mode, want_ham);
+  if (r) return fail(r, "amss_amr_step: " + A->H->err());
+  return 0;
+}
+
+double amss_amr_time(amss_amr* A) { return A ? A->H->time() : 0.0; }
 
 double* amss_level_field(amss_level* L, int patch, int list, int field) {
   if (!L) return nullptr;

@@ -110,7 +110,10 @@ class BssnLevel {
   const BatchTune& tune() const { return tune_; }
   int npatch() const { return (int)patches_.size(); }
   const Patch& patch(int i) const { return patches_[i]; }
+  const Block& block(int i) const { return *blocks_[i]; }
   long npts(int patch) const;
+  // Parallel::Sync of one list (AMSS_LIST_*) on the level's stream
+  int sync_list(int list);
 
   // the reference's lists (bssn_class.C:412-535)
   MyList<var>* OldStateList = nullptr;
@@ -147,6 +150,13 @@ class BssnLevel {
   int gauge_, bc_, gw_, symmetry_;
   double eps_, chitiny_;
   hipStream_t stream_ = nullptr;
+  // the boundary kernels' three stencil-order shells run on side streams,
+  // concurrently with the interior kernel (they write disjoint points)
+  hipStream_t side_[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev_ = nullptr, join_ev_[3] = {nullptr, nullptr, nullptr};
+  int launch_edges(const Block& b, int pi, MyList<var>* in, MyList<var>* out, const AuxOut& aux, const Rk4Param& R,
+                   bool res);
+  int join_edges();
   int* dflag_ = nullptr;
   bool ham_valid_ = false;
   // constraint norms: one L2 job per (patch, residual), their partials, the

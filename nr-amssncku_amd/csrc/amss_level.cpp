@@ -247,6 +247,13 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
     sommer_.push_back(g);
   }
   LCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (bc != AMSS_BC_FROZEN_GHOSTS || gw < 3) {
+    LCHK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    for (int o = 0; o < 3; ++o) {
+      LCHK(hipStreamCreateWithFlags(&side_[o], hipStreamNonBlocking));
+      LCHK(hipEventCreateWithFlags(&join_ev_[o], hipEventDisableTiming));
+    }
+  }
   LCHK(hipMalloc(&dflag_, sizeof(int)));
   build_ghost_segments();
   int dev = 0;
@@ -284,7 +291,39 @@ BssnLevel::~BssnLevel() {
   if (dgwork_) (void)hipFree(dgwork_);
   for (void* p : {(void*)dl2_, (void*)dl2part_, (void*)dl2last_, (void*)dseries_})
     if (p) (void)hipFree(p);
+  for (int o = 0; o < 3; ++o) {
+    if (side_[o]) {
+      (void)hipStreamSynchronize(side_[o]);
+      (void)hipStreamDestroy(side_[o]);
+    }
+    if (join_ev_[o]) (void)hipEventDestroy(join_ev_[o]);
+  }
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// The boundary kernel on the shell of patch pi (the reference's edge
+// stencils), one launch per stencil-order shell, each on its own side stream
+// after what stream_ has queued so far: they read the stage input and write
+// only shell points, so they overlap the interior kernel.  join_edges makes
+// stream_ wait for them.
+int BssnLevel::launch_edges(const Block& b, int pi, MyList<var>* in, MyList<var>* out, const AuxOut& aux,
+                            const Rk4Param& R, bool res) {
+  RCHK(hipEventRecord(fork_ev_, stream_));
+  for (int o = 0; o < 3; ++o) {
+    RCHK(hipStreamWaitEvent(side_[o], fork_ev_, 0));
+    Rk4Param Ro = R;
+    Ro.edge_order = o;
+    launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, edge_box_[pi],
+               coef_[pi], Ro, gauge_, res, dflag_, true, side_[o], /*interior=*/false, sommer_[pi]);
+    RCHK(hipEventRecord(join_ev_[o], side_[o]));
+  }
+  return 0;
+}
+
+int BssnLevel::join_edges() {
+  for (int o = 0; o < 3; ++o) RCHK(hipStreamWaitEvent(stream_, join_ev_[o], 0));
+  return 0;
 }
 
 double* BssnLevel::field(int patch, int list, int v) {
@@ -354,6 +393,20 @@ void BssnLevel::build_ghost_segments() {
     }
 }
 
+int BssnLevel::sync_list(int list) {
+  MyList<var>* l = nullptr;
+  switch (list) {
+    case AMSS_LIST_STATE: l = StateList; break;
+    case AMSS_LIST_OLD: l = OldStateList; break;
+    case AMSS_LIST_PRE: l = SynchList_pre; break;
+    case AMSS_LIST_COR: l = SynchList_cor; break;
+    case AMSS_LIST_RHS: l = RHSList; break;
+    default: err_ = "sync_list: not an evolved-field list"; return AMSS_ERR_ARG;
+  }
+  sync(l, 0);
+  return 0;
+}
+
 void BssnLevel::sync(MyList<var>* list, int /*stage*/) {
   for (const GhostSeg& g : segs_) {
     SegDesc d;
@@ -398,16 +451,15 @@ int BssnLevel::Step(double dT, bool want_res) {
       for (int a = 0; a < 7; ++a, c = c->next) aux.res[a] = b.fgfs[c->data->sgfn];
       Rk4Param R{it, variant_, cst[it], tinny};
       const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
+      if (edge)
+        if (int r = launch_edges(b, (int)pi, in, out, aux, R, want_res && it == 0)) return r;
       Ev* ev = nullptr;
       if (int r = time_event_begin(it, &ev)) return r;
       launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux, box,
                  coef_[pi], R, gauge_, want_res && it == 0, dflag_, false, stream_);
       if (ev) RCHK(hipEventRecord(ev->b, stream_));
-      if (edge) {
-        launch_rk4(list_in(b, in), list_in(b, StateList), list_out(b, RHSList), list_out(b, out), aux,
-                   edge_box_[pi], coef_[pi], R, gauge_, want_res && it == 0, dflag_, true, stream_,
-                   /*interior=*/false, sommer_[pi]);
-      }
+      if (edge)
+        if (int r = join_edges()) return r;
       if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
         Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
         launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);

@@ -39,6 +39,7 @@ struct Rk4Param {
   int variant;   // interior kernel: 0 direct (L1/L2 reads), 1 LDS-staged tiles
   double c;      // 0.5 dt, 0.5 dt, dt, dt/6
   double tinny;  // chitiny - 1 (bssn_class.C:179)
+  int edge_order = -1;  // boundary launches: the one stencil-order shell to run (-1: all three)
 };
 
 // RHS of compute_rhs_bssn on the interior box (and, with edge, on the rest
@@ -79,6 +80,11 @@ void launch_lowerbound(double* chi, long n, double tinny, hipStream_t s);
 void launch_rk4_update(const double* f0, double* f1, double* frhs, long n, double dT, int stage, hipStream_t s);
 void launch_copy_box(const double* f0, double* f, const Box& box, hipStream_t s);
 int launch_sumsq(const double* f, const Box& box, double* part, int maxblocks, hipStream_t s);
+
+// out = c1 f1 + c2 f2 (+ c3 f3) over n points of all 24 fields (the time
+// interpolation of prepare_inter_time_level).
+void launch_time_interp(const FieldsIn& f1, const FieldsIn& f2, const FieldsIn& f3, const FieldsOut& out, long n,
+                        double c1, double c2, double c3, bool three, hipStream_t s);
 
 // One L2 reduction: dv times the sum of squares of f over the index box
 // [lo, hi) (strides sy, sz), reduced through nparts workgroup partials at

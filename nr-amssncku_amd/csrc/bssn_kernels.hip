@@ -149,6 +149,19 @@ __global__ void __launch_bounds__(256) sumsq_kernel(const double* __restrict__ f
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// prepare_inter_time_level (Parallel.C:4329-4441; average / average2 of
+// fmisc.f90:1532-1592) on the 24 fields of one patch:
+// out = c1 f1 + c2 f2 (+ c3 f3 when three).
+__global__ void __launch_bounds__(256) time_interp_kernel(FieldsIn f1, FieldsIn f2, FieldsIn f3, FieldsOut out,
+                                                          long n, double c1, double c2, double c3, int three) {
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int v = blockIdx.y;
+  double r = c1 * f1.p[v][p] + c2 * f2.p[v][p];
+  if (three) r = r + c3 * f3.p[v][p];
+  out.p[v][p] = r;
+}
+
 // Partial sums of squares of the L2 jobs: workgroup x of job y covers every
 // (nparts * 256)-th point of the job's box, starting at x * 256 + lane.
 __global__ void __launch_bounds__(256) l2_parts_kernel(const L2Job* __restrict__ jobs, double* __restrict__ part) {
@@ -390,6 +403,11 @@ int launch_sumsq(const double* f, const Box& box, double* part, int maxblocks, h
   if (nb < 1) nb = 1;
   sumsq_kernel<<<nb, 256, 0, s>>>(f, box, part);
   return nb;
+}
+
+void launch_time_interp(const FieldsIn& f1, const FieldsIn& f2, const FieldsIn& f3, const FieldsOut& out, long n,
+                        double c1, double c2, double c3, bool three, hipStream_t s) {
+  if (n > 0) time_interp_kernel<<<dim3(blocks_for(n, 256), NFIELD), 256, 0, s>>>(f1, f2, f3, out, n, c1, c2, c3, three);
 }
 
 void launch_l2_jobs(const L2Job* jobs, int njobs, int maxparts, double* part, double* out, hipStream_t s) {

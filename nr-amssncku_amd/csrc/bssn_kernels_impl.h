@@ -291,7 +291,7 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
     const EdgeShells es = edge_shells(box);
     for (int o = 0; o < 3; ++o) {
       const Shell& sh = es.s[o];
-      if (sh.size() <= 0) continue;
+      if (sh.size() <= 0 || (R.edge_order >= 0 && o != R.edge_order)) continue;
       const unsigned gr = blocks_for(sh.size(), 256);
 #define AMSS_EDGE(ST, CO)                                                                                      \
   do {                                                                                                         \

@@ -367,3 +367,119 @@ def step_driver(be, U, X, Y, Z, bbox, dT, eps, chitiny, symmetry, bc, gw, want_h
     for v in range(NF):
         np.copyto(U[v], cor[v])
     return gont, ham
+
+
+# ---------------------------------------------------------------------------
+# Berger-Oliger sub-cycling over levels of one patch each, in the order of
+# bssn_class::RecursiveStep (bssn_class.C:2323-2403) and RestrictProlong
+# (:5724-5904, RPB 0, MIXOUTB 0), with the regions of Parallel::Restrict
+# (build_owned_gsl2 onto build_complete_gsl, Parallel.C:4487-4530, 2762-2815)
+# and Parallel::OutBdLow2Hi (build_buffer_gsl from build_owned_gsl4,
+# Parallel.C:4580-4612, 4794-4843, 2922-3013).
+BUFFER_WIDTH = 6  # macrodef.h:88
+
+
+class AmrLevel:
+    """One level (one patch): the array's bbox and shape, its buffer widths
+    (lli, uui), and the reference's State / Old / cor lists after a step."""
+
+    def __init__(self, U, bbox, lli=(0, 0, 0), uui=(0, 0, 0)):
+        nz, ny, nx = U[0].shape
+        self.shape = (nx, ny, nz)
+        self.bbox = tuple(bbox)
+        self.h = tuple((bbox[3 + a] - bbox[a]) / self.shape[a] for a in range(3))
+        self.X, self.Y, self.Z = [bbox[a] + (np.arange(self.shape[a]) + 0.5) * self.h[a] for a in range(3)]
+        self.lli, self.uui = tuple(lli), tuple(uui)
+        self.U = [u.copy() for u in U]
+        self.old = [u.copy() for u in U]
+        self.cor = [u.copy() for u in U]
+
+
+def _overlap(a, b, hmin):
+    lo = [max(a[0][d], b[0][d]) for d in range(3)]
+    hi = [min(a[1][d], b[1][d]) for d in range(3)]
+    if any(hi[d] - lo[d] < hmin[d] / 2 for d in range(3)):  # the Cell rule of build_gstl
+        return None
+    return lo, hi
+
+
+def _on_plane(axis, symmetry, x, h):
+    return (symmetry > 0 if axis == 2 else symmetry > 1) and abs(x) < h / 2
+
+
+def amr_restrict_prolong(be, C, F, same_time, three, symmetry=0):
+    """RestrictProlong between coarse level C and fine level F (one patch each)."""
+    if not same_time:  # prepare_inter_time_level into SynchList_pre (average2 / average)
+        if three:
+            tgt = [3.0 / 8.0 * a + 3.0 / 4.0 * b + -1.0 / 8.0 * c for a, b, c in zip(C.U, C.old, C.cor)]
+        else:
+            tgt = [0.5 * (a + b) for a, b in zip(C.U, C.old)]
+    else:
+        tgt = C.U
+    owned2 = ([F.bbox[a] + F.lli[a] * F.h[a] for a in range(3)], [F.bbox[3 + a] - F.uui[a] * F.h[a] for a in range(3)])
+    complete = (list(C.bbox[:3]), list(C.bbox[3:]))
+    r = _overlap(owned2, complete, F.h)
+    if r is not None:
+        for v in range(NF):
+            assert be.restrict3(tgt[v], C.bbox[:3], C.bbox[3:], F.U[v], F.bbox[:3], F.bbox[3:], r[0], r[1],
+                                PARITY[v], symmetry) == 0
+    # the fine buffer zone: the whole array less the inner box, as slabs
+    inner_lo, inner_hi = owned2
+    rest_lo, rest_hi = list(F.bbox[:3]), list(F.bbox[3:])
+    slabs = []
+    for a in (2, 1, 0):
+        if F.lli[a] > 0:
+            hi = list(rest_hi)
+            hi[a] = inner_lo[a]
+            slabs.append((list(rest_lo), hi))
+        if F.uui[a] > 0:
+            lo = list(rest_lo)
+            lo[a] = inner_hi[a]
+            slabs.append((lo, list(rest_hi)))
+        rest_lo[a], rest_hi[a] = inner_lo[a], inner_hi[a]
+    owned4 = ([C.bbox[a] if _on_plane(a, symmetry, C.bbox[a], C.h[a]) else C.bbox[a] + (C.lli[a] + 3) * C.h[a]
+               for a in range(3)], [C.bbox[3 + a] - (C.uui[a] + 3) * C.h[a] for a in range(3)])
+    for s in slabs:
+        r = _overlap(s, owned4, F.h)
+        if r is None:
+            continue
+        for v in range(NF):
+            assert be.prolong3(tgt[v], C.bbox[:3], C.bbox[3:], F.U[v], F.bbox[:3], F.bbox[3:], r[0], r[1],
+                               PARITY[v], symmetry) == 0
+
+
+def amr_step_driver(be, levels, dT, eps, chitiny=1e-5, gauge=None, bc0=0, symmetry=0, t=(0.0, 0.0)):
+    """One coarse step of the hierarchy `levels` (AmrLevel list, coarse first)
+    in place; t = (PhysTime, StartTime) before the step.  Returns the new
+    PhysTime."""
+    state = {"t": t[0]}
+
+    def level_step(L, lev, dTl):
+        # Step enforces StateList in place before the predictor
+        # (bssn_class.C:3051-3057), so the state that the rotation makes the
+        # old one is the enforced U^n
+        base = [u.copy() for u in L.U]
+        be.enforce_ga(base)
+        new = [u.copy() for u in L.U]
+        g, _ = be.step(new, L.X, L.Y, L.Z, L.bbox, dTl, eps, chitiny, symmetry=symmetry, gauge=gauge,
+                       bc=bc0 if lev == 0 else 1, gw=3)
+        assert g == 0
+        # the list rotation (bssn_class.C:3845-3859): State <- new, Old <- State, cor <- Old
+        L.cor, L.old, L.U = L.old, base, new
+
+    def rec(lev):
+        dTl = dT * 0.5 ** lev
+        nit = 1 if lev == 0 else 2
+        for i in range(nit):
+            same_time = i == nit - 1
+            level_step(levels[lev], lev, dTl)
+            if lev + 1 < len(levels):
+                rec(lev + 1)
+            else:
+                state["t"] += dT * 0.5 ** lev
+            if lev > 0:
+                three = (state["t"] - dTl) - t[1] > dTl / 2  # fgt(PhysTime - dT_lev, StartTime, dT_lev / 2)
+                amr_restrict_prolong(be, levels[lev - 1], levels[lev], same_time, three, symmetry)
+
+    rec(0)
+    return state["t"]

@@ -8,6 +8,13 @@ the stored arrays are the reference's outputs.
   python tests/golden/make_golden.py c0         # one RK4 step, 48^3 single puncture
   python tests/golden/make_golden.py c1small    # 100 RK4 steps, 32^3 two punctures
   python tests/golden/make_golden.py c1full     # 100 RK4 steps, 128^3 two punctures (minutes)
+  python tests/golden/make_golden.py c1full0    # the same at GAUGE 0, from the C restatement
+
+c1full0 comes from the oracle's C restatement (the reference build is
+configured with GAUGE 2, so it cannot run GAUGE 0); the restatement is
+pinned to the reference build bit for bit at GAUGE 2 and its GAUGE 0 terms
+by closed forms (tests/test_oracle_*.py).  The 128^3 fixtures also hold the
+seven constraint norms of the predictors of steps 1, 50 and 100.
 """
 import json
 import os
@@ -79,16 +86,19 @@ def make_small(ref):
     np.savez_compressed(os.path.join(HERE, "ref_rhs_stencils_8x9x10.npz"), **out)
 
 
-def run_steps(ref, cfg, nsteps, nthreads=None, ham_steps=()):
+def run_steps(ref, cfg, nsteps, nthreads=None, ham_steps=(), gauge=None):
     import cpu_baseline as CB
     p = cfg.patches[0]
     U = cfg.initial_state(0)
     X, Y, Z = p.coords()
     hams = {}
     if p.npts > 200000:
-        stp = CB.SlabStepper(ref, p, cfg.dt, cfg.eps, cfg.chitiny, gauge=None, nthreads=nthreads, gw=p.gw)
+        stp = CB.SlabStepper(ref, p, cfg.dt, cfg.eps, cfg.chitiny, gauge=gauge, nthreads=nthreads, gw=p.gw)
         for n in range(nsteps):
+            if (n + 1) in ham_steps:  # the predictor's residuals of step n + 1, from its base state
+                hams[n + 1] = stp.residual_norms(U)
             stp.step(U)
+            print(f"step {n + 1}", flush=True)
         stp.close()
     else:
         for n in range(nsteps):
@@ -120,11 +130,14 @@ def main(what):
         np.savez_compressed(os.path.join(HERE, "ref_c1_32cubed_step100_gauge2.npz"), idx=idx,
                             ham_steps=np.array(sorted(hams)), ham_l2=np.array([hams[k] for k in sorted(hams)]),
                             **summary(U, idx))
-    elif what == "c1full":
+    elif what in ("c1full", "c1full0"):
         cfg = A.config1(n=128, steps=100)
-        U, _ = run_steps(ref, cfg, 100)
+        be, gauge = (ref, None) if what == "c1full" else (O.Orc(), 0)
+        U, hams = run_steps(be, cfg, 100, ham_steps=(1, 50, 100), gauge=gauge)
         idx = sample_index(U[0].shape)
-        np.savez_compressed(os.path.join(HERE, "ref_c1_128cubed_step100_gauge2.npz"), idx=idx, **summary(U, idx))
+        name = "ref_c1_128cubed_step100_gauge2.npz" if what == "c1full" else "orc_c1_128cubed_step100_gauge0.npz"
+        np.savez_compressed(os.path.join(HERE, name), idx=idx, con_steps=np.array(sorted(hams)),
+                            con_l2=np.array([hams[k] for k in sorted(hams)]), **summary(U, idx))
     print(json.dumps(dict(fixture=what, seconds=round(time.time() - t0, 1))))
 
 

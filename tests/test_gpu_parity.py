@@ -367,6 +367,48 @@ def test_two_patches_equal_one_patch():
         assert np.array_equal(w[:, :, n // 2:], b), v
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_split_patches_match_whole_cube_oracle(orc, mode):
+    """Four patches tiling a cube (split along x and z, uneven), evolved with
+    the on-device exchange, against the oracle's whole-cube evolution: the
+    owned points of every patch to 1e-12 after 3 steps."""
+    import amssbssn as A
+    n, gw = 16, 3
+    whole = A.cube_patch(n, 8.0)
+    h = whole.h[0]
+    cuts = ((0, 6), (6, n))  # owned index ranges along x and along z
+    patches, ranges = [], []
+    for kz in cuts:
+        for ix in cuts:
+            shape = (ix[1] - ix[0] + 2 * gw, n + 2 * gw, kz[1] - kz[0] + 2 * gw)
+            lo = (-8.0 + (ix[0] - gw) * h, -8.0 - gw * h, -8.0 + (kz[0] - gw) * h)
+            bb = lo + (lo[0] + shape[0] * h, lo[1] + shape[1] * h, lo[2] + shape[2] * h)
+            patches.append(A.PatchGeom(shape, bb, gw))
+            ranges.append((ix, kz))
+    pun = [(0.5, 1.0, 0.5, 0.0), (0.5, -1.5, 0.0, 0.3)]
+    c1 = A.Config("whole", [whole], dt=0.25 * h, steps=3, punctures=pun, a_noise=1e-6)
+    X, Y, Z = whole.coords()
+    U = c1.initial_state(0)
+    lv = A.Level(patches, eps=c1.eps)
+    try:
+        for p, (ix, kz) in enumerate(ranges):
+            lv.set_state(p, [u[kz[0]:kz[1] + 2 * gw, :, ix[0]:ix[1] + 2 * gw].copy() for u in U])
+        assert lv.step(c1.dt, 3, mode) == 0
+        got = [lv.get_state(p) for p in range(len(patches))]
+    finally:
+        lv.close()
+    for _ in range(3):
+        orc.step(U, X, Y, Z, whole.bbox, c1.dt, c1.eps, c1.chitiny, gauge=0, bc=0, gw=gw)
+    for v in range(24):
+        ref = U[v][gw:-gw, gw:-gw, gw:-gw]
+        m = float(np.max(np.abs(ref)))
+        for (ix, kz), G in zip(ranges, got):
+            a = G[v][gw:-gw, gw:-gw, gw:-gw]
+            b = ref[kz[0]:kz[1], :, ix[0]:ix[1]]
+            e = float(np.max(np.abs(a - b)))
+            assert e <= 1e-12 * (m if m > 0 else 1.0), (v, ix, kz, e)
+
+
 # ---------------------------------------------------------------------------
 # Against the reference Fortran's golden vectors (tests/golden, GAUGE 2: the
 # reference build's configured gauge).  Per field: relative difference at the
