@@ -147,6 +147,23 @@ int amss_rhs_bssn_dev(const int *ex, const double *h, const double *const *state
                       int symmetry, const double *origin, double eps, int co, int gauge, int *flag,
                       void *stream);
 
+/* amss_rhs_bssn_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as amss_rhs_bssn_dev; where that entry
+ * point allocates (a zeroed matter array for a vacuum call, matter NULL; a
+ * spare NaN flag when flag is NULL), this one uses a fixed device array of
+ * the library (one per GPU, held until the process ends): a vacuum call
+ * accepts at most 2,500,000 points (ghosts included) and returns 2 with a
+ * message beyond that (pass zeroed matter arrays instead).  The first call on
+ * a device looks the array up and clears it (a kernel on stream).  Calls that
+ * overlap on one GPU share the array; only the spare flag is written, so such
+ * calls race on that flag alone. */
+int amss_rhs_bssn_dev_noalloc(const int *ex, const double *h, const double *const *state, double *const *rhs,
+                              const double *const *matter, double *const *gam, double *const *ric,
+                              double *const *res, int symmetry, const double *origin, double eps, int co,
+                              int gauge, int *flag, void *stream);
+
 /* prolong3_ / restrict3_ on device arrays, ordered on stream (no
  * synchronisation); same arguments and return codes. */
 int amss_prolong3_dev(const double *llbc, const double *uubc, const int *extc, const double *func,

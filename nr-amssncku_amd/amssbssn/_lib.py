@@ -36,6 +36,9 @@ SIGNATURES = {
                            C.c_int),
     "amss_rhs_bssn_dev": ([_ip, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                            C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    "amss_rhs_bssn_dev_noalloc": ([_ip, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
+                                  C.c_int),
     "amss_enforce_ga_dev": ([_ip, C.c_void_p, C.c_void_p], C.c_int),
     "amss_level_create": ([C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
                           C.c_void_p),

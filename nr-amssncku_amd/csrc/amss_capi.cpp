@@ -516,21 +516,33 @@ static double* g_zero = nullptr;  // zero matter arrays for vacuum calls
 static size_t g_zero_cap = 0;
 static std::mutex g_zero_mu;
 
-int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state, double* const* rhs,
-                      const double* const* matter, double* const* gam, double* const* ric, double* const* res,
-                      int symmetry, const double* origin, double eps, int co, int gauge, int* flag,
-                      void* stream) {
+// Argument checks and launch shared by amss_rhs_bssn_dev and its no-allocation
+// variant.  zero: 10 pointers to a zeroed array of >= n doubles (used when
+// matter is NULL); spare_flag: where the kernels record a NaN input when the
+// caller passes no flag.  The two entry points differ only in where these
+// two come from.
+struct RhsDevArgs {
+  Box box;
+  long n;
+};
+static int rhs_dev_check(const int* ex, const double* h, const double* const* state, double* const* rhs,
+                         double* const* gam, double* const* ric, double* const* res, int symmetry,
+                         const double* origin, int co, int gauge, const char* who, RhsDevArgs& a) {
   if (!ex || !h || !state || !rhs || !gam || !ric || (co == 0 && !res) || (symmetry != 0 && !origin))
-    return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: null argument");
-  if (gauge != 0 && gauge != 2) return fail(AMSS_ERR_UNSUPPORTED, "amss_rhs_bssn_dev: gauge 0 or 2");
-  if (ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: shape below 2");
-  Box box = make_box(ex, 3, 3);
+    return fail(AMSS_ERR_ARG, std::string(who) + ": null argument");
+  if (gauge != 0 && gauge != 2) return fail(AMSS_ERR_UNSUPPORTED, std::string(who) + ": gauge 0 or 2");
+  if (ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, std::string(who) + ": shape below 2");
+  a.box = make_box(ex, 3, 3);
   if (symmetry != 0)
-    if (int r = set_mirrors(box, symmetry, origin, h, "amss_rhs_bssn_dev")) return r;
-  if (int r = check_upwind_room(box, "amss_rhs_bssn_dev")) return r;
-  const long n = (long)ex[0] * ex[1] * ex[2];
-  if (n >= (1L << 29)) return fail(AMSS_ERR_ARG, "amss_rhs_bssn_dev: more than 2^29 points per block");
-  hipStream_t s = (hipStream_t)stream;
+    if (int r = set_mirrors(a.box, symmetry, origin, h, who)) return r;
+  if (int r = check_upwind_room(a.box, who)) return r;
+  a.n = (long)ex[0] * ex[1] * ex[2];
+  if (a.n >= (1L << 29)) return fail(AMSS_ERR_ARG, std::string(who) + ": more than 2^29 points per block");
+  return 0;
+}
+static int rhs_dev_launch(const RhsDevArgs& a, const double* h, const double* const* state, double* const* rhs,
+                          const double* const* matter, const double* zero, double* const* gam, double* const* ric,
+                          double* const* res, double eps, int co, int gauge, int* flag, hipStream_t s) {
   FieldsIn in, mt;
   FieldsOut out;
   AuxOut aux;
@@ -538,34 +550,100 @@ int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state
     in.p[v] = state[v];
     out.p[v] = rhs[v];
   }
-  if (matter) {
-    for (int m = 0; m < NMATTER; ++m) mt.p[m] = matter[m];
-  } else {
+  for (int m = 0; m < NMATTER; ++m) mt.p[m] = matter ? matter[m] : zero;
+  for (int m = NMATTER; m < NFIELD; ++m) mt.p[m] = mt.p[0];
+  for (int q = 0; q < 18; ++q) aux.gam[q] = gam[q];
+  for (int q = 0; q < 6; ++q) aux.ric[q] = ric[q];
+  for (int q = 0; q < 7; ++q) aux.res[q] = res ? res[q] : nullptr;
+  const Coef C = make_coef(h, eps > 0 ? eps : 0.0);
+  launch_rhs(in, mt, out, aux, a.box, C, gauge, co, flag, true, s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int amss_rhs_bssn_dev(const int* ex, const double* h, const double* const* state, double* const* rhs,
+                      const double* const* matter, double* const* gam, double* const* ric, double* const* res,
+                      int symmetry, const double* origin, double eps, int co, int gauge, int* flag,
+                      void* stream) {
+  RhsDevArgs a;
+  if (int r = rhs_dev_check(ex, h, state, rhs, gam, ric, res, symmetry, origin, co, gauge, "amss_rhs_bssn_dev", a))
+    return r;
+  const double* zero = nullptr;
+  if (!matter) {
     std::lock_guard<std::mutex> lk(g_zero_mu);
-    if ((size_t)n > g_zero_cap) {
+    if ((size_t)a.n > g_zero_cap) {
       if (g_zero) HIPCHK(hipFree(g_zero));
       g_zero = nullptr;
       g_zero_cap = 0;
-      HIPCHK(hipMalloc(&g_zero, n * sizeof(double)));
-      HIPCHK(hipMemset(g_zero, 0, n * sizeof(double)));
-      g_zero_cap = n;
+      HIPCHK(hipMalloc(&g_zero, a.n * sizeof(double)));
+      HIPCHK(hipMemset(g_zero, 0, a.n * sizeof(double)));
+      g_zero_cap = a.n;
     }
-    for (int m = 0; m < NMATTER; ++m) mt.p[m] = g_zero;
+    zero = g_zero;
   }
-  for (int m = NMATTER; m < NFIELD; ++m) mt.p[m] = mt.p[0];
-  for (int a = 0; a < 18; ++a) aux.gam[a] = gam[a];
-  for (int a = 0; a < 6; ++a) aux.ric[a] = ric[a];
-  for (int a = 0; a < 7; ++a) aux.res[a] = res ? res[a] : nullptr;
   int* fl = flag;
   if (!fl) {
     std::lock_guard<std::mutex> lk(g_shim.mu);
     if (int r = g_shim.ensure(1)) return r;
     fl = g_shim.flag;
   }
-  const Coef C = make_coef(h, eps > 0 ? eps : 0.0);
-  launch_rhs(in, mt, out, aux, box, C, gauge, co, fl, true, s);
-  HIPCHK(hipGetLastError());
+  return rhs_dev_launch(a, h, state, rhs, matter, zero, gam, ric, res, eps, co, gauge, fl, (hipStream_t)stream);
+}
+
+// ---- the no-allocation variant --------------------------------------------
+// Its scratch is one file-scope device array per GPU: the zeroed vacuum
+// matter array (kNoAllocMaxPoints doubles) followed by the spare NaN flag.
+// In host code `scratch` names a host-side stand-in, so its device address
+// is looked up once per device (hipGetSymbolAddress) and only that address
+// is used afterwards.
+static constexpr long kNoAllocMaxPoints = 2500000;
+static constexpr long kScratchDoubles = kNoAllocMaxPoints + 32;  // + flag, padded to 256 bytes
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices];  // device address of `scratch` per device (null: not yet set up)
+
+__global__ void zero_fill_kernel(double* __restrict__ p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// The current device's scratch address; on the first call on a device it is
+// looked up and the array cleared (a fill kernel ordered on `s`: no memset
+// node, so a caller may capture this call in a graph).
+static int scratch_for_current_device(hipStream_t s, double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(AMSS_ERR_UNSUPPORTED, "amss_rhs_bssn_dev_noalloc: device index");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    double* p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(AMSS_ERR_HIP, "amss_rhs_bssn_dev_noalloc: no device address for the scratch array");
+    zero_fill_kernel<<<1024, 256, 0, s>>>(p, kScratchDoubles);
+    HIPCHK(hipGetLastError());
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
   return 0;
+}
+
+int amss_rhs_bssn_dev_noalloc(const int* ex, const double* h, const double* const* state, double* const* rhs,
+                              const double* const* matter, double* const* gam, double* const* ric,
+                              double* const* res, int symmetry, const double* origin, double eps, int co, int gauge,
+                              int* flag, void* stream) {
+  const char* who = "amss_rhs_bssn_dev_noalloc";
+  RhsDevArgs a;
+  if (int r = rhs_dev_check(ex, h, state, rhs, gam, ric, res, symmetry, origin, co, gauge, who, a)) return r;
+  if (!matter && a.n > kNoAllocMaxPoints)
+    return fail(AMSS_ERR_ARG, std::string(who) + ": a vacuum block (matter NULL) of " + std::to_string(a.n) +
+                                  " points exceeds the scratch of " + std::to_string(kNoAllocMaxPoints) +
+                                  " points; pass zeroed matter arrays or use amss_rhs_bssn_dev");
+  hipStream_t s = (hipStream_t)stream;
+  double* sc = nullptr;
+  if (int r = scratch_for_current_device(s, &sc)) return r;
+  int* fl = flag ? flag : reinterpret_cast<int*>(sc + kNoAllocMaxPoints);
+  return rhs_dev_launch(a, h, state, rhs, matter, sc, gam, ric, res, eps, co, gauge, fl, s);
 }
 
 int amss_enforce_ga_dev(const int* ex, double* const* ga, void* stream) {
