@@ -6,7 +6,8 @@
   constraint norms of every step to 1e-12.
 * 64^3 patch, 200 RK4 steps, checkpoint every 20 (the differentiation
   workload): dJ/dm1 and dJ/dx1 from the adjoint against central differences
-  (h = 1e-6) to 1e-6; and on a 24^3 copy (4 steps) dJ/dtheta against torch
+  (dJ/dm1 to 1e-6; dJ/dx1, 2e4 times smaller, to 1e-5); and on a 24^3 copy
+  (4 steps) dJ/dtheta against torch
   autograd of the PyTorch FP64 restatement on the CPU to 1e-9.
 * 16 uneven patches (multiples of 8 in [48,128], seed 2), 400 RK4 steps,
   batched hipGraph forward, checkpoint spacing planned from the free HBM:
@@ -56,8 +57,11 @@ def test_37_patch_level_batched_and_graph_equal_per_patch(mode):
         worst = max(worst, e)
         assert e <= 1e-12, (v, e)
     assert np.max(_rel(nb, na)) <= 1e-12
-    for x, y in zip(pa, pb):
-        assert np.max(_rel(y, x)) <= 1e-12
+    # per patch, relative to the largest patch's norm (a far patch's residual
+    # is rounding-sized, and the tile kernel rounds differently)
+    A_, B_ = np.stack(pa), np.stack(pb)
+    scale = np.max(np.abs(A_), axis=0)
+    assert np.all(np.abs(B_ - A_) <= 1e-12 * scale + 1e-300)
     print(f"37 patches, mode {mode} vs per patch: worst state {worst:.2e}, Ham norms {nb[:, 0]}")
 
 
@@ -69,15 +73,19 @@ def _central(G, cfg, theta, k, h=1e-6, mode=0):
 
 
 def test_64cubed_200_steps_gradient_matches_central_differences():
+    """dJ/dx1 (about 0.022) is 2e4 times smaller than dJ/dm1 on J of about
+    757, whose rounding over 200 steps is about 1e-13 absolute: at h = 1e-6
+    that alone is 4e-6 of dJ/dx1, so x1 takes h = 1e-5, where the truncation
+    error is 4e-6 (it falls as h^2 from 3.8e-4 at h = 1e-4), checked to 1e-5."""
     import amssbssn.gradient as G
     cfg = G.config3(64, steps=200, checkpoint=20)
     theta = cfg.theta0()
     J, g, info = G.value_and_grad(cfg, theta)
-    for k, name in ((0, "m1"), (2, "x1")):
-        fd = _central(G, cfg, theta, k)
+    for k, name, h, tol in ((0, "m1", 1e-6, 1e-6), (2, "x1", 1e-5, 1e-5)):
+        fd = _central(G, cfg, theta, k, h=h)
         rel = abs(fd - g[k]) / abs(g[k])
-        print(f"64^3 200 steps dJ/d{name}: adjoint {g[k]:.12e} fd {fd:.12e} rel {rel:.2e}")
-        assert rel <= 1e-6, (name, rel)
+        print(f"64^3 200 steps dJ/d{name}: adjoint {g[k]:.12e} fd (h {h:.0e}) {fd:.12e} rel {rel:.2e}")
+        assert rel <= tol, (name, rel)
     print(f"J {J:.12e}; forward {info['forward_s']:.3f} s, backward {info['backward_s']:.3f} s")
 
 

@@ -268,14 +268,16 @@ def test_gpu_level_step_closed_form(case, gauge, mode):
     lv = A.Level([PatchGeom((n, n, n), bbox, GW)], gauge=gauge, eps=EPS, chitiny=CHITINY)
     try:
         lv.set_state(0, uniform_state(case, n))
-        want_ham = mode != 2  # graph replays run without residuals
-        assert lv.step(CASES[case]["dt"], 1, mode, want_ham=want_ham) == 0
+        assert lv.step(CASES[case]["dt"], 1, mode, want_ham=A.RES_LAST_STEP) == 0
         U = lv.get_state(0)
         exp, ham = expected_step(case, gauge)
         check_centre(U, exp, f"GPU level {case} gauge {gauge} mode {mode}")
-        if want_ham:
-            got = lv.ham_l2()
-            want = _ham_l2_expected(ham, n)
-            assert abs(got - want) <= 1e-13 * want + 1e-18, (got, want)
+        got = lv.ham_l2()
+        want = _ham_l2_expected(ham, n)
+        # the kernels' fused multiply-adds leave the second-derivative
+        # stencils of a constant field at rounding size rather than exactly
+        # zero (the Ham residual of the vacuum case reads 3.6e-15 per point,
+        # 8e-14 as the norm); a wrong term would be of the order of K^2
+        assert abs(got - want) <= 1e-13 * want + 1e-12, (got, want)
     finally:
         lv.close()
