@@ -115,6 +115,22 @@ void sommerfeld_routbam_(int *ex, double *X, double *Y, double *Z, double *xmin,
                          double *xmax, double *ymax, double *zmax, double *f_rhs, double *f0, double *velocity,
                          double *SoA, int *Symmetry);
 
+/* Replaces f_getnp4 (getnp4.h:97-105; getnp4.f90:10-452 with tetradtype 2 and
+ * ABV 0, called at bssn_class.C:6247): the Newman-Penrose Weyl scalar Psi4 =
+ * Rpsi4 + i Ipsi4 of the BSSN state in the radial tetrad, from the physical
+ * Christoffels and Ricci tensor compute_rhs_bssn_ returned for that state
+ * (read only).  The first derivatives of A_ij, chi and trK are fderivs'
+ * (diff_new.f90:946-1104): fourth order with +-2 room on every axis, second
+ * order with +-1, zero at the edges and on the last index of each axis.
+ * Returns 0, or 2 for a shape below 2 or Symmetry outside 0..2. */
+int getnp4_(int *ex, double *X, double *Y, double *Z, double *chi, double *trK, double *dxx, double *gxy,
+            double *gxz, double *dyy, double *gyz, double *dzz, double *Axx, double *Axy, double *Axz,
+            double *Ayy, double *Ayz, double *Azz, double *Gamxxx, double *Gamxxy, double *Gamxxz,
+            double *Gamxyy, double *Gamxyz, double *Gamxzz, double *Gamyxx, double *Gamyxy, double *Gamyxz,
+            double *Gamyyy, double *Gamyyz, double *Gamyzz, double *Gamzxx, double *Gamzxy, double *Gamzxz,
+            double *Gamzyy, double *Gamzyz, double *Gamzzz, double *Rxx, double *Rxy, double *Rxz,
+            double *Ryy, double *Ryz, double *Rzz, double *Rpsi4, double *Ipsi4, int *Symmetry);
+
 /* Replace f_prolong3 / f_restrict3 (prolongrestrict.h:26-37) with the
  * Cell, ghost_width 3 routines of prolongrestrict_cell.f90 (prolong3
  * :1915-2164, restrict3 :2330-2468): 6-point, 5th-order interpolation from a
@@ -175,6 +191,16 @@ int amss_restrict3_dev(const double *llbc, const double *uubc, const int *extc, 
 
 /* enforce_ga on device arrays (12 pointers: dxx gxy gxz dyy gyz dzz Axx..Azz). */
 int amss_enforce_ga_dev(const int *ex, double *const *ga, void *stream);
+
+/* getnp4_ on device arrays, ordered on stream (no synchronisation).
+ * h: dX, dY, dZ; fields: 14 device pointers chi trK dxx gxy gxz dyy gyz dzz
+ * Axx Axy Axz Ayy Ayz Azz; gam (18), ric (6): as compute_rhs_bssn_ wrote
+ * them; X, Y, Z: device coordinates (ex[0], ex[1], ex[2] values); rpsi,
+ * ipsi: outputs; origin: X[0], Y[0], Z[0] on the host (read only when
+ * symmetry != 0). */
+int amss_psi4_dev(const int *ex, const double *h, const double *const *fields, const double *const *gam,
+                  const double *const *ric, const double *X, const double *Y, const double *Z, double *rpsi,
+                  double *ipsi, int symmetry, const double *origin, void *stream);
 
 /* ---- levels: Blocks + the Step driver ------------------------------------ */
 typedef struct amss_level amss_level;

@@ -40,6 +40,9 @@ SIGNATURES = {
                                    C.c_void_p, C.c_int, _dp, C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p],
                                   C.c_int),
     "amss_enforce_ga_dev": ([_ip, C.c_void_p, C.c_void_p], C.c_int),
+    "getnp4_": ([_ip, _dp, _dp, _dp] + [_dp] * 40 + [_ip], C.c_int),
+    "amss_psi4_dev": ([_ip, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_void_p], C.c_int),
     "amss_level_create": ([C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
                           C.c_void_p),
     "amss_level_destroy": ([C.c_void_p], None),

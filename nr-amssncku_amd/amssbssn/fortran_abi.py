@@ -1,6 +1,6 @@
 """Calls of the library's Fortran-ABI replacements (compute_rhs_bssn_,
 rungekutta4_rout_, enforce_ga_, lowerboundset_, l2normhelper_,
-sommerfeld_rout_, sommerfeld_routbam_, prolong3_, restrict3_) on host numpy
+sommerfeld_rout_, sommerfeld_routbam_, prolong3_, restrict3_, getnp4_) on host numpy
 arrays, with the same Python signatures as the reference build's, so a
 caller switches by changing the library.
 Arrays: float64, shape (nz, ny, nx), C-contiguous.
@@ -96,6 +96,16 @@ def sommerfeld_bam(f_rhs, f0, X, Y, Z, bbox, velocity, soa, symmetry=0):
                               _d(velocity), sa, _i(symmetry))
 
 
+def getnp4(st, gam, ric, X, Y, Z, symmetry=0):
+    """getnp4_ (Psi4 in the radial tetrad) of the state st (24 or the first 14
+    arrays) from compute_rhs's gam (18) and ric (6); returns (Rpsi4, Ipsi4)."""
+    shape = st[0].shape
+    rp, ip = np.zeros(shape), np.zeros(shape)
+    check(lib().getnp4_(_ex(shape), _p(X), _p(Y), _p(Z), *[_p(a) for a in st[:14]], *[_p(a) for a in gam],
+                        *[_p(a) for a in ric], _p(rp), _p(ip), _i(symmetry)), "getnp4_")
+    return rp, ip
+
+
 def _d3(v):
     return (C.c_double * 3)(*[float(x) for x in v])
 
@@ -144,3 +154,29 @@ def compute_rhs_dev(state, h, eps, co=1, symmetry=0, origin=(0.0, 0.0, 0.0), gau
                                   C.c_void_p(s.cuda_stream)), "amss_rhs_bssn_dev")
     s.synchronize()
     return dict(flag=int(flag.item()), rhs=rhs, gam=gam, ric=ric, res=res)
+
+
+def psi4_dev(state, gam, ric, X, Y, Z, h, symmetry=0, origin=(0.0, 0.0, 0.0), stream=None):
+    """amss_psi4_dev on torch tensors: state (24 or 14, nz, ny, nx), gam
+    (18, ...), ric (6, ...) and the coordinate vectors X, Y, Z, all float64 on
+    the GPU.  Returns (rpsi, ipsi) after synchronising the stream."""
+    import torch
+    for t in (state, gam, ric, X, Y, Z):
+        assert t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+    shape = tuple(state.shape[1:])
+    assert state.shape[0] >= 14 and gam.shape == (18,) + shape and ric.shape == (6,) + shape
+    assert (X.numel(), Y.numel(), Z.numel()) == shape[::-1]
+    dev = state.device
+
+    def ptrs(t, n):
+        return (C.c_void_p * n)(*[t[i].data_ptr() for i in range(n)])
+
+    rpsi = torch.empty(shape, dtype=torch.float64, device=dev)
+    ipsi = torch.empty(shape, dtype=torch.float64, device=dev)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    vp = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    check(lib().amss_psi4_dev(_ex(shape), (C.c_double * 3)(*h), ptrs(state, 14), ptrs(gam, 18), ptrs(ric, 6),
+                              vp(X), vp(Y), vp(Z), vp(rpsi), vp(ipsi), symmetry, (C.c_double * 3)(*origin),
+                              C.c_void_p(s.cuda_stream)), "amss_psi4_dev")
+    s.synchronize()
+    return rpsi, ipsi

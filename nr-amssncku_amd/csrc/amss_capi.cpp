@@ -16,6 +16,7 @@
 #include "amss_adjoint.h"
 #include "amss_amr.h"
 #include "amss_host.h"
+#include "amss_psi4.h"
 #include "amss_xfer.h"
 
 namespace amss {
@@ -654,6 +655,102 @@ int amss_enforce_ga_dev(const int* ex, double* const* ga, void* stream) {
   launch_enforce(f, (long)ex[0] * ex[1] * ex[2], (hipStream_t)stream);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// ---- Psi4 ------------------------------------------------------------------
+// The state arrays getnp4 reads: chi, trK, dxx..dzz, Axx..Azz (14).
+static const int kPsi4Fields[14] = {CHI, TRK, DXX, DXX + 1, DXX + 2, DXX + 3, DXX + 4, DXX + 5,
+                                    AXX, AXX + 1, AXX + 2, AXX + 3, AXX + 4, AXX + 5};
+
+static int psi4_check(const int* ex, const double* h, int symmetry, const double* origin, const char* who,
+                      Box& box) {
+  if (!ex || !h || (symmetry != 0 && !origin)) return fail(AMSS_ERR_ARG, std::string(who) + ": null argument");
+  if (ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, std::string(who) + ": shape below 2");
+  if ((long)ex[0] * ex[1] * ex[2] >= (1L << 31))
+    return fail(AMSS_ERR_ARG, std::string(who) + ": more than 2^31 points per block");
+  box = make_box(ex, 0, 0);
+  return set_mirrors(box, symmetry, origin, h, who);
+}
+
+static int psi4_launch(const Box& box, const double* h, const double* const* fields, const double* const* gam,
+                       const double* const* ric, const double* X, const double* Y, const double* Z, double* rpsi,
+                       double* ipsi, hipStream_t s) {
+  FieldsIn st;
+  for (int v = 0; v < NFIELD; ++v) st.p[v] = nullptr;
+  for (int q = 0; q < 14; ++q) st.p[kPsi4Fields[q]] = fields[q];
+  Psi4Io io;
+  for (int q = 0; q < 18; ++q) io.gam[q] = gam[q];
+  for (int q = 0; q < 6; ++q) io.ric[q] = ric[q];
+  io.X = X;
+  io.Y = Y;
+  io.Z = Z;
+  io.rpsi = rpsi;
+  io.ipsi = ipsi;
+  launch_psi4(st, io, box, make_coef(h, 0.0), s);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int amss_psi4_dev(const int* ex, const double* h, const double* const* fields, const double* const* gam,
+                  const double* const* ric, const double* X, const double* Y, const double* Z, double* rpsi,
+                  double* ipsi, int symmetry, const double* origin, void* stream) {
+  if (!fields || !gam || !ric || !X || !Y || !Z || !rpsi || !ipsi)
+    return fail(AMSS_ERR_ARG, "amss_psi4_dev: null argument");
+  Box box;
+  if (int r = psi4_check(ex, h, symmetry, origin, "amss_psi4_dev", box)) return r;
+  return psi4_launch(box, h, fields, gam, ric, X, Y, Z, rpsi, ipsi, (hipStream_t)stream);
+}
+
+static int psi4_host(const int* ex, const double* X, const double* Y, const double* Z, const double* const* fields,
+                     const double* const* gam, const double* const* ric, double* rpsi, double* ipsi, int symmetry) {
+  if (!ex || !X || !Y || !Z) return fail(AMSS_ERR_ARG, "getnp4_: null argument");
+  if (ex[0] < 2 || ex[1] < 2 || ex[2] < 2) return fail(AMSS_ERR_ARG, "getnp4_: shape below 2");
+  const double h[3] = {X[1] - X[0], Y[1] - Y[0], Z[1] - Z[0]};  // getnp4.f90:72-74
+  const double origin[3] = {X[0], Y[0], Z[0]};
+  Box box;
+  if (int r = psi4_check(ex, h, symmetry, origin, "getnp4_", box)) return r;
+  const size_t n = (size_t)ex[0] * ex[1] * ex[2], bytes = n * sizeof(double);
+  const size_t nc = (size_t)ex[0] + ex[1] + ex[2];
+  std::lock_guard<std::mutex> lk(g_shim.mu);
+  if (int r = g_shim.ensure(40 * n + nc)) return r;
+  double* d = g_shim.buf;
+  const double *df[14], *dg[18], *dr[6];
+  for (int q = 0; q < 14; ++q, d += n) {
+    df[q] = d;
+    HIPCHK(hipMemcpyAsync(d, fields[q], bytes, hipMemcpyHostToDevice, g_shim.s));
+  }
+  for (int q = 0; q < 18; ++q, d += n) {
+    dg[q] = d;
+    HIPCHK(hipMemcpyAsync(d, gam[q], bytes, hipMemcpyHostToDevice, g_shim.s));
+  }
+  for (int q = 0; q < 6; ++q, d += n) {
+    dr[q] = d;
+    HIPCHK(hipMemcpyAsync(d, ric[q], bytes, hipMemcpyHostToDevice, g_shim.s));
+  }
+  double *drp = d, *dip = d + n, *dx = d + 2 * n, *dy = dx + ex[0], *dz = dy + ex[1];
+  HIPCHK(hipMemcpyAsync(dx, X, ex[0] * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dy, Y, ex[1] * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  HIPCHK(hipMemcpyAsync(dz, Z, ex[2] * sizeof(double), hipMemcpyHostToDevice, g_shim.s));
+  if (int r = psi4_launch(box, h, df, dg, dr, dx, dy, dz, drp, dip, g_shim.s)) return r;
+  HIPCHK(hipMemcpyAsync(rpsi, drp, bytes, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipMemcpyAsync(ipsi, dip, bytes, hipMemcpyDeviceToHost, g_shim.s));
+  HIPCHK(hipStreamSynchronize(g_shim.s));
+  return 0;
+}
+
+int getnp4_(int* ex, double* X, double* Y, double* Z, double* chi, double* trK, double* dxx, double* gxy,
+            double* gxz, double* dyy, double* gyz, double* dzz, double* Axx, double* Axy, double* Axz, double* Ayy,
+            double* Ayz, double* Azz, double* Gamxxx, double* Gamxxy, double* Gamxxz, double* Gamxyy,
+            double* Gamxyz, double* Gamxzz, double* Gamyxx, double* Gamyxy, double* Gamyxz, double* Gamyyy,
+            double* Gamyyz, double* Gamyzz, double* Gamzxx, double* Gamzxy, double* Gamzxz, double* Gamzyy,
+            double* Gamzyz, double* Gamzzz, double* Rxx, double* Rxy, double* Rxz, double* Ryy, double* Ryz,
+            double* Rzz, double* Rpsi4, double* Ipsi4, int* Symmetry) {
+  const double* fields[14] = {chi, trK, dxx, gxy, gxz, dyy, gyz, dzz, Axx, Axy, Axz, Ayy, Ayz, Azz};
+  const double* gam[18] = {Gamxxx, Gamxxy, Gamxxz, Gamxyy, Gamxyz, Gamxzz, Gamyxx, Gamyxy, Gamyxz,
+                           Gamyyy, Gamyyz, Gamyzz, Gamzxx, Gamzxy, Gamzxz, Gamzyy, Gamzyz, Gamzzz};
+  const double* ric[6] = {Rxx, Rxy, Rxz, Ryy, Ryz, Rzz};
+  if (!Symmetry) return fail(AMSS_ERR_ARG, "getnp4_: null argument");
+  return psi4_host(ex, X, Y, Z, fields, gam, ric, Rpsi4, Ipsi4, *Symmetry);
 }
 
 // ---- levels ----------------------------------------------------------------

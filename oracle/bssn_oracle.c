@@ -938,6 +938,179 @@ void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const 
     }
 }
 
+/* getnp4.f90:10-452 at every point, statement by statement (the Fortran's
+ * whole-array statements only combine values of one point, except the
+ * fderivs calls, which are taken first). */
+void orc_getnp4(const int *ex, const double *X, const double *Y, const double *Z, const double *chi,
+                const double *trK, const double *const *g, const double *const *A, const double *const *Gam,
+                const double *const *R, double *Rpsi4, double *Ipsi4, int symmetry)
+{
+    const size_t N = (size_t)ex[0] * ex[1] * ex[2];
+    const double SS[3] = {1.0, 1.0, 1.0};
+    /* parities of A_ij (getnp4.f90:343-348) */
+    const double PA[6][3] = {{1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}, {1, -1, -1}, {1, 1, 1}};
+    double *d = malloc(24 * N * sizeof(double));
+    double *dA[6][3], *dchi[3], *dK[3];
+    for (int s = 0; s < 6; s++)
+        for (int a = 0; a < 3; a++) dA[s][a] = d + (3 * s + a) * N;
+    for (int a = 0; a < 3; a++) { dchi[a] = d + (18 + a) * N; dK[a] = d + (21 + a) * N; }
+    for (int s = 0; s < 6; s++)
+        orc_fderivs(ex, A[s], dA[s][0], dA[s][1], dA[s][2], X, Y, Z, PA[s][0], PA[s][1], PA[s][2], symmetry);
+    orc_fderivs(ex, chi, dchi[0], dchi[1], dchi[2], X, Y, Z, SS[0], SS[1], SS[2], symmetry);
+    orc_fderivs(ex, trK, dK[0], dK[1], dK[2], X, Y, Z, SS[0], SS[1], SS[2], symmetry);
+    const double TINYRR = 1.e-14, F1o3 = 1.0 / 3.0, TWO = 2.0;
+    for (int k = 0; k < ex[2]; k++)
+        for (int j = 0; j < ex[1]; j++)
+            for (int i = 0; i < ex[0]; i++) {
+                const size_t p = IX(ex, i, j, k);
+                const double gxx = g[0][p] + 1.0, gxy = g[1][p], gxz = g[2][p];
+                const double gyy = g[3][p] + 1.0, gyz = g[4][p], gzz = g[5][p] + 1.0;
+                const double chipn1 = chi[p] + 1.0;
+                const double Axx = A[0][p], Axy = A[1][p], Axz = A[2][p], Ayy = A[3][p], Ayz = A[4][p],
+                             Azz = A[5][p];
+                const double tK = trK[p];
+                /* Christoffels in compute_rhs_bssn's argument order */
+                const double Gamxxx = Gam[0][p], Gamxxy = Gam[1][p], Gamxxz = Gam[2][p], Gamxyy = Gam[3][p],
+                             Gamxyz = Gam[4][p], Gamxzz = Gam[5][p], Gamyxx = Gam[6][p], Gamyxy = Gam[7][p],
+                             Gamyxz = Gam[8][p], Gamyyy = Gam[9][p], Gamyyz = Gam[10][p], Gamyzz = Gam[11][p],
+                             Gamzxx = Gam[12][p], Gamzxy = Gam[13][p], Gamzxz = Gam[14][p], Gamzyy = Gam[15][p],
+                             Gamzyz = Gam[16][p], Gamzzz = Gam[17][p];
+                /* invert the conformal metric (:97-105) */
+                double gupzz = gxx * gyy * gzz + gxy * gyz * gxz + gxz * gxy * gyz - gxz * gyy * gxz -
+                               gxy * gxy * gzz - gxx * gyz * gyz;
+                const double gupxx = (gyy * gzz - gyz * gyz) / gupzz;
+                const double gupxy = -(gxy * gzz - gyz * gxz) / gupzz;
+                const double gupxz = (gxy * gyz - gyy * gxz) / gupzz;
+                const double gupyy = (gxx * gzz - gxz * gxz) / gupzz;
+                const double gupyz = -(gxx * gyz - gxy * gxz) / gupzz;
+                gupzz = (gxx * gyy - gxy * gxy) / gupzz;
+                /* the radial tetrad (:259-341) */
+                double vx, vy, vz, ux, uy, uz, wx, wy, wz, fx, fy, fz, f;
+                if (fabs(X[i]) < TINYRR && fabs(Y[j]) < TINYRR && fabs(Z[k]) < TINYRR) {
+                    vx = TINYRR; vy = TINYRR; vz = TINYRR;
+                } else {
+                    vx = X[i]; vy = Y[j]; vz = Z[k];
+                }
+                if (fabs(X[i]) < TINYRR && fabs(Y[j]) < TINYRR) {
+                    ux = -TINYRR; uy = TINYRR; uz = 0.0;
+                    wx = TINYRR * Z[k]; wy = TINYRR * Z[k]; wz = -2 * TINYRR * TINYRR;
+                } else {
+                    ux = -Y[j]; uy = X[i]; uz = 0.0;
+                    wx = X[i] * Z[k]; wy = Y[j] * Z[k]; wz = -(X[i] * X[i] + Y[j] * Y[j]);
+                }
+                fx = vx; fy = vy; fz = vz;
+                vx = gupxx * fx + gupxy * fy + gupxz * fz;
+                vy = gupxy * fx + gupyy * fy + gupyz * fz;
+                vz = gupxz * fx + gupyz * fy + gupzz * fz;
+                f = 1.0 / chipn1;
+                fx = gxx * vx * vx + gyy * vy * vy + gzz * vz * vz + (gxy * vx * vy + gxz * vx * vz + gyz * vy * vz) * TWO;
+                fx = sqrt(fx * f);
+                vx = vx / fx; vy = vy / fx; vz = vz / fx;
+                fx = gxx * vx * ux + gxy * vx * uy + gxz * vx * uz + gxy * vy * ux + gyy * vy * uy + gyz * vy * uz +
+                     gxz * vz * ux + gyz * vz * uy + gzz * vz * uz;
+                fx = fx * f;
+                ux = ux - fx * vx; uy = uy - fx * vy; uz = uz - fx * vz;
+                fx = gxx * ux * ux + gyy * uy * uy + gzz * uz * uz + (gxy * ux * uy + gxz * ux * uz + gyz * uy * uz) * TWO;
+                fx = sqrt(fx * f);
+                ux = ux / fx; uy = uy / fx; uz = uz / fx;
+                fx = gxx * vx * wx + gxy * vx * wy + gxz * vx * wz + gxy * vy * wx + gyy * vy * wy + gyz * vy * wz +
+                     gxz * vz * wx + gyz * vz * wy + gzz * vz * wz;
+                fx = fx * f;
+                wx = wx - fx * vx; wy = wy - fx * vy; wz = wz - fx * vz;
+                fx = gxx * ux * wx + gxy * ux * wy + gxz * ux * wz + gxy * uy * wx + gyy * uy * wy + gyz * uy * wz +
+                     gxz * uz * wx + gyz * uz * wy + gzz * uz * wz;
+                fx = fx * f;
+                wx = wx - fx * ux; wy = wy - fx * uy; wz = wz - fx * uz;
+                fx = gxx * wx * wx + gyy * wy * wy + gzz * wz * wz + (gxy * wx * wy + gxz * wx * wz + gyz * wy * wz) * TWO;
+                fx = sqrt(fx * f);
+                wx = wx / fx; wy = wy / fx; wz = wz / fx;
+                /* D_k K_ij up to chi^-1 (:343-385) */
+                const double chix = dchi[0][p], chiy = dchi[1][p], chiz = dchi[2][p];
+                fx = dK[0][p]; fy = dK[1][p]; fz = dK[2][p];
+                double Axxx = dA[0][0][p], Axxy = dA[0][1][p], Axxz = dA[0][2][p];
+                double Axyx = dA[1][0][p], Axyy = dA[1][1][p], Axyz = dA[1][2][p];
+                double Axzx = dA[2][0][p], Axzy = dA[2][1][p], Axzz = dA[2][2][p];
+                double Ayyx = dA[3][0][p], Ayyy = dA[3][1][p], Ayyz = dA[3][2][p];
+                double Ayzx = dA[4][0][p], Ayzy = dA[4][1][p], Ayzz = dA[4][2][p];
+                double Azzx = dA[5][0][p], Azzy = dA[5][1][p], Azzz = dA[5][2][p];
+                Axxx = Axxx - (Gamxxx * Axx + Gamyxx * Axy + Gamzxx * Axz) * TWO - chix / chipn1 * Axx + F1o3 * gxx * fx;
+                Axxy = Axxy - (Gamxxy * Axx + Gamyxy * Axy + Gamzxy * Axz) * TWO - chiy / chipn1 * Axx + F1o3 * gxx * fy;
+                Axxz = Axxz - (Gamxxz * Axx + Gamyxz * Axy + Gamzxz * Axz) * TWO - chiz / chipn1 * Axx + F1o3 * gxx * fz;
+                Ayyx = Ayyx - (Gamxxy * Axy + Gamyxy * Ayy + Gamzxy * Ayz) * TWO - chix / chipn1 * Ayy + F1o3 * gyy * fx;
+                Ayyy = Ayyy - (Gamxyy * Axy + Gamyyy * Ayy + Gamzyy * Ayz) * TWO - chiy / chipn1 * Ayy + F1o3 * gyy * fy;
+                Ayyz = Ayyz - (Gamxyz * Axy + Gamyyz * Ayy + Gamzyz * Ayz) * TWO - chiz / chipn1 * Ayy + F1o3 * gyy * fz;
+                Azzx = Azzx - (Gamxxz * Axz + Gamyxz * Ayz + Gamzxz * Azz) * TWO - chix / chipn1 * Azz + F1o3 * gzz * fx;
+                Azzy = Azzy - (Gamxyz * Axz + Gamyyz * Ayz + Gamzyz * Azz) * TWO - chiy / chipn1 * Azz + F1o3 * gzz * fy;
+                Azzz = Azzz - (Gamxzz * Axz + Gamyzz * Ayz + Gamzzz * Azz) * TWO - chiz / chipn1 * Azz + F1o3 * gzz * fz;
+                Axyx = Axyx - (Gamxxy * Axx + Gamyxy * Axy + Gamzxy * Axz + Gamxxx * Axy + Gamyxx * Ayy + Gamzxx * Ayz) -
+                       chix / chipn1 * Axy + F1o3 * gxy * fx;
+                Axyy = Axyy - (Gamxyy * Axx + Gamyyy * Axy + Gamzyy * Axz + Gamxxy * Axy + Gamyxy * Ayy + Gamzxy * Ayz) -
+                       chiy / chipn1 * Axy + F1o3 * gxy * fy;
+                Axyz = Axyz - (Gamxyz * Axx + Gamyyz * Axy + Gamzyz * Axz + Gamxxz * Axy + Gamyxz * Ayy + Gamzxz * Ayz) -
+                       chiz / chipn1 * Axy + F1o3 * gxy * fz;
+                Axzx = Axzx - (Gamxxz * Axx + Gamyxz * Axy + Gamzxz * Axz + Gamxxx * Axz + Gamyxx * Ayz + Gamzxx * Azz) -
+                       chix / chipn1 * Axz + F1o3 * gxz * fx;
+                Axzy = Axzy - (Gamxyz * Axx + Gamyyz * Axy + Gamzyz * Axz + Gamxxy * Axz + Gamyxy * Ayz + Gamzxy * Azz) -
+                       chiy / chipn1 * Axz + F1o3 * gxz * fy;
+                Axzz = Axzz - (Gamxzz * Axx + Gamyzz * Axy + Gamzzz * Axz + Gamxxz * Axz + Gamyxz * Ayz + Gamzxz * Azz) -
+                       chiz / chipn1 * Axz + F1o3 * gxz * fz;
+                Ayzx = Ayzx - (Gamxxz * Axy + Gamyxz * Ayy + Gamzxz * Ayz + Gamxxy * Axz + Gamyxy * Ayz + Gamzxy * Azz) -
+                       chix / chipn1 * Ayz + F1o3 * gyz * fx;
+                Ayzy = Ayzy - (Gamxyz * Axy + Gamyyz * Ayy + Gamzyz * Ayz + Gamxyy * Axz + Gamyyy * Ayz + Gamzyy * Azz) -
+                       chiy / chipn1 * Ayz + F1o3 * gyz * fy;
+                Ayzz = Ayzz - (Gamxzz * Axy + Gamyzz * Ayy + Gamzzz * Ayz + Gamxyz * Axz + Gamyyz * Ayz + Gamzyz * Azz) -
+                       chiz / chipn1 * Ayz + F1o3 * gyz * fz;
+                /* B_ij = v^k (D_k K_ij - D_j K_ik), symmetrised (:386-392) */
+                const double Bxx = (vy * (Axxy - Axyx) + vz * (Axxz - Axzx)) * f;
+                const double Byy = (vx * (Ayyx - Axyy) + vz * (Ayyz - Ayzy)) * f;
+                const double Bzz = (vx * (Azzx - Axzz) + vy * (Azzy - Ayzz)) * f;
+                const double Bxy = (vx * (Axyx - (Axxy + Axyx) / TWO) + vy * (Axyy - Ayyx) / TWO +
+                                    vz * (Axyz - (Axzy + Ayzx) / TWO)) * f;
+                const double Bxz = (vx * (Axzx - (Axxz + Axzx) / TWO) + vy * (Axzy - (Axyz + Ayzx) / TWO) +
+                                    vz * (Axzz - Azzx) / TWO) * f;
+                const double Byz = (vx * (Ayzx - (Axyz + Axzy) / TWO) + vy * (Ayzy - (Ayyz + Ayzy) / TWO) +
+                                    vz * (Ayzz - Azzy) / TWO) * f;
+                /* K_ij up to chi^-1, E_ij = R_ij - K_ik K^k_j + K K_ij - B_ij (:393-440) */
+                Axxx = Axx + F1o3 * tK * gxx;
+                Axyx = Axy + F1o3 * tK * gxy;
+                Axzx = Axz + F1o3 * tK * gxz;
+                Ayyx = Ayy + F1o3 * tK * gyy;
+                Ayzx = Ayz + F1o3 * tK * gyz;
+                Azzx = Azz + F1o3 * tK * gzz;
+                double Exx = gupxx * Axxx * Axxx + gupyy * Axyx * Axyx + gupzz * Axzx * Axzx +
+                             TWO * (gupxy * Axxx * Axyx + gupxz * Axxx * Axzx + gupyz * Axyx * Axzx);
+                double Eyy = gupxx * Axyx * Axyx + gupyy * Ayyx * Ayyx + gupzz * Ayzx * Ayzx +
+                             TWO * (gupxy * Axyx * Ayyx + gupxz * Axyx * Ayzx + gupyz * Ayyx * Ayzx);
+                double Ezz = gupxx * Axzx * Axzx + gupyy * Ayzx * Ayzx + gupzz * Azzx * Azzx +
+                             TWO * (gupxy * Axzx * Ayzx + gupxz * Axzx * Azzx + gupyz * Ayzx * Azzx);
+                double Exy = gupxx * Axxx * Axyx + gupyy * Axyx * Ayyx + gupzz * Axzx * Ayzx +
+                             gupxy * (Axxx * Ayyx + Axyx * Axyx) + gupxz * (Axxx * Ayzx + Axzx * Axyx) +
+                             gupyz * (Axyx * Ayzx + Axzx * Ayyx);
+                double Exz = gupxx * Axxx * Axzx + gupyy * Axyx * Ayzx + gupzz * Axzx * Azzx +
+                             gupxy * (Axxx * Ayzx + Axyx * Axzx) + gupxz * (Axxx * Azzx + Axzx * Axzx) +
+                             gupyz * (Axyx * Azzx + Axzx * Ayzx);
+                double Eyz = gupxx * Axyx * Axzx + gupyy * Ayyx * Ayzx + gupzz * Ayzx * Azzx +
+                             gupxy * (Axyx * Ayzx + Ayyx * Axzx) + gupxz * (Axyx * Azzx + Ayzx * Axzx) +
+                             gupyz * (Ayyx * Azzx + Ayzx * Ayzx);
+                Exx = R[0][p] - (Exx - Axxx * tK) * f - Bxx;
+                Exy = R[1][p] - (Exy - Axyx * tK) * f - Bxy;
+                Exz = R[2][p] - (Exz - Axzx * tK) * f - Bxz;
+                Eyy = R[3][p] - (Eyy - Ayyx * tK) * f - Byy;
+                Eyz = R[4][p] - (Eyz - Ayzx * tK) * f - Byz;
+                Ezz = R[5][p] - (Ezz - Azzx * tK) * f - Bzz;
+                /* m = (u - i w) / sqrt(2) (:441-452) */
+                const double uuwwxx = ux * ux - wx * wx, uuwwxy = ux * uy - wx * wy, uuwwxz = ux * uz - wx * wz;
+                const double uuwwyy = uy * uy - wy * wy, uuwwyz = uy * uz - wy * wz, uuwwzz = uz * uz - wz * wz;
+                const double uwxx = ux * wx + wx * ux, uwxy = ux * wy + wx * uy, uwxz = ux * wz + wx * uz;
+                const double uwyy = uy * wy + wy * uy, uwyz = uy * wz + wy * uz, uwzz = uz * wz + wz * uz;
+                double rp = Exx * uuwwxx + Eyy * uuwwyy + Ezz * uuwwzz + (Exy * uuwwxy + Exz * uuwwxz + Eyz * uuwwyz) * TWO;
+                double ip = Exx * uwxx + Eyy * uwyy + Ezz * uwzz + (Exy * uwxy + Exz * uwxz + Eyz * uwyz) * TWO;
+                Rpsi4[p] = -rp / TWO;
+                Ipsi4[p] = -ip / TWO;
+            }
+    free(d);
+}
+
 /* polint (fmisc.f90:1121-1179): Neville's tableau on xa(1..n), ya(1..n)
  * (1-based arrays of n + 1 entries), starting from the sample nearest x. */
 static double neville(const double *xa, const double *ya, double x, int n)

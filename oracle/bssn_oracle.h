@@ -62,6 +62,13 @@ double orc_l2normhelper(const int *ex, const double *X, const double *Y, const d
 /* sommerfeld_rout.f90:9-221, precor == 1 (copy) branch */
 void orc_sommerfeld_copy(const int *ex, const double *X, const double *Y, const double *Z,
                          const double *bbox, const double *f0, double *f, int symmetry);
+/* getnp4.f90:10-452 (Cartesian, tetradtype 2 and ABV 0 of the reference's
+ * macrodef.fh): the Newman-Penrose Weyl scalar Psi4 = -(E - B)_ij m^i m^j
+ * with the radial tetrad, from the state, the physical Christoffels (18, in
+ * compute_rhs_bssn's order) and Ricci (6) of the same state. */
+void orc_getnp4(const int *ex, const double *X, const double *Y, const double *Z, const double *chi,
+                const double *trK, const double *const *g, const double *const *A, const double *const *Gam,
+                const double *const *R, double *Rpsi4, double *Ipsi4, int symmetry);
 /* sommerfeld_rout.f90:9-121, 141-217: precor == 0 (the interpolating
  * predictor): on the outermost layer of each physical face of bbox,
  * f = (1 - r) f0(x - r x), r = alpha sqrt(chi) dT / |x| (alpha, chi from

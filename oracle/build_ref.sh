@@ -1,5 +1,6 @@
 #!/usr/bin/env bash
-# Builds the reference AMSS-NCKU Fortran of the BSSN RHS + RK4 path into
+# Builds the reference AMSS-NCKU Fortran of the BSSN RHS + RK4 path (and the
+# Psi4 analysis routine getnp4) into
 # oracle/_ref/libamssref.so (parity build: -O2 -ffp-contract=off) and
 # oracle/_ref/libamssref_o3.so (timing build: -O3), straight from the
 # reference sources where they lie.  Test infrastructure only.
@@ -29,7 +30,9 @@ trap 'rm -rf "$TMP"' EXIT
 # fmisc.f90 minus fwritefile; the include of macrodef.fh is resolved with -I.
 awk '/^ *subroutine fwritefile\(/{skip=1} !skip{print} /^ *end subroutine fwritefile/{skip=0}' \
     "$REF/fmisc.f90" > "$TMP/fmisc_nofwrite.f90"
-SRCS="bssn_rhs diff_new lopsidediff kodiss enforce_algebra rungekutta4_rout sommerfeld_rout prolongrestrict_cell"
+# getnp4 (the Newman-Penrose Psi4) also holds the shell variant, which calls
+# diff_new_sh's fderivs_shc: both are built so that the library loads whole
+SRCS="bssn_rhs diff_new lopsidediff kodiss enforce_algebra rungekutta4_rout sommerfeld_rout prolongrestrict_cell getnp4 diff_new_sh"
 build() {  # $1 = suffix, $2.. = flags
   local sfx=$1; shift
   local objs=()

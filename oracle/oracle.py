@@ -148,6 +148,16 @@ class Orc:
         return self.lib.orc_sommerfeld_interp(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), bb, C.c_double(dT),
                                               _ptr(chi0), _ptr(lap0), _ptr(f0), _ptr(f), sa, C.c_int(symmetry))
 
+    def getnp4(self, st, gam, ric, X, Y, Z, symmetry=0):
+        """Psi4 (getnp4, tetradtype 2) of state st from compute_rhs's gam and
+        ric; returns (Rpsi4, Ipsi4)."""
+        shape = st[0].shape
+        rp, ip = np.zeros(shape), np.zeros(shape)
+        self.lib.orc_getnp4(_ex(shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(st[0]), _ptr(st[1]),
+                            _arr_ptrs(st[2:8]), _arr_ptrs(st[8:14]), _arr_ptrs(gam), _arr_ptrs(ric), _ptr(rp),
+                            _ptr(ip), C.c_int(symmetry))
+        return rp, ip
+
     def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
         """prolong3 into funf in place; returns the status (0 ok, 1 bad region)."""
         d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
@@ -281,6 +291,13 @@ class Ref:
                                   self._d(dT), _ptr(chi0), _ptr(lap0), _ptr(f0), _ptr(f), sa,
                                   self._i(symmetry), self._i(0))
         return 0
+
+    def getnp4(self, st, gam, ric, X, Y, Z, symmetry=0):
+        shape = st[0].shape
+        rp, ip = np.zeros(shape), np.zeros(shape)
+        self.lib.getnp4_(_ex(shape), _ptr(X), _ptr(Y), _ptr(Z), *[_ptr(a) for a in st[:14]],
+                         *[_ptr(a) for a in gam], *[_ptr(a) for a in ric], _ptr(rp), _ptr(ip), self._i(symmetry))
+        return rp, ip
 
     def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
         """prolong3_ (a subroutine: no status) into funf in place."""
