@@ -336,6 +336,21 @@ int amss_level_checkpoint_write(amss_level *L, const char *path, double time, in
  * (may be NULL when maxbh is 0; more punctures than maxbh is an error). */
 int amss_level_checkpoint_read(amss_level *L, const char *path, double *time, int *movls, int *nbh, double *porg,
                                int maxbh);
+/* The hierarchy as a multi-level checkpoint (checkpoint.C:132-191, every
+ * level): time (PhysTime), levels, movls, BH_num_in = nbh, grids[levels],
+ * Lt[levels] (each level's time), then per level its patch table (bbox,
+ * shape, handle: handles holds 3 doubles per patch, level after level, or is
+ * NULL for zeros) and its nbh puncture positions (porg: 3 doubles per
+ * puncture, level after level), then per level and patch the 24 StateList
+ * and 24 OldStateList grid functions.  Synchronous. */
+int amss_amr_checkpoint_write(amss_amr *A, const char *path, int movls, int nbh, const double *porg,
+                              const double *handles);
+/* Restore such a file into a hierarchy of the same levels, patch shapes and
+ * bboxes (checkpoint.C:193-292): both lists of every patch, PhysTime and each
+ * level's time (the start time, which chooses between two- and three-level
+ * time interpolation, is kept, as bssn_class.C:699-731 keeps StartTime).
+ * porg: room for 3 * maxbh doubles per level. */
+int amss_amr_checkpoint_read(amss_amr *A, const char *path, int *movls, int *nbh, double *porg, int maxbh);
 /* Interior kernel of the fused RHS + RK4 sub-step: 0 reads every stencil
  * point through the vector L1/L2 (raw buffer loads); 1 stages a halo tile of
  * each field group in LDS (8x8x4-point tiles, double-buffered, filled by

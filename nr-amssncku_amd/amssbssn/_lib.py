@@ -62,6 +62,9 @@ SIGNATURES = {
     "amss_level_checkpoint_write": ([C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, _dp, _dp], C.c_int),
     "amss_level_checkpoint_read": ([C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp,
                                    C.c_int], C.c_int),
+    "amss_amr_checkpoint_write": ([C.c_void_p, C.c_char_p, C.c_int, C.c_int, _dp, _dp], C.c_int),
+    "amss_amr_checkpoint_read": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, C.c_int],
+                                 C.c_int),
     "amss_level_timing": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_set_variant": ([C.c_void_p, C.c_int], C.c_int),
     "amss_level_variant": ([C.c_void_p], C.c_int),

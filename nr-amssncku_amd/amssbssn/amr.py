@@ -52,6 +52,33 @@ class Hierarchy:
     def time(self) -> float:
         return lib().amss_amr_time(self.h)
 
+    def checkpoint_write(self, path: str, movls: int = 0, porg=(), handles=None) -> None:
+        """Every level's StateList and OldStateList in checkpoint::writecheck_cgh's
+        layout; porg: per level a list of puncture positions (the same count on
+        every level), handles: 3 per patch, level after level."""
+        nbh = len(porg[0]) if porg else 0
+        assert not porg or (len(porg) == len(self.levels) and all(len(p) == nbh for p in porg))
+        pts = [x for lev in porg for p in lev for x in p]
+        pa = (C.c_double * max(len(pts), 1))(*pts)
+        hd = None
+        if handles is not None:
+            hl = [x for p in handles for x in p]
+            hd = (C.c_double * len(hl))(*hl)
+        check(lib().amss_amr_checkpoint_write(self.h, path.encode(), movls, nbh, pa, hd),
+              "amss_amr_checkpoint_write")
+
+    def checkpoint_read(self, path: str, maxbh: int = 16):
+        """Restore both lists of every level and the times; returns (time,
+        movls, porg per level)."""
+        mv, nb = C.c_int(0), C.c_int(0)
+        nl = len(self.levels)
+        pa = (C.c_double * (3 * max(maxbh, 1) * nl))()
+        check(lib().amss_amr_checkpoint_read(self.h, path.encode(), C.byref(mv), C.byref(nb), pa, maxbh),
+              "amss_amr_checkpoint_read")
+        n = nb.value
+        porg = [[tuple(pa[3 * (l * n + b):3 * (l * n + b) + 3]) for b in range(n)] for l in range(nl)]
+        return self.time, mv.value, porg
+
     def close(self):
         if getattr(self, "h", None):
             for lv in self.levels:

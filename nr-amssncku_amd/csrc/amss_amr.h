@@ -36,6 +36,13 @@ class AmrHierarchy {
   // want_res as BssnLevel::step, for every level step
   int step(double dT, int nsteps, int mode, int want_res);
   double time() const { return phys_time_; }
+  double level_time(int l) const { return lt_[l]; }
+  // after a restart (checkpoint::readcheck_cgh): PhysTime and each level's
+  // time; the start time keeps its value (bssn_class.C:699-731)
+  void restore_time(double t, const double* lt) {
+    phys_time_ = t;
+    for (size_t l = 0; l < lt_.size(); ++l) lt_[l] = lt[l];
+  }
 
   static constexpr int kBufferWidth = 6;
 

@@ -907,6 +907,64 @@ template <class T>
 bool put(std::FILE* f, const T* p, size_t n) { return std::fwrite(p, sizeof(T), n, f) == n; }
 template <class T>
 bool get(std::FILE* f, T* p, size_t n) { return std::fread(p, sizeof(T), n, f) == n; }
+
+// One level's patch table (checkpoint.C:157-162): bbox, shape, handle
+// (handles: 3 per patch, or NULL for zeros).
+bool put_patch_table(std::FILE* f, BssnLevel* L, const double* handles) {
+  bool ok = true;
+  for (int p = 0; ok && p < L->npatch(); ++p) {
+    const Patch& P = L->patch(p);
+    const double zero[3] = {0.0, 0.0, 0.0};
+    ok = put(f, P.bbox, 6) && put(f, P.shape, 3) && put(f, handles ? handles + 3 * p : zero, 3);
+  }
+  return ok;
+}
+// Reads a patch table and checks it against the level's patches.
+int check_patch_table(std::FILE* f, BssnLevel* L, const std::string& bad) {
+  for (int p = 0; p < L->npatch(); ++p) {
+    double bb[6], hd[3];
+    int sh[3];
+    if (!get(f, bb, 6) || !get(f, sh, 3) || !get(f, hd, 3)) return fail(AMSS_ERR_ARG, bad + "short patch table");
+    const Patch& P = L->patch(p);
+    for (int a = 0; a < 3; ++a)
+      if (sh[a] != P.shape[a]) return fail(AMSS_ERR_ARG, bad + "patch shape differs from the level's");
+    for (int a = 0; a < 6; ++a)
+      if (std::fabs(bb[a] - P.bbox[a]) > 1e-12 * std::max(1.0, std::fabs(P.bbox[a])))
+        return fail(AMSS_ERR_ARG, bad + "patch bbox differs from the level's");
+  }
+  return 0;
+}
+// One level's data (checkpoint.C:171-186): per patch the CheckList, i.e.
+// the 24 StateList then the 24 OldStateList grid functions
+// (bssn_class.C:682-683).
+int put_lists(std::FILE* f, BssnLevel* L, const std::string& bad) {
+  HIPCHK(hipStreamSynchronize(L->stream()));
+  std::vector<double> h;
+  for (int p = 0; p < L->npatch(); ++p) {
+    const size_t n = (size_t)L->npts(p);
+    h.resize(n);
+    for (int list : {AMSS_LIST_STATE, AMSS_LIST_OLD})
+      for (int v = 0; v < 24; ++v) {
+        HIPCHK(hipMemcpy(h.data(), L->field(p, list, v), n * sizeof(double), hipMemcpyDeviceToHost));
+        if (!put(f, h.data(), n)) return fail(AMSS_ERR_ARG, bad + "short write");
+      }
+  }
+  return 0;
+}
+int get_lists(std::FILE* f, BssnLevel* L, const std::string& bad) {
+  HIPCHK(hipStreamSynchronize(L->stream()));
+  std::vector<double> h;
+  for (int p = 0; p < L->npatch(); ++p) {
+    const size_t n = (size_t)L->npts(p);
+    h.resize(n);
+    for (int list : {AMSS_LIST_STATE, AMSS_LIST_OLD})
+      for (int v = 0; v < 24; ++v) {
+        if (!get(f, h.data(), n)) return fail(AMSS_ERR_ARG, bad + "short data");
+        HIPCHK(hipMemcpy(L->field(p, list, v), h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+      }
+  }
+  return 0;
+}
 }  // namespace
 extern "C" {
 
@@ -936,30 +994,14 @@ int amss_level_checkpoint_write(amss_level* L, const char* path, double time, in
   File F;
   F.f = std::fopen(path, "wb");
   if (!F.f) return fail(AMSS_ERR_ARG, std::string("amss_level_checkpoint_write: cannot open ") + path);
+  const std::string bad = std::string("amss_level_checkpoint_write: ") + path + ": ";
   const int levels = 1;
   // header (checkpoint.C:150-168): one level
   bool ok = put(F.f, &time, 1) && put(F.f, &levels, 1) && put(F.f, &movls, 1) && put(F.f, &nbh, 1) &&
-            put(F.f, &np, 1) && put(F.f, &time, 1);
-  for (int p = 0; ok && p < np; ++p) {
-    const Patch& P = L->L->patch(p);
-    const double zero[3] = {0.0, 0.0, 0.0};
-    ok = put(F.f, P.bbox, 6) && put(F.f, P.shape, 3) && put(F.f, handles ? handles + 3 * p : zero, 3);
-  }
+            put(F.f, &np, 1) && put(F.f, &time, 1) && put_patch_table(F.f, L->L, handles);
   for (int b = 0; ok && b < nbh; ++b) ok = put(F.f, porg + 3 * b, 3);
-  // data (checkpoint.C:171-186): per patch, StateList then OldStateList
-  HIPCHK(hipStreamSynchronize(L->L->stream()));
-  std::vector<double> h;
-  for (int p = 0; ok && p < np; ++p) {
-    const size_t n = (size_t)L->L->npts(p);
-    h.resize(n);
-    for (int list : {AMSS_LIST_STATE, AMSS_LIST_OLD})
-      for (int v = 0; ok && v < 24; ++v) {
-        HIPCHK(hipMemcpy(h.data(), L->L->field(p, list, v), n * sizeof(double), hipMemcpyDeviceToHost));
-        ok = put(F.f, h.data(), n);
-      }
-  }
-  if (!ok) return fail(AMSS_ERR_ARG, std::string("amss_level_checkpoint_write: short write to ") + path);
-  return 0;
+  if (!ok) return fail(AMSS_ERR_ARG, bad + "short write");
+  return put_lists(F.f, L->L, bad);
 }
 
 int amss_level_checkpoint_read(amss_level* L, const char* path, double* time, int* movls, int* nbh, double* porg,
@@ -976,33 +1018,73 @@ int amss_level_checkpoint_read(amss_level* L, const char* path, double* time, in
     return fail(AMSS_ERR_ARG, bad + "short header");
   if (levels != 1) return fail(AMSS_ERR_ARG, bad + "not a one-level checkpoint");
   if (*nbh < 0 || *nbh > maxbh) return fail(AMSS_ERR_ARG, bad + "more punctures than maxbh");
-  const int np = L->L->npatch();
   if (!get(F.f, &grids, 1) || !get(F.f, &lt, 1)) return fail(AMSS_ERR_ARG, bad + "short header");
-  if (grids != np) return fail(AMSS_ERR_ARG, bad + "patch count differs from the level's");
-  for (int p = 0; p < np; ++p) {
-    double bb[6], hd[3];
-    int sh[3];
-    if (!get(F.f, bb, 6) || !get(F.f, sh, 3) || !get(F.f, hd, 3)) return fail(AMSS_ERR_ARG, bad + "short patch table");
-    const Patch& P = L->L->patch(p);
-    for (int a = 0; a < 3; ++a)
-      if (sh[a] != P.shape[a]) return fail(AMSS_ERR_ARG, bad + "patch shape differs from the level's");
-    for (int a = 0; a < 6; ++a)
-      if (std::fabs(bb[a] - P.bbox[a]) > 1e-12 * std::max(1.0, std::fabs(P.bbox[a])))
-        return fail(AMSS_ERR_ARG, bad + "patch bbox differs from the level's");
-  }
+  if (grids != L->L->npatch()) return fail(AMSS_ERR_ARG, bad + "patch count differs from the level's");
+  if (int r = check_patch_table(F.f, L->L, bad)) return r;
   for (int b = 0; b < *nbh; ++b)
     if (!get(F.f, porg + 3 * b, 3)) return fail(AMSS_ERR_ARG, bad + "short puncture table");
-  HIPCHK(hipStreamSynchronize(L->L->stream()));
-  std::vector<double> h;
-  for (int p = 0; p < np; ++p) {
-    const size_t n = (size_t)L->L->npts(p);
-    h.resize(n);
-    for (int list : {AMSS_LIST_STATE, AMSS_LIST_OLD})
-      for (int v = 0; v < 24; ++v) {
-        if (!get(F.f, h.data(), n)) return fail(AMSS_ERR_ARG, bad + "short data");
-        HIPCHK(hipMemcpy(L->L->field(p, list, v), h.data(), n * sizeof(double), hipMemcpyHostToDevice));
-      }
+  return get_lists(F.f, L->L, bad);
+}
+
+int amss_amr_checkpoint_write(amss_amr* A, const char* path, int movls, int nbh, const double* porg,
+                              const double* handles) {
+  if (!A || !path || nbh < 0 || (nbh > 0 && !porg)) return fail(AMSS_ERR_ARG, "amss_amr_checkpoint_write: bad arguments");
+  AmrHierarchy& H = *A->H;
+  const int levels = H.nlev();
+  File F;
+  F.f = std::fopen(path, "wb");
+  if (!F.f) return fail(AMSS_ERR_ARG, std::string("amss_amr_checkpoint_write: cannot open ") + path);
+  const std::string bad = std::string("amss_amr_checkpoint_write: ") + path + ": ";
+  // header (checkpoint.C:150-155): time, levels, movls, BH_num_in, grids[], Lt[]
+  const double time = H.time();
+  std::vector<int> grids(levels);
+  std::vector<double> lt(levels);
+  for (int l = 0; l < levels; ++l) {
+    grids[l] = H.level(l)->npatch();
+    lt[l] = H.level_time(l);
   }
+  bool ok = put(F.f, &time, 1) && put(F.f, &levels, 1) && put(F.f, &movls, 1) && put(F.f, &nbh, 1) &&
+            put(F.f, grids.data(), levels) && put(F.f, lt.data(), levels);
+  // per level: the patch table, then the level's puncture positions (:156-167)
+  int first = 0;
+  for (int l = 0; ok && l < levels; ++l) {
+    ok = put_patch_table(F.f, H.level(l), handles ? handles + 3 * first : nullptr);
+    first += grids[l];
+    for (int b = 0; ok && b < nbh; ++b) ok = put(F.f, porg + 3 * (l * nbh + b), 3);
+  }
+  if (!ok) return fail(AMSS_ERR_ARG, bad + "short write");
+  for (int l = 0; l < levels; ++l)
+    if (int r = put_lists(F.f, H.level(l), bad)) return r;
+  return 0;
+}
+
+int amss_amr_checkpoint_read(amss_amr* A, const char* path, int* movls, int* nbh, double* porg, int maxbh) {
+  if (!A || !path || !movls || !nbh || maxbh < 0 || (maxbh > 0 && !porg))
+    return fail(AMSS_ERR_ARG, "amss_amr_checkpoint_read: bad arguments");
+  AmrHierarchy& H = *A->H;
+  const int levels = H.nlev();
+  File F;
+  F.f = std::fopen(path, "rb");
+  if (!F.f) return fail(AMSS_ERR_ARG, std::string("amss_amr_checkpoint_read: cannot open ") + path);
+  const std::string bad = std::string("amss_amr_checkpoint_read: ") + path + ": ";
+  double time = 0.0;
+  int nl = 0;
+  if (!get(F.f, &time, 1) || !get(F.f, &nl, 1) || !get(F.f, movls, 1) || !get(F.f, nbh, 1))
+    return fail(AMSS_ERR_ARG, bad + "short header");
+  if (nl != levels) return fail(AMSS_ERR_ARG, bad + "level count differs from the hierarchy's");
+  if (*nbh < 0 || *nbh > maxbh) return fail(AMSS_ERR_ARG, bad + "more punctures than maxbh");
+  std::vector<int> grids(levels);
+  std::vector<double> lt(levels);
+  if (!get(F.f, grids.data(), levels) || !get(F.f, lt.data(), levels)) return fail(AMSS_ERR_ARG, bad + "short header");
+  for (int l = 0; l < levels; ++l) {
+    if (grids[l] != H.level(l)->npatch()) return fail(AMSS_ERR_ARG, bad + "patch count differs from the hierarchy's");
+    if (int r = check_patch_table(F.f, H.level(l), bad)) return r;
+    for (int b = 0; b < *nbh; ++b)
+      if (!get(F.f, porg + 3 * (l * *nbh + b), 3)) return fail(AMSS_ERR_ARG, bad + "short puncture table");
+  }
+  for (int l = 0; l < levels; ++l)
+    if (int r = get_lists(F.f, H.level(l), bad)) return r;
+  H.restore_time(time, lt.data());
   return 0;
 }
 

@@ -136,3 +136,72 @@ def test_hierarchy_refuses_a_fine_patch_too_close_to_the_coarse_edge():
     fine = A.nested_box(coarse, (0, 4, 4), (8, 12, 12))  # its buffer reaches the coarse ghosts
     with pytest.raises(A.AmssError, match="buffer zone"):
         A.Hierarchy([[coarse], [fine]])
+
+
+def test_two_level_checkpoint_restart_continues_bitwise(tmp_path):
+    """checkpoint::writecheck_cgh's layout over both levels: a hierarchy read
+    back from it continues exactly as the one that wrote it (two coarse steps
+    in, so the next steps interpolate the coarse level over three time
+    levels from the restored OldStateList)."""
+    import struct
+    import amssbssn as A
+    coarse, fine, cc, cf = _setup()
+    dT = cc.dt
+    porg = [[(1.5, 0.3, 0.0), (-1.5, -0.2, 0.1)], [(1.5, 0.3, 0.0), (-1.5, -0.2, 0.1)]]
+    path = str(tmp_path / "amr_cgh.CHK")
+
+    def make():
+        return A.Hierarchy([[coarse], [fine]], gauge=0, eps=0.02)
+
+    H = make()
+    try:
+        H.level(0).set_state(0, cc.initial_state(0))
+        H.level(1).set_state(0, cf.initial_state(0))
+        assert H.step(dT, 2) == 0
+        H.checkpoint_write(path, movls=1, porg=porg)
+        assert H.step(dT, 2) == 0
+        want = [[H.level(l).get_field(0, lst, v) for lst in (0, 1) for v in range(NF)] for l in (0, 1)]
+        t_want = H.time
+    finally:
+        H.close()
+    with open(path, "rb") as f:
+        raw = f.read()
+    t, levels, movls, nbh = struct.unpack_from("<diii", raw, 0)
+    grids = struct.unpack_from("<2i", raw, 20)
+    lt = struct.unpack_from("<2d", raw, 28)
+    assert (levels, movls, nbh, grids) == (2, 1, 2, (1, 1))
+    assert t == pytest.approx(2 * dT) and lt == pytest.approx((2 * dT, 2 * dT))
+    table = 2 * (6 * 8 + 3 * 4 + 3 * 8 + 2 * 3 * 8)
+    data = 2 * NF * 8 * (int(np.prod(coarse.shape)) + int(np.prod(fine.shape)))
+    assert len(raw) == 44 + table + data
+    H2 = make()
+    try:
+        t2, mv, pg = H2.checkpoint_read(path)
+        assert t2 == pytest.approx(2 * dT) and mv == 1
+        assert np.allclose(pg, porg, rtol=0, atol=0)
+        assert H2.step(dT, 2) == 0
+        got = [[H2.level(l).get_field(0, lst, v) for lst in (0, 1) for v in range(NF)] for l in (0, 1)]
+        assert H2.time == t_want
+    finally:
+        H2.close()
+    for l in (0, 1):
+        for i, (a, b) in enumerate(zip(got[l], want[l])):
+            assert np.array_equal(a, b), (l, i)
+
+
+def test_checkpoint_of_another_hierarchy_is_refused(tmp_path):
+    import amssbssn as A
+    coarse, fine, cc, cf = _setup()
+    path = str(tmp_path / "amr_cgh.CHK")
+    H = A.Hierarchy([[coarse], [fine]], gauge=0, eps=0.02)
+    try:
+        H.checkpoint_write(path)
+    finally:
+        H.close()
+    other = A.nested_box(coarse, (4, 6, 6), (16, 18, 18))
+    H2 = A.Hierarchy([[coarse], [other]], gauge=0, eps=0.02)
+    try:
+        with pytest.raises(A.AmssError, match="bbox differs"):
+            H2.checkpoint_read(path)
+    finally:
+        H2.close()
