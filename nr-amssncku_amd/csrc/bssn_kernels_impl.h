@@ -148,32 +148,34 @@ AMSS_POINT_KERNEL rk4_interior_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, F
   S.finish();
 }
 
-// LDS-staged variant: 8x8x4-point tiles, halo tiles of every field group in
-// LDS (double-buffered, filled by direct global->LDS loads).  A 1-D grid of
+// LDS-staged variant: 8x8xTZ-point tiles (TZ = 8 where the box holds whole
+// 8-deep tiles, else 4), halo tiles of every field group in LDS
+// (double-buffered, filled by direct global->LDS loads).  A 1-D grid of
 // tiles in XCD-contiguous order: consecutive tiles, whose halos overlap, run
-// behind the same L2.
+// behind the same L2.  The 8-deep tile (512 threads, one workgroup per CU)
+// re-reads 14/8 instead of 10/4 z-planes of halo per interior plane.
 constexpr int kLdsTX = 8, kLdsTY = 8, kLdsTZ = 4;
 #ifndef AMSS_LDS_WAVES
 #define AMSS_LDS_WAVES 2
 #endif
-template <int GAUGE, int STAGE, bool CO>
-__global__ void __launch_bounds__(kRhsThreads) __attribute__((amdgpu_waves_per_eu(AMSS_LDS_WAVES, AMSS_LDS_WAVES)))
+template <int GAUGE, int STAGE, bool CO, int TZ>
+__global__ void __launch_bounds__(kLdsTX * kLdsTY * TZ) __attribute__((amdgpu_waves_per_eu(AMSS_LDS_WAVES, AMSS_LDS_WAVES)))
 rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
                                  Coef C, Rk4Param R, int* flag) {
-  using Tl = LdsTile<kLdsTX, kLdsTY, kLdsTZ>;
+  using Tl = LdsTile<kLdsTX, kLdsTY, TZ>;
   __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
   const int tx = threadIdx.x % kLdsTX, ty = tile_row((threadIdx.x / kLdsTX) % kLdsTY, kLdsTY),
             tz = threadIdx.x / (kLdsTX * kLdsTY);
   const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
   const int b = xcd_spread(blockIdx.x, gridDim.x);
   const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;
-  const int k0 = box.lo[2] + (b / (gx * gy)) * kLdsTZ;
+  const int k0 = box.lo[2] + (b / (gx * gy)) * TZ;
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   // the launch covers whole tiles only (remainder slabs go to the direct kernel)
   const long p = i + j * box.sy + k * box.sz;
   const int N = box.n[0] * box.n[1] * box.n[2];
   lds_double* t0 = (lds_double*)tiles;
-  LdsRk4<STAGE, kLdsTX, kLdsTY, kLdsTZ> X{
+  LdsRk4<STAGE, kLdsTX, kLdsTY, TZ> X{
       {in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p, (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
        (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N, (int)threadIdx.x},
       {u0, acc, out, aux, p, R.c, R.tinny, {}}};
@@ -251,21 +253,29 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
   if (interior && !box_empty(box) && R.variant == 1 && (box.sy % 2) == 0) {
     // LDS kernel on the largest tile-aligned sub-box, the direct kernel on the rest
     Box core = box;
-    const int T[3] = {kLdsTX, kLdsTY, kLdsTZ};
+    const int tz = box.hi[2] - box.lo[2] >= 8 ? 8 : kLdsTZ;
+    const int T[3] = {kLdsTX, kLdsTY, tz};
     for (int a = 0; a < 3; ++a) core.hi[a] = box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a];
     if (!box_empty(core)) {
       const unsigned gr = (unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
-                          ((core.hi[2] - core.lo[2]) / kLdsTZ);
-      const int nt = kLdsTX * kLdsTY * kLdsTZ;
-      switch (R.stage) {
-        case 0:
-          if (want_res) rk4_lds_kernel<G, 0, true><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag);
-          else rk4_lds_kernel<G, 0, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag);
-          break;
-        case 1: rk4_lds_kernel<G, 1, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break;
-        case 2: rk4_lds_kernel<G, 2, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break;
-        default: rk4_lds_kernel<G, 3, false><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break;
+                          ((core.hi[2] - core.lo[2]) / tz);
+      const int nt = kLdsTX * kLdsTY * tz;
+#define AMSS_LDS(TZ)                                                                                          \
+  switch (R.stage) {                                                                                          \
+    case 0:                                                                                                   \
+      if (want_res) rk4_lds_kernel<G, 0, true, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); \
+      else rk4_lds_kernel<G, 0, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag);         \
+      break;                                                                                                  \
+    case 1: rk4_lds_kernel<G, 1, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break; \
+    case 2: rk4_lds_kernel<G, 2, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break; \
+    default: rk4_lds_kernel<G, 3, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break; \
+  }
+      if (tz == 8) {
+        AMSS_LDS(8)
+      } else {
+        AMSS_LDS(4)
       }
+#undef AMSS_LDS
     }
     Rk4Param Rd = R;
     Rd.variant = 0;

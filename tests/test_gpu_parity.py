@@ -317,21 +317,27 @@ def test_sommerfeld_routbam_abi_matches_oracle(fa, orc):
             assert np.array_equal(g == r0, o == r0)  # the same points written
 
 
-def test_lds_variant_matches_direct_kernel():
+@pytest.mark.parametrize("n", [24, 44])
+def test_lds_variant_matches_direct_kernel(n):
     """Both interior kernels evaluate the same expression tree per point (the
     compiler may contract different products into FMAs), so a 24^3 patch
-    (three whole tiles on x and y, six on z) agrees to rounding."""
+    (three whole tiles on x and y, six on z) and a 44^3 one (5 x 5 x 11
+    tiles, with remainder slabs for the direct kernel) agree to rounding,
+    the seven constraint norms of both steps included."""
     import amssbssn as A
-    cfg = A.config1(n=24)
-    out = []
+    cfg = A.config1(n=n)
+    out, res = [], []
     for variant in (0, 1):
         lv = A.Level.from_config(cfg, gauge=0)
         lv.set_variant(variant)
-        assert lv.step(cfg.dt, 2) == 0
+        assert lv.variant() == variant
+        assert lv.step(cfg.dt, 2, want_ham=A.RES_EVERY_STEP) == 0
         out.append(lv.get_state(0))
+        res.append(lv.constraint_series())
         lv.close()
     for v in range(24):
         assert rel_err(out[1][v], out[0][v]) <= 1e-13, v
+    assert np.all(np.abs(res[1] - res[0]) <= 1e-12 * np.max(np.abs(res[0]), axis=0))
 
 
 def test_two_patches_equal_one_patch():
