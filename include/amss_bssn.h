@@ -202,6 +202,23 @@ int amss_psi4_dev(const int *ex, const double *h, const double *const *fields, c
                   const double *const *ric, const double *X, const double *Y, const double *Z, double *rpsi,
                   double *ipsi, int symmetry, const double *origin, void *stream);
 
+/* The spin-weighted modes of Psi4 on the sphere of radius rex about the
+ * origin (surface_integral::surf_Wave, surface_integral.C:197-377, with the
+ * Gauss-Legendre rule of :39-181 that macrodef.h's GaussInt selects):
+ * Rpsi4 / Ipsi4 of one block (device arrays of shape ex, e.g. from
+ * amss_psi4_dev) are interpolated by global_interp's 6-point rule
+ * (fmisc.f90:712-794) at 2N x 4N points (N = nquarter; a hemisphere for
+ * Symmetry 1, an octant for 2, unfolded through the parities soa_r /
+ * soa_i), and RP[k] + i IP[k] = rex int Psi4 sqrt((2l+1)/4pi)
+ * d^l_{m,spinw}(theta) e^{-i m phi} dOmega for l = spinw..maxl, m = -l..l
+ * (k in that order; sum (2l+1) entries).  X, Y, Z: the block's coordinates on
+ * the host.  The sphere must lie in the block's owned cells (all but three
+ * ghost cells at each end of an axis, none below a symmetry plane), else 2.
+ * Synchronous: RP, IP are host arrays filled on return. */
+int amss_surf_wave_dev(const int *ex, const double *X, const double *Y, const double *Z, const double *rpsi,
+                       const double *ipsi, const double *soa_r, const double *soa_i, int symmetry, double rex,
+                       int nquarter, int spinw, int maxl, double *RP, double *IP, void *stream);
+
 /* ---- levels: Blocks + the Step driver ------------------------------------ */
 typedef struct amss_level amss_level;
 

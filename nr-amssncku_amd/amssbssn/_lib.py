@@ -41,6 +41,8 @@ SIGNATURES = {
                                   C.c_int),
     "amss_enforce_ga_dev": ([_ip, C.c_void_p, C.c_void_p], C.c_int),
     "getnp4_": ([_ip, _dp, _dp, _dp] + [_dp] * 40 + [_ip], C.c_int),
+    "amss_surf_wave_dev": ([_ip, _dp, _dp, _dp, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                            C.c_int, C.c_int, _dp, _dp, C.c_void_p], C.c_int),
     "amss_psi4_dev": ([_ip, _dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                        C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_void_p], C.c_int),
     "amss_level_create": ([C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],

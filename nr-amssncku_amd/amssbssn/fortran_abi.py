@@ -180,3 +180,25 @@ def psi4_dev(state, gam, ric, X, Y, Z, h, symmetry=0, origin=(0.0, 0.0, 0.0), st
                               C.c_void_p(s.cuda_stream)), "amss_psi4_dev")
     s.synchronize()
     return rpsi, ipsi
+
+
+def surf_wave_dev(rpsi, ipsi, X, Y, Z, rex, nquarter, maxl, spinw=2, soa_r=(1.0, 1.0, 1.0), soa_i=(1.0, 1.0, 1.0),
+                  symmetry=0, stream=None):
+    """amss_surf_wave_dev: the spin-weighted modes of Psi4 (torch float64
+    device arrays of the block's shape) on the sphere of radius rex; X, Y, Z
+    are the block's coordinates (numpy).  Returns (RP, IP) as numpy arrays in
+    the order l = spinw..maxl, m = -l..l."""
+    import torch
+    for t in (rpsi, ipsi):
+        assert t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+    shape = tuple(rpsi.shape)
+    assert tuple(ipsi.shape) == shape and (len(X), len(Y), len(Z)) == shape[::-1]
+    nn = sum(2 * l + 1 for l in range(spinw, maxl + 1))
+    rp, ip = np.zeros(nn), np.zeros(nn)
+    Xc, Yc, Zc = (np.ascontiguousarray(a, dtype=np.float64) for a in (X, Y, Z))
+    s = stream if stream is not None else torch.cuda.current_stream(rpsi.device)
+    check(lib().amss_surf_wave_dev(_ex(shape), _p(Xc), _p(Yc), _p(Zc), C.c_void_p(rpsi.data_ptr()),
+                                   C.c_void_p(ipsi.data_ptr()), (C.c_double * 3)(*soa_r), (C.c_double * 3)(*soa_i),
+                                   symmetry, rex, nquarter, spinw, maxl, _p(rp), _p(ip), C.c_void_p(s.cuda_stream)),
+          "amss_surf_wave_dev")
+    return rp, ip

@@ -17,6 +17,7 @@
 #include "amss_amr.h"
 #include "amss_host.h"
 #include "amss_psi4.h"
+#include "amss_surface.h"
 #include "amss_xfer.h"
 
 namespace amss {
@@ -751,6 +752,180 @@ int getnp4_(int* ex, double* X, double* Y, double* Z, double* chi, double* trK, 
   const double* ric[6] = {Rxx, Rxy, Rxz, Ryy, Ryz, Rzz};
   if (!Symmetry) return fail(AMSS_ERR_ARG, "getnp4_: null argument");
   return psi4_host(ex, X, Y, Z, fields, gam, ric, Rpsi4, Ipsi4, *Symmetry);
+}
+
+// ---- Psi4 modes on an extraction sphere ----------------------------------------
+// Gauss-Legendre nodes and weights of order n on [-1, 1], ascending: Newton
+// iteration on P_n from the usual cosine estimates, P_n and P_n' by the
+// three-term recurrence.
+static void gauss_legendre(int n, double* x, double* w) {
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double p2 = p1;
+        p1 = p0;
+        p0 = ((2.0 * j + 1.0) * z * p1 - j * p2) / (j + 1.0);
+      }
+      dp = n * (z * p0 - p1) / (z * z - 1.0);
+      const double dz = p0 / dp;
+      z -= dz;
+      if (std::fabs(dz) <= 1e-16) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+// d^l_{m,s}(theta) of cos(theta): the alternating factorial sum over t
+// (the Wigner formula, e.g. misc.C:271-296).
+static double wigner_d_host(int l, int m, int s, double ct) {
+  auto fact = [](int k) {
+    double f = 1.0;
+    for (int q = 2; q <= k; ++q) f *= q;
+    return f;
+  };
+  const int t0 = std::max(0, m - s), t1 = std::min(l + m, l - s);
+  const double sh = std::sqrt((1.0 - ct) / 2.0), ch = std::sqrt((1.0 + ct) / 2.0);
+  double v = 0.0;
+  for (int t = t0; t <= t1; ++t)
+    v += ((t % 2) ? -1.0 : 1.0) * std::pow(ch, 2 * l + m - s - 2 * t) * std::pow(sh, 2 * t + s - m) /
+         (fact(l + m - t) * fact(l - s - t) * fact(t) * fact(t + s - m));
+  return v * std::sqrt(fact(l + m) * fact(l - m) * fact(l + s) * fact(l - s));
+}
+
+struct SurfShim {
+  std::mutex mu;
+  void* buf = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes > cap) {
+      if (buf) HIPCHK(hipFree(buf));
+      buf = nullptr;
+      cap = 0;
+      HIPCHK(hipMalloc(&buf, bytes));
+      cap = bytes;
+    }
+    return 0;
+  }
+};
+static SurfShim g_surf;
+
+int amss_surf_wave_dev(const int* ex, const double* X, const double* Y, const double* Z, const double* rpsi,
+                       const double* ipsi, const double* soa_r, const double* soa_i, int symmetry, double rex,
+                       int nquarter, int spinw, int maxl, double* RP, double* IP, void* stream) {
+  const char* who = "amss_surf_wave_dev";
+  if (!ex || !X || !Y || !Z || !rpsi || !ipsi || !soa_r || !soa_i || !RP || !IP)
+    return fail(AMSS_ERR_ARG, std::string(who) + ": null argument");
+  if (symmetry < 0 || symmetry > 2) return fail(AMSS_ERR_ARG, std::string(who) + ": Symmetry must be 0, 1 or 2");
+  if (ex[0] < 6 || ex[1] < 6 || ex[2] < 6) return fail(AMSS_ERR_ARG, std::string(who) + ": an axis below 6 points");
+  if (nquarter < 1 || nquarter > 4096 || spinw < 0 || maxl < spinw || maxl > 40 || !(rex > 0.0))
+    return fail(AMSS_ERR_ARG, std::string(who) + ": bad sphere (N, spin weight, maxl or radius)");
+  const double pi = 3.14159265358979323846;
+  // the sphere (surface_integral.C:92-181, GaussInt): 2N nodes over [-1, 1]
+  // ordered from cos(theta) = 1 down, 4N in phi; a hemisphere / octant
+  // under the symmetries
+  const int ntf = 2 * nquarter;
+  std::vector<double> ct(ntf), wt(ntf);
+  gauss_legendre(ntf, ct.data(), wt.data());
+  std::reverse(ct.begin(), ct.end());
+  std::reverse(wt.begin(), wt.end());
+  int n_phi = 4 * nquarter, n_theta = ntf, lpsy = 1;
+  double dphi = 2.0 * pi / n_phi;
+  if (symmetry == 2) {
+    n_phi /= 4;
+    n_theta /= 2;
+    dphi = pi / (2.0 * n_phi);
+    lpsy = 8;
+  } else if (symmetry == 1) {
+    n_theta /= 2;
+    lpsy = 2;
+  }
+  // every sphere point inside the block's owned cells (three ghost cells at
+  // each end of an axis, none below a symmetry plane)
+  const double* C[3] = {X, Y, Z};
+  double lo[3], hi[3];
+  int cmin[3];
+  for (int a = 0; a < 3; ++a) {
+    const double h = C[a][1] - C[a][0];
+    const bool plane = (a == 2 ? symmetry > 0 : symmetry > 1) && std::fabs(C[a][0]) < h;
+    lo[a] = C[a][plane ? 0 : 3] - h / 2;
+    hi[a] = C[a][ex[a] - 4] + h / 2;
+    cmin[a] = plane ? -2 : 1;
+  }
+  for (int i = 0; i < n_theta; ++i) {
+    const double st = std::sqrt(1.0 - ct[i] * ct[i]);
+    for (int j = 0; j < n_phi; ++j) {
+      const double ph = (j + 0.5) * dphi;
+      const double q[3] = {rex * (st * std::cos(ph)), rex * (st * std::sin(ph)), rex * ct[i]};
+      for (int a = 0; a < 3; ++a)
+        if (q[a] < lo[a] || q[a] > hi[a])
+          return fail(AMSS_ERR_ARG, std::string(who) + ": the sphere leaves the block's owned cells");
+    }
+  }
+  // sqrt((2l+1)/4pi) d^l_{m,s} at +cos(theta) and -cos(theta) per mode
+  std::vector<int> mm;
+  std::vector<double> tp;
+  for (int l = spinw; l <= maxl; ++l)
+    for (int m = -l; m <= l; ++m) {
+      mm.push_back(m);
+      for (int sgn = 0; sgn < 2; ++sgn)
+        for (int i = 0; i < n_theta; ++i)
+          tp.push_back(std::sqrt((2 * l + 1.0) / 4.0 / pi) * wigner_d_host(l, m, spinw, sgn ? -ct[i] : ct[i]));
+    }
+  const int nm = (int)mm.size(), npts = n_theta * n_phi;
+  const size_t ncoord = (size_t)ex[0] + ex[1] + ex[2];
+  const size_t nd = ncoord + 2 * (size_t)n_theta + tp.size() + 2 * (size_t)npts + 2 * (size_t)nm;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<double> out(2 * (size_t)nm);
+  {
+    std::lock_guard<std::mutex> lk(g_surf.mu);
+    if (int r = g_surf.ensure(nd * sizeof(double) + nm * sizeof(int))) return r;
+    double* d = (double*)g_surf.buf;
+    double *dX = d, *dY = dX + ex[0], *dZ = dY + ex[1], *dct = dZ + ex[2], *dwt = dct + n_theta;
+    double *dtp = dwt + n_theta, *dvals = dtp + tp.size(), *dout = dvals + 2 * (size_t)npts;
+    int* dmm = (int*)(dout + 2 * (size_t)nm);
+    HIPCHK(hipMemcpyAsync(dX, X, ex[0] * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dY, Y, ex[1] * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dZ, Z, ex[2] * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dct, ct.data(), n_theta * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dwt, wt.data(), n_theta * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dtp, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dmm, mm.data(), nm * sizeof(int), hipMemcpyHostToDevice, s));
+    SphereGrid G;
+    G.rpsi = rpsi;
+    G.ipsi = ipsi;
+    G.X = dX;
+    G.Y = dY;
+    G.Z = dZ;
+    SphereRule R;
+    R.costheta = dct;
+    R.weight = dwt;
+    R.n_theta = n_theta;
+    R.n_phi = n_phi;
+    R.lpsy = lpsy;
+    R.dphi = dphi;
+    R.rex = rex;
+    for (int a = 0; a < 3; ++a) {
+      G.n[a] = ex[a];
+      G.cmin[a] = cmin[a];
+      G.soa_r[a] = R.soa_r[a] = soa_r[a];
+      G.soa_i[a] = R.soa_i[a] = soa_i[a];
+    }
+    launch_sphere_interp(G, R, dvals, s);
+    launch_sphere_modes(R, dvals, dtp, dmm, nm, dout, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  for (int k = 0; k < nm; ++k) {
+    RP[k] = out[2 * k];
+    IP[k] = out[2 * k + 1];
+  }
+  return 0;
 }
 
 // ---- levels ----------------------------------------------------------------

@@ -1236,6 +1236,206 @@ int orc_sommerfeld_interp(const int *ex, const double *X, const double *Y, const
     return bad;
 }
 
+/* global_interp, the Cell branch (fmisc.f90:712-794): the 6-point Lagrange
+ * interpolant of f at (x1, y1, z1) through decide3d's block (:798-874,
+ * mirrored with SoA below a symmetry plane) and polin3 (Neville along z,
+ * then y, then x).  Returns 0, or 1 where decide3d rejects the block (the
+ * reference stops there). */
+int orc_global_interp(const int *ex, const double *X, const double *Y, const double *Z, const double *f,
+                      double x1, double y1, double z1, const double *SoA, int symmetry, double *out)
+{
+    enum { ORDN = 6 };
+    const double F04f = (double)0.4f; /* the Fortran literal 0.4 is single precision */
+    const double dX = X[1] - X[0], dY = Y[1] - Y[0], dZ = Z[1] - Z[0];
+    const double d[3] = {dX, dY, dZ}, p[3] = {x1, y1, z1};
+    const double *C[3] = {X, Y, Z};
+    int cxB[3], cxT[3], cmin[3] = {1, 1, 1}, cmax[3] = {ex[0], ex[1], ex[2]};
+    for (int m = 0; m < 3; m++) {
+        const int cxI = (int)((p[m] - C[m][0]) / d[m] + F04f) + 1; /* idint truncates */
+        cxB[m] = cxI - ORDN / 2 + 1;
+        cxT[m] = cxB[m] + ORDN - 1;
+    }
+    if (symmetry == 2 && fabs(X[0]) < dX) cmin[0] = -ORDN / 2 + 1;
+    if (symmetry == 2 && fabs(Y[0]) < dY) cmin[1] = -ORDN / 2 + 1;
+    if (symmetry != 0 && fabs(Z[0]) < dZ) cmin[2] = -ORDN / 2 + 1;
+    double cx[3];
+    for (int m = 0; m < 3; m++) {
+        if (cxB[m] < cmin[m]) { cxB[m] = cmin[m]; cxT[m] = cxB[m] + ORDN - 1; }
+        if (cxT[m] > cmax[m]) { cxT[m] = cmax[m]; cxB[m] = cxT[m] + 1 - ORDN; }
+        if (cxB[m] > 0) cx[m] = (p[m] - C[m][cxB[m] - 1]) / d[m];
+        else cx[m] = (p[m] + C[m][1 - cxB[m] - 1]) / d[m];
+    }
+    /* decide3d: indices >= 1 read f, indices <= 0 read f(1 - i) * SoA */
+    for (int m = 0; m < 3; m++) {
+        const int fmin1 = cxB[m] > 1 ? cxB[m] : 1, fmax1 = cxT[m];
+        const int fmin2 = cxB[m], fmax2 = cxT[m] < 0 ? cxT[m] : 0;
+        if (fmin1 <= fmax1 && (fmin1 < 1 || fmax1 > ex[m])) return 1;
+        if (fmin2 <= fmax2 && (1 - fmax2 < 1 || 1 - fmin2 > ex[m])) return 1;
+    }
+    double ya[ORDN][ORDN][ORDN];
+    for (int c = 0; c < ORDN; c++)
+        for (int b = 0; b < ORDN; b++)
+            for (int a = 0; a < ORDN; a++) {
+                const int g[3] = {cxB[0] + a, cxB[1] + b, cxB[2] + c};
+                double sg = 1.0;
+                int q[3];
+                for (int m = 0; m < 3; m++) {
+                    if (g[m] >= 1) q[m] = g[m];
+                    else { q[m] = 1 - g[m]; sg *= SoA[m]; }
+                }
+                const double v = f[IX(ex, q[0] - 1, q[1] - 1, q[2] - 1)];
+                ya[a][b][c] = (sg == 1.0) ? v : v * sg;
+            }
+    double xa[ORDN + 1], line[ORDN + 1], ym[ORDN + 1], yat[ORDN];
+    for (int a = 1; a <= ORDN; a++) xa[a] = (double)(a - 1);
+    for (int a = 0; a < ORDN; a++) {
+        for (int b = 0; b < ORDN; b++) {
+            for (int c = 0; c < ORDN; c++) line[c + 1] = ya[a][b][c];
+            yat[b] = neville(xa, line, cx[2], ORDN);
+        }
+        for (int b = 0; b < ORDN; b++) line[b + 1] = yat[b];
+        ym[a + 1] = neville(xa, line, cx[1], ORDN);
+    }
+    *out = neville(xa, ym, cx[0], ORDN);
+    return 0;
+}
+
+/* misc::fact and misc::Wigner_d_function (misc.C:271-309): d^l_{m s} of
+ * cos(theta), the alternating sum over t. */
+static double wfact(int n) { return n <= 0 ? 1.0 : n * wfact(n - 1); }
+static double wigner_d(int l, int m, int s, double costheta)
+{
+    const int C1 = m - s > 0 ? m - s : 0, C2 = l + m < l - s ? l + m : l - s;
+    double vv = 0.0;
+    const double sinht = sqrt((1 - costheta) / 2.0), cosht = sqrt((1 + costheta) / 2.0);
+    const double first = (C1 % 2 == 0) ? 1.0 : -1.0;
+    for (int t = C1; t < C2 + 1; t += 2)
+        vv = vv + first * pow(cosht, 2 * l + m - s - 2 * t) * pow(sinht, 2 * t + s - m) /
+                      (wfact(l + m - t) * wfact(l - s - t) * wfact(t) * wfact(t + s - m));
+    for (int t = C1 + 1; t < C2 + 1; t += 2)
+        vv = vv - first * pow(cosht, 2 * l + m - s - 2 * t) * pow(sinht, 2 * t + s - m) /
+                      (wfact(l + m - t) * wfact(l - s - t) * wfact(t) * wfact(t + s - m));
+    return vv * sqrt(wfact(l + m) * wfact(l - m) * wfact(l + s) * wfact(l - s));
+}
+
+/* misc::gaulegf (misc.C): Gauss-Legendre nodes and weights on [x1, x2]. */
+static void gaulegf(double x1, double x2, double *x, double *w, int n)
+{
+    const double eps = 1.2E-16, PI = acos(-1.0);
+    const int m = (n + 1) / 2;
+    const double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+    for (int i = 0; i < m; i++) {
+        double z = cos(PI * ((double)i + 0.75) / ((double)n + 0.5)), z1, pp, p1, p2, p3;
+        do {
+            p1 = 1.0;
+            p2 = 0.0;
+            for (int j = 0; j < n; j++) {
+                p3 = p2;
+                p2 = p1;
+                p1 = ((2 * (double)j + 1) * z * p2 - (double)j * p3) / ((double)j + 1);
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+        } while (fabs(z - z1) > eps);
+        x[i] = xm - xl * z;
+        x[n - 1 - i] = xm + xl * z;
+        w[i] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+        w[n - 1 - i] = w[i];
+    }
+}
+
+/* surface_integral::surface_integral (surface_integral.C:39-181, GaussInt as
+ * macrodef.h:16 sets it) and surf_Wave on one block (:197-377): Psi4 is
+ * interpolated (global_interp, order 6) at the n_tot points of the sphere of
+ * radius rex -- Gauss-Legendre in cos(theta) (2N nodes over [-1, 1], upper
+ * hemisphere first), the midpoint rule in phi (4N points; a quarter of them
+ * for Symmetry 2, half the theta nodes for Symmetry 1 and 2, the other
+ * octants or the other hemisphere added through the parities) -- and
+ * RP + i IP = rex dphi sum_n w_theta sqrt((2l+1)/4pi) d^l_{m,spinw}(theta)
+ * Psi4 e^{-i m phi} for l = spinw..maxl, m = -l..l, in that order.  The
+ * points must lie in the block's owned cells (all but 3 ghost cells at each
+ * end of an axis; none at a lower end on a symmetry plane).
+ * Returns 0; 2 for a point outside; 1 where decide3d rejects a block. */
+int orc_surf_wave(const int *ex, const double *X, const double *Y, const double *Z, const double *rpsi,
+                  const double *ipsi, const double *soa_r, const double *soa_i, int symmetry, double rex, int N,
+                  int spinw, int maxl, double *RP, double *IP)
+{
+    const double PI = acos(-1.0);
+    int N_phi = 4 * N, N_theta = 2 * N;
+    double *arcos = malloc(sizeof(double) * 2 * N), *wt = malloc(sizeof(double) * 2 * N);
+    gaulegf(-1.0, 1.0, arcos, wt, N_theta);
+    for (int i = 0; i < (N_theta + 1) / 2; i++) { /* misc::inversearray */
+        double r = arcos[i]; arcos[i] = arcos[N_theta - 1 - i]; arcos[N_theta - 1 - i] = r;
+        r = wt[i]; wt[i] = wt[N_theta - 1 - i]; wt[N_theta - 1 - i] = r;
+    }
+    double dphi;
+    if (symmetry == 2) { N_phi /= 4; N_theta /= 2; dphi = PI / (2.0 * N_phi); }
+    else if (symmetry == 1) { N_theta /= 2; dphi = 2.0 * PI / N_phi; }
+    else { dphi = 2.0 * PI / N_phi; }
+    const int n_tot = N_theta * N_phi;
+    double *sf = malloc(sizeof(double) * 2 * n_tot);
+    int rc = 0;
+    /* the block's owned cells: three ghost cells at each end, none below a
+     * symmetry plane (the array starts there) */
+    const double *Cs[3] = {X, Y, Z};
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        const double h = Cs[a][1] - Cs[a][0];
+        const int plane = (a == 2 ? symmetry > 0 : symmetry > 1) && fabs(Cs[a][0]) < h;
+        lo[a] = Cs[a][plane ? 0 : 3] - h / 2;
+        hi[a] = Cs[a][ex[a] - 4] + h / 2;
+    }
+    for (int i = 0, n = 0; i < N_theta && !rc; ++i) {
+        const double ct = arcos[i], st = sqrt(1.0 - ct * ct);
+        for (int j = 0; j < N_phi && !rc; ++j, ++n) {
+            const double ph = (j + 0.5) * dphi;
+            const double q[3] = {rex * (st * cos(ph)), rex * (st * sin(ph)), rex * ct};
+            for (int a = 0; a < 3; a++)
+                if (q[a] < lo[a] || q[a] > hi[a]) rc = 2;
+            if (rc) break;
+            rc = orc_global_interp(ex, X, Y, Z, rpsi, q[0], q[1], q[2], soa_r, symmetry, &sf[2 * n]);
+            if (!rc) rc = orc_global_interp(ex, X, Y, Z, ipsi, q[0], q[1], q[2], soa_i, symmetry, &sf[2 * n + 1]);
+        }
+    }
+    if (!rc) {
+        int NN = 0;
+        for (int l = spinw; l <= maxl; l++) NN += 2 * l + 1;
+        for (int k = 0; k < NN; k++) RP[k] = IP[k] = 0.0;
+        const int lpsy = symmetry == 0 ? 1 : (symmetry == 1 ? 2 : 8);
+        for (int n = 0; n < n_tot; n++) {
+            const int i = n / N_phi, j = n - i * N_phi;
+            int k = 0;
+            for (int pl = spinw; pl <= maxl; pl++)
+                for (int pm = -pl; pm <= pl; pm++, k++)
+                    for (int lp = 0; lp < lpsy; lp++) {
+                        /* (theta or pi - theta) x (phi, 2 pi - phi, pi - phi, pi + phi), the
+                         * parities of the reflected octant (surface_integral.C:265-326) */
+                        const int flip_t = lp & 1;
+                        double arg, sgn_sin = 1.0, sr = 1.0, si = 1.0;
+                        switch (lp >> 1) {
+                        case 0: arg = pm * (j + 0.5) * dphi; break;
+                        case 1: arg = pm * (j + 0.5) * dphi; sgn_sin = -1.0; sr = soa_r[1]; si = soa_i[1]; break;
+                        case 2: arg = pm * (PI - (j + 0.5) * dphi); sr = soa_r[0]; si = soa_i[0]; break;
+                        default: arg = pm * (PI + (j + 0.5) * dphi); sr = soa_r[1] * soa_r[0]; si = soa_i[1] * soa_i[0]; break;
+                        }
+                        if (flip_t) { sr *= soa_r[2]; si *= soa_i[2]; }
+                        const double costheta = flip_t ? -arcos[i] : arcos[i];
+                        const double cm = cos(arg), sm = sgn_sin * sin(arg);
+                        const double R = sr * sf[2 * n], I = si * sf[2 * n + 1];
+                        const double thetap = sqrt((2 * pl + 1.0) / 4.0 / PI) * wigner_d(pl, pm, spinw, costheta);
+                        RP[k] = RP[k] + thetap * (R * cm + I * sm) * wt[i];
+                        IP[k] = IP[k] + thetap * (I * cm - R * sm) * wt[i];
+                    }
+        }
+        for (int k = 0; k < NN; k++) { RP[k] = RP[k] * rex * dphi; IP[k] = IP[k] * rex * dphi; }
+    }
+    free(arcos);
+    free(wt);
+    free(sf);
+    return rc;
+}
+
 /* One axis of sommerfeld_routbam's derivative (sommerfeld_rout.f90:383-436):
  * second-order differences taken from the side the outgoing wave arrives
  * from (w = v x_a / r > 0: backward), centred or reversed where the array

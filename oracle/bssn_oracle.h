@@ -77,6 +77,17 @@ void orc_getnp4(const int *ex, const double *X, const double *Y, const double *Z
  * mirrored with SoA below index 1).  Returns the number of points whose
  * block decide3d rejects (left unchanged; the reference prints an error and
  * interpolates undefined data there). */
+/* fmisc.f90:712-794 (global_interp, Cell): 6-point interpolation of f at
+ * (x1, y1, z1); returns 0, or 1 where decide3d rejects the block. */
+int orc_global_interp(const int *ex, const double *X, const double *Y, const double *Z, const double *f,
+                      double x1, double y1, double z1, const double *SoA, int symmetry, double *out);
+/* surface_integral.C:39-181 + 197-377 (GaussInt): the spin-weighted modes of
+ * Psi4 on the sphere of radius rex, N points per quarter sphere; RP, IP hold
+ * sum_{l=spinw..maxl} (2l+1) values.  Returns 0, 2 for a sphere point outside
+ * the block's owned cells, 1 where decide3d rejects a block. */
+int orc_surf_wave(const int *ex, const double *X, const double *Y, const double *Z, const double *rpsi,
+                  const double *ipsi, const double *soa_r, const double *soa_i, int symmetry, double rex, int N,
+                  int spinw, int maxl, double *RP, double *IP);
 int orc_sommerfeld_interp(const int *ex, const double *X, const double *Y, const double *Z, const double *bbox,
                           double dT, const double *chi0, const double *Lap0, const double *f0, double *f,
                           const double *SoA, int symmetry);

@@ -158,6 +158,25 @@ class Orc:
                             _ptr(ip), C.c_int(symmetry))
         return rp, ip
 
+    def global_interp(self, f, X, Y, Z, pt, soa=(1.0, 1.0, 1.0), symmetry=0):
+        """global_interp (Cell, order 6) of f at pt; returns (status, value)."""
+        out = C.c_double(0.0)
+        rc = self.lib.orc_global_interp(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(f), C.c_double(pt[0]),
+                                        C.c_double(pt[1]), C.c_double(pt[2]), (C.c_double * 3)(*soa),
+                                        C.c_int(symmetry), C.byref(out))
+        return rc, out.value
+
+    def surf_wave(self, rpsi, ipsi, X, Y, Z, rex, nquarter, maxl, spinw=2, soa_r=(1.0, 1.0, 1.0),
+                  soa_i=(1.0, 1.0, 1.0), symmetry=0):
+        """surface_integral::surf_Wave on one block; returns (status, RP, IP)."""
+        nn = sum(2 * l + 1 for l in range(spinw, maxl + 1))
+        rp, ip = np.zeros(nn), np.zeros(nn)
+        rc = self.lib.orc_surf_wave(_ex(rpsi.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(rpsi), _ptr(ipsi),
+                                    (C.c_double * 3)(*soa_r), (C.c_double * 3)(*soa_i), C.c_int(symmetry),
+                                    C.c_double(rex), C.c_int(nquarter), C.c_int(spinw), C.c_int(maxl), _ptr(rp),
+                                    _ptr(ip))
+        return rc, rp, ip
+
     def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
         """prolong3 into funf in place; returns the status (0 ok, 1 bad region)."""
         d3 = lambda v: (C.c_double * 3)(*v)  # noqa: E731
@@ -298,6 +317,15 @@ class Ref:
         self.lib.getnp4_(_ex(shape), _ptr(X), _ptr(Y), _ptr(Z), *[_ptr(a) for a in st[:14]],
                          *[_ptr(a) for a in gam], *[_ptr(a) for a in ric], _ptr(rp), _ptr(ip), self._i(symmetry))
         return rp, ip
+
+    def global_interp(self, f, X, Y, Z, pt, soa=(1.0, 1.0, 1.0), symmetry=0):
+        """global_interp_ (stops the program where decide3d fails: call it with
+        points whose block stays inside); returns (0, value)."""
+        out = C.c_double(0.0)
+        self.lib.global_interp_(_ex(f.shape), _ptr(X), _ptr(Y), _ptr(Z), _ptr(f), C.byref(out), self._d(pt[0]),
+                                self._d(pt[1]), self._d(pt[2]), self._i(6), (C.c_double * 3)(*soa),
+                                self._i(symmetry))
+        return 0, out.value
 
     def prolong3(self, func, llbc, uubc, funf, llbf, uubf, llbp, uubp, soa, symmetry=0):
         """prolong3_ (a subroutine: no status) into funf in place."""
