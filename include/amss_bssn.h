@@ -225,9 +225,12 @@ typedef struct amss_level amss_level;
 /* Boundary handling after each RK4 sub-step:
  *  AMSS_BC_FROZEN_GHOSTS: every point within gw of a patch face that no
  *    other patch of the level owns keeps the step's base state (the copy
- *    outer boundary of the benchmark configurations);
+ *    outer boundary of the benchmark configurations; gw 3 is the fast case,
+ *    wider zones overwrite the first interior layers after each sub-step);
  *  AMSS_BC_COPY_LAYER: the reference's sommerfeld_rout(precor = 1) one-layer
- *    copy on the patch faces, with the reference's stencils up to the edge;
+ *    copy on the patch faces, with the reference's stencils up to the edge
+ *    (the copied face points skip the RHS in the correctors and the
+ *    residual-free predictor, so AMSS_LIST_RHS holds no RHS there);
  *  AMSS_BC_NONE: no boundary step;
  *  AMSS_BC_SOMMERFELD: the reference's level-0 outer boundary: each stage's
  *    RHS on the outermost layer of the patch faces is replaced by

@@ -692,8 +692,10 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
     const bool res = want_res && it == 0;
     // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap after
     // sub-step 1), whose fill points already hold the same values when
-    // nothing else writes ghost points (frozen ghosts, no edge kernel).
-    const bool frozen = bc_ == AMSS_BC_FROZEN_GHOSTS && gw_ >= 3;
+    // nothing else writes ghost points (frozen ghosts 3 deep, no edge kernel;
+    // wider frozen zones overlap the tiles, which start at index 3, so their
+    // fill must follow the tiles in every sub-step).
+    const bool frozen = bc_ == AMSS_BC_FROZEN_GHOSTS && gw_ == 3;
     const bool fill = !(it == 2 && frozen);
     // With frozen ghosts the fill reads only the state and writes only
     // points no tile writes: it runs in extra workgroups of the tile launch.

@@ -190,3 +190,22 @@ def test_batched_odd_row_pitch_matches_per_patch(mode):
     ref, _ = _run(cfg, A.MODE_PLAIN, 2)
     got, _ = _run(cfg, mode, 2)
     _compare(got, ref, 1e-12)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batched_wider_frozen_zone_matches_per_patch(mode):
+    """Frozen ghost zones 4 deep: the fill overlaps the first layer of tiles
+    and must land after them in every sub-step, as in the per-patch step."""
+    import amssbssn as A
+    cfg = _uneven(npatch=4, seed=5)
+    out = []
+    for m in (0, mode):
+        lv = A.Level(cfg.patches, gauge=0, eps=cfg.eps, chitiny=cfg.chitiny, gw=4)
+        try:
+            for i in range(len(cfg.patches)):
+                lv.set_state(i, cfg.initial_state(i))
+            assert lv.step(cfg.dt, 3, m) == 0
+            out.append([lv.get_state(i) for i in range(len(cfg.patches))])
+        finally:
+            lv.close()
+    _compare(out[1], out[0], 1e-12)
