@@ -461,8 +461,12 @@ int BssnLevel::Step(double dT, bool want_res) {
       if (edge)
         if (int r = join_edges()) return r;
       if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
+        // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap),
+        // whose frozen points already hold the same values unless the zone
+        // is deeper than the interior kernel's start (gw > 3).
         Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
-        launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
+        if (!(it == 2 && gw_ == 3))
+          launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
       } else if (bc_ == AMSS_BC_COPY_LAYER) {
         // sommerfeld_rout(precor = 1) faces (sommerfeld_rout.f90:61-137)
         Faces w{{0, 0, 0}, {0, 0, 0}};

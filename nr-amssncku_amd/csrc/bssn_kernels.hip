@@ -186,14 +186,21 @@ __global__ void __launch_bounds__(256) l2_parts_kernel(const L2Job* __restrict__
   if (threadIdx.x == 0) part[J.part0 + blockIdx.x] = red[0];
 }
 
-__global__ void l2_final_kernel(const L2Job* __restrict__ jobs, int njobs, const double* __restrict__ part,
-                                double* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= njobs) return;
-  const L2Job& J = jobs[j];
+// One wave per job: lane t sums partials t, t + 64, ... in order, then a
+// fixed pairwise tree (the same sum on every run).
+__global__ void __launch_bounds__(64) l2_final_kernel(const L2Job* __restrict__ jobs, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  const L2Job& J = jobs[blockIdx.x];
+  __shared__ double red[64];
   double s = 0.0;
-  for (int b = 0; b < J.nparts; ++b) s += part[J.part0 + b];
-  out[j] = s * J.dv;
+  for (int b = threadIdx.x; b < J.nparts; b += 64) s += part[J.part0 + b];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0] * J.dv;
 }
 
 // --------------------------------------------------------------------------
@@ -375,7 +382,7 @@ void launch_time_interp(const FieldsIn& f1, const FieldsIn& f2, const FieldsIn& 
 void launch_l2_jobs(const L2Job* jobs, int njobs, int maxparts, double* part, double* out, hipStream_t s) {
   if (njobs <= 0) return;
   l2_parts_kernel<<<dim3((unsigned)maxparts, (unsigned)njobs), 256, 0, s>>>(jobs, part);
-  l2_final_kernel<<<(njobs + 63) / 64, 64, 0, s>>>(jobs, njobs, part, out);
+  l2_final_kernel<<<(unsigned)njobs, 64, 0, s>>>(jobs, part, out);
 }
 
 }  // namespace amss
