@@ -158,6 +158,10 @@ class BssnLevel {
                    bool res);
   int join_edges();
   int* dflag_ = nullptr;
+  // right-hand sides of the LDS boundary kernel (Rk4Param::edge_k), sized
+  // for the largest patch's order-1 and order-2 shells
+  double* edge_k_ = nullptr;
+  long edge_k_tiles_ = 0;
   bool ham_valid_ = false;
   // constraint norms: one L2 job per (patch, residual), their partials, the
   // last evaluation and the per-step series (device, then host copy)

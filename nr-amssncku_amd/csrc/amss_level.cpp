@@ -10,6 +10,7 @@
 
 #include "../../include/amss_bssn.h"
 #include "amss_host.h"
+#include "bssn_edge_tiles.h"
 
 namespace amss {
 
@@ -248,6 +249,9 @@ BssnLevel::BssnLevel(int npatch, const int* shapes, const double* bboxes, int ga
   }
   LCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   if (bc != AMSS_BC_FROZEN_GHOSTS || gw < 3) {
+    for (const Box& eb : edge_box_)
+      edge_k_tiles_ = std::max<long>(edge_k_tiles_, edge_tile_set(edge_shells_of(eb.n, eb.lo, eb.hi, eb.mlo)).tiles());
+    if (edge_k_tiles_ > 0) LCHK(hipMalloc(&edge_k_, edge_k_tiles_ * NFIELD * kEdgeThreads * sizeof(double)));
     LCHK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
     for (int o = 0; o < 3; ++o) {
       LCHK(hipStreamCreateWithFlags(&side_[o], hipStreamNonBlocking));
@@ -284,6 +288,7 @@ BssnLevel::~BssnLevel() {
   }
   clear_graphs();
   if (dflag_) (void)hipFree(dflag_);
+  if (edge_k_) (void)hipFree(edge_k_);
   if (dpd_) (void)hipFree(dpd_);
   for (TileRef* t : dtiles_)
     if (t) (void)hipFree(t);
@@ -302,11 +307,12 @@ BssnLevel::~BssnLevel() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-// The boundary kernel on the shell of patch pi (the reference's edge
-// stencils), one launch per stencil-order shell, each on its own side stream
-// after what stream_ has queued so far: they read the stage input and write
-// only shell points, so they overlap the interior kernel.  join_edges makes
-// stream_ wait for them.
+// The boundary kernels on the shell of patch pi (the reference's edge
+// stencils), one side stream per stencil-order shell (where the LDS kernels
+// take orders 1 and 2, both run on the order-1 stream and the order-2 call
+// queues nothing), each after what stream_ has queued so far: they read the
+// stage input and write only shell points, so they overlap the interior
+// kernel.  join_edges makes stream_ wait for them.
 int BssnLevel::launch_edges(const Block& b, int pi, MyList<var>* in, MyList<var>* out, const AuxOut& aux,
                             const Rk4Param& R, bool res) {
   RCHK(hipEventRecord(fork_ev_, stream_));
@@ -450,6 +456,8 @@ int BssnLevel::Step(double dT, bool want_res) {
       MyList<var>* c = ConsList;
       for (int a = 0; a < 7; ++a, c = c->next) aux.res[a] = b.fgfs[c->data->sgfn];
       Rk4Param R{it, variant_, cst[it], tinny};
+      R.edge_k = edge_k_;
+      R.edge_k_tiles = edge_k_tiles_;
       const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
       if (edge)
         if (int r = launch_edges(b, (int)pi, in, out, aux, R, want_res && it == 0)) return r;
@@ -692,7 +700,9 @@ int BssnLevel::StepBatched(double dT, bool want_res) {
     MyList<var>* in = it == 0 ? StateList : SynchList_pre;
     MyList<var>* out = it == 0 ? SynchList_pre : SynchList_cor;
     const Roles m{slot_of(in), slot_of(StateList), slot_of(RHSList), slot_of(out)};
-    const Rk4Param R{it, 1, cst[it], tinny};
+    Rk4Param R{it, 1, cst[it], tinny};
+    R.edge_k = edge_k_;
+    R.edge_k_tiles = edge_k_tiles_;
     const bool res = want_res && it == 0;
     // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap after
     // sub-step 1), whose fill points already hold the same values when

@@ -1,0 +1,9 @@
+// bssn_edge_g2_res.hip -- the LDS boundary stencil kernel for GAUGE 2 with the constraint residuals.
+#include "bssn_edge_lds.h"
+
+namespace amss {
+
+template void launch_edge_lds_rhs<2, true>(const FieldsIn&, const AuxOut&, const Box&, const EdgeTileSet&,
+                                              const Coef&, double*, int*, hipStream_t);
+
+}  // namespace amss

@@ -89,31 +89,28 @@ struct InteriorEnv {
 };
 
 // ---------------------------------------------------------------------------
-// Edge environment: the reference's behaviour at every point of the array
-// (Symmetry = 0): the derivative order drops for all three directions
-// together when any axis lacks +-2 (then +-1) room, the last index of any
-// axis gets zero derivatives and no advection, the upwind stencil falls back
-// per direction, KO needs +-3 room on every axis, and the halo index -1
-// mirrors index 0 with the field's parity (symmetry_bd, fmisc.f90:877-901).
+// The reference's stencils at a boundary point (Symmetry = 0): the
+// derivative order drops for all three directions together when any axis
+// lacks +-2 (then +-1) room, the last index of any axis gets zero
+// derivatives and no advection, the upwind stencil falls back per direction,
+// KO needs +-3 room on every axis, and the halo index -1 mirrors index 0
+// with the field's parity (symmetry_bd, fmisc.f90:877-901).
 // ORD: the order of the centred stencils at the point (2, 1 or 0), fixed per
 // launch by the shell the point lies in, so that each launch compiles only
-// its own stencils.
-template <int ORD>
-struct EdgeEnv {
-  static constexpr bool kFence = true;
+// its own stencils.  D supplies rel(v, di, dj, dk, off): field v (plus off)
+// at the point's neighbour (i + di, j + dj, k + dk), which every stencil
+// here keeps within [lo, n - 1] per axis and within +-3 of the point.
+template <int ORD, class D>
+struct EdgeStencils {
   static constexpr int ord = ORD;
-  const FieldsIn& F;
-  const FieldsIn& M;
   const Coef& C;
   int i, j, k;
   int n[3];
   int lo[3];  // lowest readable index per axis (0, or -3 on a symmetry plane)
-  long sy, sz;
   bool last, kook;
 
-  __device__ EdgeEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, int i_, int j_, int k_,
-                     const int* n_, const int* lo_, long sy_, long sz_)
-      : F(F_), M(M_), C(C_), i(i_), j(j_), k(k_), sy(sy_), sz(sz_) {
+  __device__ EdgeStencils(const Coef& C_, int i_, int j_, int k_, const int* n_, const int* lo_)
+      : C(C_), i(i_), j(j_), k(k_) {
     #pragma unroll
     for (int a = 0; a < 3; ++a) {
       n[a] = n_[a];
@@ -128,21 +125,11 @@ struct EdgeEnv {
     };
     kook = ORD == 2 && rm(3);
   }
-  __device__ double rd(int v, int a, int b, int c, double off) const {
-    double s = 1.0;
-    if (a < 0) { a = -a - 1; s *= parity(v, 0); }
-    if (b < 0) { b = -b - 1; s *= parity(v, 1); }
-    if (c < 0) { c = -c - 1; s *= parity(v, 2); }
-    return (F.p[v][a + b * sy + c * sz] + off) * s;
-  }
-  __device__ double at3(int v, int di, int dj, int dk) const { return rd(v, i + di, j + dj, k + dk, 0.0); }
-  __device__ double val(int v) const { return F.p[v][i + j * sy + k * sz]; }
-  __device__ double mat(int m) const { return M.p[m][i + j * sy + k * sz]; }
-  __device__ void begin(int) const {}
-  __device__ void end() const {}
+  __device__ __forceinline__ const D& self() const { return *static_cast<const D*>(this); }
+  __device__ double at3(int v, int di, int dj, int dk) const { return self().rel(v, di, dj, dk, 0.0); }
 
   __device__ double ax(int v, int axis, int m, double off = 0.0) const {
-    return axis == 0 ? rd(v, i + m, j, k, off) : (axis == 1 ? rd(v, i, j + m, k, off) : rd(v, i, j, k + m, off));
+    return axis == 0 ? self().rel(v, m, 0, 0, off) : (axis == 1 ? self().rel(v, 0, m, 0, off) : self().rel(v, 0, 0, m, off));
   }
   __device__ void grad(int v, double d[3]) const {
     #pragma unroll
@@ -242,6 +229,31 @@ struct EdgeEnv {
                       15.0 * (ax(v, a, -1) + ax(v, a, 1)) - c20);
     return r;
   }
+};
+
+// Edge environment: the boundary stencils on the arrays in global memory.
+template <int ORD>
+struct EdgeEnv : EdgeStencils<ORD, EdgeEnv<ORD>> {
+  static constexpr bool kFence = true;
+  const FieldsIn& F;
+  const FieldsIn& M;
+  long sy, sz;
+
+  __device__ EdgeEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, int i_, int j_, int k_,
+                     const int* n_, const int* lo_, long sy_, long sz_)
+      : EdgeStencils<ORD, EdgeEnv<ORD>>(C_, i_, j_, k_, n_, lo_), F(F_), M(M_), sy(sy_), sz(sz_) {}
+  __device__ double rel(int v, int di, int dj, int dk, double off) const {
+    int a = this->i + di, b = this->j + dj, c = this->k + dk;
+    double s = 1.0;
+    if (a < 0) { a = -a - 1; s *= parity(v, 0); }
+    if (b < 0) { b = -b - 1; s *= parity(v, 1); }
+    if (c < 0) { c = -c - 1; s *= parity(v, 2); }
+    return (F.p[v][a + b * sy + c * sz] + off) * s;
+  }
+  __device__ double val(int v) const { return F.p[v][this->i + this->j * sy + this->k * sz]; }
+  __device__ double mat(int m) const { return M.p[m][this->i + this->j * sy + this->k * sz]; }
+  __device__ void begin(int) const {}
+  __device__ void end() const {}
 };
 
 // ---------------------------------------------------------------------------
@@ -478,6 +490,11 @@ struct Rk4SinkT {
       u0v = U0.ptr(v)[p];
       if (STAGE > 0) av = ACC.ptr(v)[p];
     }
+    put_loaded(v, k, u0v, av);
+  }
+  // the update of field v from its rhs k, U0 value u0v and (STAGE > 0)
+  // accumulator value av
+  __device__ __forceinline__ void put_loaded(int v, double k, double u0v, double av) {
     double kk = k;
     if (STAGE == 0)
       ACC.ptr(v)[p] = k;

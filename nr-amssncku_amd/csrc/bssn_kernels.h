@@ -40,6 +40,10 @@ struct Rk4Param {
   double c;      // 0.5 dt, 0.5 dt, dt, dt/6
   double tinny;  // chitiny - 1 (bssn_class.C:179)
   int edge_order = -1;  // boundary launches: the one stencil-order shell to run (-1: all three)
+  // right-hand sides of the LDS boundary kernel: edge_k_tiles tiles x NFIELD
+  // x 256 doubles (null: the direct boundary kernel takes those shells)
+  double* edge_k = nullptr;
+  long edge_k_tiles = 0;
 };
 
 // RHS of compute_rhs_bssn on the interior box (and, with edge, on the rest

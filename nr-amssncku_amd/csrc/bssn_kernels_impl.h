@@ -3,6 +3,7 @@
 // heavy point kernels compile in parallel translation units.
 #pragma once
 #include "bssn_batch.h"
+#include "bssn_edge_tiles.h"
 #include "bssn_lds.h"
 
 namespace amss {
@@ -27,80 +28,30 @@ __device__ __forceinline__ bool interior_index(const Box& box, int& i, int& j, i
   k = box.lo[2] + blockIdx.z;
   return i < box.hi[0] && j < box.hi[1];
 }
-// The points of an outer box [OL, OH) outside an inner box [IL, IH) (clamped
-// into it), enumerated as six disjoint slabs (z faces whole, then y faces
-// over the inner z range, then x faces over the inner y and z ranges), so
-// that every lane of a wave has a point: walking the whole array instead
-// leaves an x-face row's points alone in a wave.  An inner box empty along
-// any axis holds no point, and the slabs then cover the whole outer box.
-struct Shell {
-  int OL[3], OH[3], IL[3], IH[3];
-  long cnt[6];
-  __host__ __device__ Shell(const int ol[3], const int oh[3], const int il[3], const int ih[3]) {
-    bool empty = false;
-    for (int a = 0; a < 3; ++a) {
-      OL[a] = ol[a];
-      OH[a] = oh[a] < ol[a] ? ol[a] : oh[a];
-      IL[a] = il[a] < OL[a] ? OL[a] : (il[a] > OH[a] ? OH[a] : il[a]);
-      IH[a] = ih[a] < IL[a] ? IL[a] : (ih[a] > OH[a] ? OH[a] : ih[a]);
-      empty = empty || IH[a] == IL[a];
-    }
-    if (empty)
-      for (int a = 0; a < 3; ++a) IH[a] = IL[a];
-    const long ex = OH[0] - OL[0], ey = OH[1] - OL[1];
-    const long mk = IH[2] - IL[2], mj = IH[1] - IL[1];
-    cnt[0] = ex * ey * (IL[2] - OL[2]);
-    cnt[1] = ex * ey * (OH[2] - IH[2]);
-    cnt[2] = ex * (IL[1] - OL[1]) * mk;
-    cnt[3] = ex * (OH[1] - IH[1]) * mk;
-    cnt[4] = (long)(IL[0] - OL[0]) * mj * mk;
-    cnt[5] = (long)(OH[0] - IH[0]) * mj * mk;
-  }
-  __host__ __device__ long size() const { return cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4] + cnt[5]; }
-  // point t of the shell (t < size())
-  __device__ void point(long t, int& i, int& j, int& k) const {
-    int s = 0;
-    while (s < 5 && t >= cnt[s]) t -= cnt[s++];
-    const int x0 = s < 4 ? OL[0] : (s == 4 ? OL[0] : IH[0]);
-    const int ex = s < 4 ? OH[0] - OL[0] : (s == 4 ? IL[0] - OL[0] : OH[0] - IH[0]);
-    const int y0 = s < 2 ? OL[1] : (s == 2 ? OL[1] : (s == 3 ? IH[1] : IL[1]));
-    const int ey = s < 2 ? OH[1] - OL[1] : (s == 2 ? IL[1] - OL[1] : (s == 3 ? OH[1] - IH[1] : IH[1] - IL[1]));
-    const int z0 = s == 0 ? OL[2] : (s == 1 ? IH[2] : IL[2]);
-    i = x0 + (int)(t % ex);
-    j = y0 + (int)((t / ex) % ey);
-    k = z0 + (int)(t / ((long)ex * ey));
-  }
-};
+inline EdgeShells edge_shells(const Box& b) { return edge_shells_of(b.n, b.lo, b.hi, b.mlo); }
 
-// The boundary points of an array outside the interior box, split by the
-// order of their centred stencils (the joint three-axis room test of
-// fderivs/fdderivs, diff_new.f90:996-1100, with the lowest readable index
-// mlo): order 2 on [max(0, mlo + 2), n - 2), order 1 on the next layer out to
-// [max(0, mlo + 1), n - 1), order 0 (and the last index) on the rest.
-struct EdgeShells {
-  Shell s[3];  // by stencil order 0, 1, 2
-};
-inline EdgeShells edge_shells(const Box& b) {
-  int z[3] = {0, 0, 0}, L1[3], H1[3], L2[3], H2[3];
-  for (int a = 0; a < 3; ++a) {
-    L1[a] = b.mlo[a] + 1 > 0 ? b.mlo[a] + 1 : 0;
-    H1[a] = b.n[a] - 1;
-    L2[a] = b.mlo[a] + 2 > 0 ? b.mlo[a] + 2 : 0;
-    H2[a] = b.n[a] - 2;
-    if (H2[a] < L2[a]) H2[a] = L2[a];
-    if (H1[a] < L1[a]) H1[a] = L1[a];
-  }
-  int blo[3], bhi[3];
-  bool empty = false;
-  for (int a = 0; a < 3; ++a) {
-    blo[a] = b.lo[a];
-    bhi[a] = b.hi[a];
-    empty = empty || b.hi[a] <= b.lo[a];
-  }
-  if (empty)  // no interior launch: the order-2 shell takes its box whole
-    for (int a = 0; a < 3; ++a) bhi[a] = blo[a] = L2[a];
-  return EdgeShells{{Shell(z, b.n, L1, H1), Shell(L1, H1, L2, H2), Shell(L2, H2, blo, bhi)}};
+// Whether the LDS boundary kernels may take the order-1 and order-2 shells
+// of this box: an even row pitch (the direct loads move 16 bytes from even
+// indices), no Sommerfeld or copy face on them (a physical lower face has
+// no mirrored halo, so its index 0 is an outermost-layer point; upper faces
+// are at n - 1), and room for their tiles in the right-hand-side buffer.
+inline bool edge_lds_ok(const Box& box, const SommerGeom& sg, const EdgeTileSet& T, const Rk4Param& R) {
+  if (!R.edge_k || (box.sy % 2) != 0 || (box.sz % 2) != 0 || T.tiles() > R.edge_k_tiles) return false;
+  const int mask = sg.faces | sg.copy;
+  for (int a = 0; a < 3; ++a)
+    if (((mask >> (3 + a)) & 1) && box.mlo[a] != 0) return false;
+  return true;
 }
+
+// The right-hand side on the tiles of the order-1 and order-2 shells
+// (bssn_edge_lds.h; instantiated per gauge and residual flag in
+// bssn_edge_g<G>[_res].hip), and the RK4 update of their points.
+template <int G, bool CO>
+void launch_edge_lds_rhs(const FieldsIn& in, const AuxOut& aux, const Box& box, const EdgeTileSet& T,
+                         const Coef& C, double* K, int* flag, hipStream_t s);
+void launch_edge_lds_update(const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out, const AuxOut& aux,
+                            const Box& box, const EdgeTileSet& T, const double* K, const Rk4Param& R,
+                            hipStream_t s);
 
 __device__ __forceinline__ bool edge_index(const Shell& sh, int& i, int& j, int& k) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -299,9 +250,21 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
   }
   if (edge) {
     const EdgeShells es = edge_shells(box);
+    // the order-1 and order-2 shells in one LDS launch pair where it may
+    // take them (run with edge_order 1; edge_order 2 then has nothing left)
+    const EdgeTileSet ts = edge_tile_set(es);
+    const bool lds = R.variant == 1 && edge_lds_ok(box, sg, ts, R);
     for (int o = 0; o < 3; ++o) {
       const Shell& sh = es.s[o];
       if (sh.size() <= 0 || (R.edge_order >= 0 && o != R.edge_order)) continue;
+      if (lds && o >= 1) {
+        if (o == 1 && ts.tiles() > 0) {
+          if (want_res) launch_edge_lds_rhs<G, true>(in, aux, box, ts, C, R.edge_k, flag, s);
+          else launch_edge_lds_rhs<G, false>(in, aux, box, ts, C, R.edge_k, flag, s);
+          launch_edge_lds_update(u0, acc, out, aux, box, ts, R.edge_k, R, s);
+        }
+        continue;
+      }
       const unsigned gr = blocks_for(sh.size(), 256);
 #define AMSS_EDGE(ST, CO)                                                                                      \
   do {                                                                                                         \
