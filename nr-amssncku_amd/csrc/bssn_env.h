@@ -388,7 +388,11 @@ __host__ __device__ __forceinline__ double prop_speed(int v) {
 // The boundary RHS at point p (sommerfeld_rout.f90:376-440).  rd(a, m): the
 // stage input m points along axis a (|m| <= 2; index -1 is the mirror of 0).
 // Where w = s x_a / r is 0 the reference leaves the derivative unset; x_a = 0
-// there, so its term vanishes, and here it is 0.
+// there, so its term vanishes, and here it is 0.  Every neighbour the three
+// difference forms may use is read before the form is chosen (one the array
+// lacks room for reads the point itself and goes unused: with 5 or more
+// points per axis the chosen form always has its room), so the reads do not
+// wait on the choice.
 template <class Rd>
 __device__ __forceinline__ double sommer_rhs(const SommerGeom& G, const int n[3], const int p[3], double vel,
                                              Rd rd) {
@@ -401,14 +405,23 @@ __device__ __forceinline__ double sommer_rhs(const SommerGeom& G, const int n[3]
     const double w = vel * x[a] / r, d2 = 0.5 / G.h[a];
     const bool room_dn2 = p[a] - 2 >= G.lo[a], room_dn1 = p[a] - 1 >= G.lo[a];
     const bool room_up2 = p[a] + 2 <= n[a] - 1, room_up1 = p[a] + 1 <= n[a] - 1;
-    // 0: backward, 1: centred, 2: forward
-    int form = -1;
-    if (w > 0.0) form = room_dn2 ? 0 : (room_dn1 ? 1 : 2);
-    else if (w < 0.0) form = room_up2 ? 2 : (room_up1 ? 1 : 0);
+    const double m2 = rd(a, room_dn2 ? -2 : 0), m1 = rd(a, room_dn1 ? -1 : 0);
+    const double p1 = rd(a, room_up1 ? 1 : 0), p2 = rd(a, room_up2 ? 2 : 0);
+    // 0: backward, 1: centred, 2: forward.  All three are formed and one
+    // is selected (the empty asm keeps the compiler from sinking each into
+    // a branch of its own), so the reads have no branch to wait behind.
+    const bool up = w > 0.0, dn = w < 0.0;
+    const bool use_b = (up && room_dn2) || (dn && !room_up2 && !room_up1);
+    const bool use_c = (up && !room_dn2 && room_dn1) || (dn && !room_up2 && room_up1);
+    const bool use_f = (up && !room_dn2 && !room_dn1) || (dn && room_up2);
+    double db = d2 * (3.0 * f - 4.0 * m1 + m2);
+    double dc = d2 * (-m1 + p1);
+    double df = d2 * (-p2 + 4.0 * p1 - 3.0 * f);
+    asm volatile("" : "+v"(db), "+v"(dc), "+v"(df));
     double d = 0.0;
-    if (form == 0) d = d2 * (3.0 * f - 4.0 * rd(a, -1) + rd(a, -2));
-    else if (form == 1) d = d2 * (-rd(a, -1) + rd(a, 1));
-    else if (form == 2) d = d2 * (-rd(a, 2) + 4.0 * rd(a, 1) - 3.0 * f);
+    d = use_b ? db : d;
+    d = use_c ? dc : d;
+    d = use_f ? df : d;
     s += d * x[a];
   }
   return -(vel * (s + f)) / r;

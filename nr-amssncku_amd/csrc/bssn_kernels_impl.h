@@ -30,6 +30,22 @@ __device__ __forceinline__ bool interior_index(const Box& box, int& i, int& j, i
 }
 inline EdgeShells edge_shells(const Box& b) { return edge_shells_of(b.n, b.lo, b.hi, b.mlo); }
 
+// Every point of the shell lies on the outermost layer of a face in mask
+// (the on_faces bits): each of its slabs is one point deep along its axis,
+// at index 0 of a lower face or n - 1 of an upper one in mask.
+inline bool shell_on_faces(const Shell& sh, const int n[3], int mask) {
+  // slab s: axis 2 - s / 2 (z, z, y, y, x, x), lower slab for even s
+  for (int s = 0; s < 6; ++s) {
+    if (sh.cnt[s] == 0) continue;
+    const int a = 2 - s / 2;
+    const int lo = (s & 1) ? sh.IH[a] : sh.OL[a], hi = (s & 1) ? sh.OH[a] : sh.IL[a];
+    if (hi - lo != 1) return false;
+    const bool on = (s & 1) ? (lo == n[a] - 1 && ((mask >> a) & 1)) : (lo == 0 && ((mask >> (3 + a)) & 1));
+    if (!on) return false;
+  }
+  return true;
+}
+
 // Whether the LDS boundary kernels may take the order-1 and order-2 shells
 // of this box: an even row pitch (the direct loads move 16 bytes from even
 // indices), no Sommerfeld or copy face on them (a physical lower face has
@@ -150,19 +166,58 @@ rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut a
   if (!edge_index(sh, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
   if (bad_input(in, p)) *flag = 1;
-  if (ORD == 0 && !CO && on_faces(sg.copy, box.n, i, j, k)) return;
+  // without the residuals, the points of the copy faces need nothing and
+  // those of the Sommerfeld faces take sommer_face_kernel
+  if (ORD == 0 && !CO && on_faces(sg.copy | sg.faces, box.n, i, j, k)) return;
   EdgeEnv<ORD> E(in, in, C, i, j, k, box.n, box.mlo, box.sy, box.sz);
   SommerRk4Sink<EdgeEnv<ORD>, Rk4SinkT<STAGE>> S{
       {u0, acc, out, aux, p, R.c, R.tinny, {}}, E, sg, sommer_point(sg, box.n, i, j, k)};
-  // On a Sommerfeld face every field's RHS is replaced before the update, so
-  // unless the residuals are wanted the RHS algebra there is skipped (face
-  // points lie on the outermost layer: stencil order 0).
-  if (ORD == 0 && !CO && S.on) {
+  bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  S.finish();
+}
+
+// The RK4 sub-step on the Sommerfeld-face points of the outermost layer
+// when no residuals are wanted: every field's RHS there is the boundary
+// condition's (sommerfeld_rout.f90:376-440, applied to the stage's RHS
+// before rungekutta4_rout, bssn_class.C:3106-3117), so the point algebra is
+// skipped.  All 24 boundary RHS are formed before the first store, and all
+// U0 and accumulator values loaded together, so a point waits for memory
+// about twice rather than once per field.
+// With test_nan the kernel also applies the reference's NaN test to the
+// shell's points (the launcher then runs no other kernel on them).
+template <int STAGE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) sommer_face_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
+                                                          AuxOut aux, Box box, Shell sh, Rk4Param R, SommerGeom sg,
+                                                          int* flag, bool test_nan) {
+  int i, j, k;
+  if (!edge_index(sh, i, j, k)) return;
+  const long p = i + j * box.sy + k * box.sz;
+  if (test_nan && bad_input(in, p)) *flag = 1;
+  if (on_faces(sg.copy, box.n, i, j, k) || !on_faces(sg.faces, box.n, i, j, k)) return;
+  const int q[3] = {i, j, k};
+  double kv[NFIELD], uv[NFIELD], av[NFIELD];
 #pragma unroll
-    for (int v = 0; v < NFIELD; ++v) S.put(v, 0.0);
-  } else {
-    bssn_rhs_point<GAUGE, CO, false, false>(E, S);
+  for (int v = 0; v < NFIELD; ++v) {
+    // field v m points along axis a; index -1 is index 0 times the field's
+    // parity (the one mirrored layer below a symmetry plane), taken without
+    // a branch so that the reads of all fields can be in flight together
+    auto rd = [&](int a, int m) {
+      int c[3] = {q[0], q[1], q[2]};
+      const int t = c[a] + m, neg = t >> 31;
+      c[a] = t ^ neg;
+      const double x = in.p[v][c[0] + c[1] * box.sy + c[2] * box.sz];
+      return (x + 0.0) * (neg ? (double)parity(v, a) : 1.0);
+    };
+    kv[v] = sommer_rhs(sg, box.n, q, prop_speed(v), rd);
   }
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) {
+    uv[v] = u0.p[v][p];
+    av[v] = STAGE > 0 ? acc.p[v][p] : 0.0;
+  }
+  Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) S.put_loaded(v, kv[v], uv[v], av[v]);
   S.finish();
 }
 
@@ -266,22 +321,37 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
         continue;
       }
       const unsigned gr = blocks_for(sh.size(), 256);
+      // Without the residuals, an outermost layer made of Sommerfeld and
+      // copy faces only needs the face kernel.
+      const bool faces_only = o == 0 && !want_res && (sg.faces | sg.copy) != 0 &&
+                              shell_on_faces(sh, box.n, sg.faces | sg.copy);
+      if (!faces_only) {
 #define AMSS_EDGE(ST, CO)                                                                                      \
   do {                                                                                                         \
     if (o == 0) rk4_edge_kernel<G, ST, CO, 0><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, C, R, flag, sg); \
     else if (o == 1) rk4_edge_kernel<G, ST, CO, 1><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, C, R, flag, sg); \
     else rk4_edge_kernel<G, ST, CO, 2><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, C, R, flag, sg);       \
   } while (0)
-      switch (R.stage) {
-        case 0:
-          if (want_res) AMSS_EDGE(0, true);
-          else AMSS_EDGE(0, false);
-          break;
-        case 1: AMSS_EDGE(1, false); break;
-        case 2: AMSS_EDGE(2, false); break;
-        default: AMSS_EDGE(3, false); break;
-      }
+        switch (R.stage) {
+          case 0:
+            if (want_res) AMSS_EDGE(0, true);
+            else AMSS_EDGE(0, false);
+            break;
+          case 1: AMSS_EDGE(1, false); break;
+          case 2: AMSS_EDGE(2, false); break;
+          default: AMSS_EDGE(3, false); break;
+        }
 #undef AMSS_EDGE
+      }
+      if (o == 0 && !want_res && (faces_only || sg.faces != 0)) {
+        const bool nan = faces_only;
+        switch (R.stage) {
+          case 0: sommer_face_kernel<0><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
+          case 1: sommer_face_kernel<1><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
+          case 2: sommer_face_kernel<2><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
+          default: sommer_face_kernel<3><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
+        }
+      }
     }
   }
 }
