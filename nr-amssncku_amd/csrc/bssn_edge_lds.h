@@ -232,7 +232,10 @@ __device__ __forceinline__ EdgeTile edge_set_tile(const EdgeTileSet& T, int b) {
 
 // rungekutta4_rout (rungekutta4_rout.f90:110-128), the chi floor and, before
 // sub-step 3, enforce_ga on the points of the tiles, from their compact
-// right-hand sides: the epilogue of the fused interior kernel, per point.
+// right-hand sides: the epilogue of the fused interior kernel, per point and
+// field.  blockIdx.y is the point's work item (UpdateItems): the fields a
+// thread updates alone cost it one memory round trip, and the points of an
+// x-face, one cache line per point and field, get many threads in flight.
 template <int STAGE>
 __global__ void __launch_bounds__(kEdgeThreads) edge_update_kernel(FieldsIn u0, FieldsOut acc, FieldsOut out,
                                                                    AuxOut aux, Box box, EdgeTileSet T,
@@ -244,19 +247,24 @@ __global__ void __launch_bounds__(kEdgeThreads) edge_update_kernel(FieldsIn u0, 
   if (!live) return;
   const long p = q[0] + q[1] * box.sy + q[2] * box.sz;
   const double* k = K + (long)blockIdx.x * NFIELD * kEdgeThreads + threadIdx.x;
-  // every load first: an x-face point's values lie one per cache line, and
-  // a point's loads in flight together cost one latency, not 48
-  double kv[NFIELD], uv[NFIELD], av[NFIELD];
-#pragma unroll
-  for (int v = 0; v < NFIELD; ++v) {
-    kv[v] = k[v * kEdgeThreads];
-    uv[v] = u0.p[v][p];
-    av[v] = STAGE > 0 ? acc.p[v][p] : 0.0;
-  }
   Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+  const int y = (int)blockIdx.y;
+  if (STAGE < 3 && y == 0) {
+    // the 12 fields enforce_ga couples, every load first
+    double kv[12], uv[12], av[12];
 #pragma unroll
-  for (int v = 0; v < NFIELD; ++v) S.put_loaded(v, kv[v], uv[v], av[v]);
-  S.finish();
+    for (int s = 0; s < 12; ++s) {
+      kv[s] = k[(DXX + s) * kEdgeThreads];
+      uv[s] = u0.p[DXX + s][p];
+      av[s] = STAGE > 0 ? acc.p[DXX + s][p] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 12; ++s) S.put_loaded(DXX + s, kv[s], uv[s], av[s]);
+    S.finish();
+  } else {
+    const int v = UpdateItems<STAGE>::field(y);
+    S.put_loaded(v, k[v * kEdgeThreads], u0.p[v][p], STAGE > 0 ? acc.p[v][p] : 0.0);
+  }
 }
 
 template <int G, bool CO>

@@ -450,6 +450,20 @@ struct SommerRk4Sink {
   __device__ __forceinline__ void finish() { S.finish(); }
 };
 
+// The update's independent work items of one point: before sub-step 3 the
+// metric and A fields go together (enforce_ga couples them; item 0) and each
+// other field alone (items 1..12); in sub-step 3 every field alone.
+template <int STAGE>
+struct UpdateItems {
+  static constexpr int kCount = STAGE < 3 ? 1 + (NFIELD - 12) : NFIELD;
+  // the field of single-field item y (y >= 1 before sub-step 3)
+  __host__ __device__ static constexpr int field(int y) {
+    if (STAGE == 3) return y;
+    constexpr signed char f[NFIELD - 12] = {CHI, TRK, GAMX, GAMY, GAMZ, LAP, BETAX, BETAY, BETAZ, DTSFX, DTSFY, DTSFZ};
+    return f[y - 1];
+  }
+};
+
 // Same epilogue with the sub-step fixed at compile time: the interior kernels
 // use this form, since a runtime stage makes the compiler duplicate the whole
 // point computation per stage branch (and blows the register budget).

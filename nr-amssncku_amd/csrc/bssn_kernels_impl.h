@@ -180,27 +180,27 @@ rk4_edge_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut a
 // when no residuals are wanted: every field's RHS there is the boundary
 // condition's (sommerfeld_rout.f90:376-440, applied to the stage's RHS
 // before rungekutta4_rout, bssn_class.C:3106-3117), so the point algebra is
-// skipped.  All 24 boundary RHS are formed before the first store, and all
-// U0 and accumulator values loaded together, so a point waits for memory
-// about twice rather than once per field.
-// With test_nan the kernel also applies the reference's NaN test to the
-// shell's points (the launcher then runs no other kernel on them).
+// skipped.  blockIdx.y is the point's work item (UpdateItems): one field
+// alone, or before sub-step 3 the 12 fields enforce_ga couples, whose
+// boundary RHS are all formed and whose U0 and accumulator values are all
+// loaded before the first store.  With test_nan the kernel also applies the
+// reference's NaN test to the shell's points (the launcher then runs no
+// other kernel on them).
 template <int STAGE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) sommer_face_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
+__global__ void __launch_bounds__(256) sommer_face_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out,
                                                           AuxOut aux, Box box, Shell sh, Rk4Param R, SommerGeom sg,
                                                           int* flag, bool test_nan) {
   int i, j, k;
   if (!edge_index(sh, i, j, k)) return;
   const long p = i + j * box.sy + k * box.sz;
-  if (test_nan && bad_input(in, p)) *flag = 1;
+  const int y = (int)blockIdx.y;
+  if (y == 0 && test_nan && bad_input(in, p)) *flag = 1;
   if (on_faces(sg.copy, box.n, i, j, k) || !on_faces(sg.faces, box.n, i, j, k)) return;
   const int q[3] = {i, j, k};
-  double kv[NFIELD], uv[NFIELD], av[NFIELD];
-#pragma unroll
-  for (int v = 0; v < NFIELD; ++v) {
-    // field v m points along axis a; index -1 is index 0 times the field's
-    // parity (the one mirrored layer below a symmetry plane), taken without
-    // a branch so that the reads of all fields can be in flight together
+  // the boundary RHS of field v; index -1 is index 0 times the field's
+  // parity (the one mirrored layer below a symmetry plane), taken without a
+  // branch so that all the reads can be in flight together
+  auto rhs = [&](int v) {
     auto rd = [&](int a, int m) {
       int c[3] = {q[0], q[1], q[2]};
       const int t = c[a] + m, neg = t >> 31;
@@ -208,17 +208,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       const double x = in.p[v][c[0] + c[1] * box.sy + c[2] * box.sz];
       return (x + 0.0) * (neg ? (double)parity(v, a) : 1.0);
     };
-    kv[v] = sommer_rhs(sg, box.n, q, prop_speed(v), rd);
-  }
-#pragma unroll
-  for (int v = 0; v < NFIELD; ++v) {
-    uv[v] = u0.p[v][p];
-    av[v] = STAGE > 0 ? acc.p[v][p] : 0.0;
-  }
+    return sommer_rhs(sg, box.n, q, prop_speed(v), rd);
+  };
   Rk4SinkT<STAGE> S{u0, acc, out, aux, p, R.c, R.tinny, {}};
+  if (STAGE < 3 && y == 0) {
+    double kv[12], uv[12], av[12];
 #pragma unroll
-  for (int v = 0; v < NFIELD; ++v) S.put_loaded(v, kv[v], uv[v], av[v]);
-  S.finish();
+    for (int s = 0; s < 12; ++s) kv[s] = rhs(DXX + s);
+#pragma unroll
+    for (int s = 0; s < 12; ++s) {
+      uv[s] = u0.p[DXX + s][p];
+      av[s] = STAGE > 0 ? acc.p[DXX + s][p] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 12; ++s) S.put_loaded(DXX + s, kv[s], uv[s], av[s]);
+    S.finish();
+  } else {
+    const int v = UpdateItems<STAGE>::field(y);
+    const double kv = rhs(v), uv = u0.p[v][p], av = STAGE > 0 ? acc.p[v][p] : 0.0;
+    S.put_loaded(v, kv, uv, av);
+  }
 }
 
 inline dim3 interior_grid(const Box& b) {
@@ -345,12 +354,16 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
       }
       if (o == 0 && !want_res && (faces_only || sg.faces != 0)) {
         const bool nan = faces_only;
+#define AMSS_FACE(ST)                                                                                      \
+  sommer_face_kernel<ST><<<dim3(gr, UpdateItems<ST>::kCount), 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, \
+                                                                          flag, nan)
         switch (R.stage) {
-          case 0: sommer_face_kernel<0><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
-          case 1: sommer_face_kernel<1><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
-          case 2: sommer_face_kernel<2><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
-          default: sommer_face_kernel<3><<<gr, 256, 0, s>>>(in, u0, acc, out, aux, box, sh, R, sg, flag, nan); break;
+          case 0: AMSS_FACE(0); break;
+          case 1: AMSS_FACE(1); break;
+          case 2: AMSS_FACE(2); break;
+          default: AMSS_FACE(3); break;
         }
+#undef AMSS_FACE
       }
     }
   }
