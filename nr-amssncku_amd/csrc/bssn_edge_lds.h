@@ -14,7 +14,7 @@
 // kernel stores each field's right-hand side tile-compact (field v of tile b
 // at K[(b * NFIELD + v) * 256 + thread], coalesced whatever the slab's
 // orientation); the update kernel then applies rungekutta4_rout, the chi
-// floor and enforce_ga per point.  Fused, the per-point RK4 loads and
+// floor and enforce_ga, per point and field.  Fused, the per-point RK4 loads and
 // stores of an x-face -- one cache line per point -- would sit inside the
 // stencil kernel's barriers, which wait for every store before the next
 // field group.
@@ -48,8 +48,8 @@ struct EdgeLdsEnv : EdgeStencils<ORD, EdgeLdsEnv<ORD, NRM>> {
   bool bad = false;    // a NaN among the centre values of the first 21 fields (bssn_rhs.f90:110-113)
   double top[NFIELD];  // centre values the algebra reads before its first group
 
-  __device__ __forceinline__ EdgeLdsEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, const int q[3], const Box& box,
-                        const EdgeTile& t, lds_double* buf_, int tid_, bool live_)
+  __device__ __forceinline__ EdgeLdsEnv(const FieldsIn& F_, const FieldsIn& M_, const Coef& C_, const int q[3],
+                                        const Box& box, const EdgeTile& t, lds_double* buf_, int tid_, bool live_)
       : Base(C_, q[0], q[1], q[2], box.n, box.mlo), F(F_), M(M_), buf(buf_), tid(tid_), live(live_) {
     sy = (int)box.sy;
     sz = (int)box.sz;
