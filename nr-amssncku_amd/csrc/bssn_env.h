@@ -507,8 +507,9 @@ struct Rk4SinkT {
     }
   }
 
-  __device__ __forceinline__ void put(int v, double k) {
-    double u0v, av = 0.0;
+  // U0 and (STAGE > 0) ACC of field v at p: prefetched, or loaded now
+  __device__ __forceinline__ void u0_acc(int v, double& u0v, double& av) const {
+    av = 0.0;
     if (v == pv[0]) {
       u0v = pu[0], av = pa[0];
     } else if (v == pv[1]) {
@@ -517,20 +518,32 @@ struct Rk4SinkT {
       u0v = U0.ptr(v)[p];
       if (STAGE > 0) av = ACC.ptr(v)[p];
     }
+  }
+  // rungekutta4_rout at p for field v from its rhs k: the accumulator's new
+  // value (sub-steps 0-2) and the stage output (chi bounded below)
+  __device__ __forceinline__ void rk4_values(int v, double k, double u0v, double av, double& accv, double& u) const {
+    double kk = k;
+    accv = 0.0;
+    if (STAGE == 0)
+      accv = k;
+    else if (STAGE < 3)
+      accv = av + 2.0 * k;
+    else
+      kk = k + av;
+    u = u0v + c * kk;
+    if (v == CHI) u = u < tinny ? tinny : u;
+  }
+  __device__ __forceinline__ void put(int v, double k) {
+    double u0v, av;
+    u0_acc(v, u0v, av);
     put_loaded(v, k, u0v, av);
   }
   // the update of field v from its rhs k, U0 value u0v and (STAGE > 0)
   // accumulator value av
   __device__ __forceinline__ void put_loaded(int v, double k, double u0v, double av) {
-    double kk = k;
-    if (STAGE == 0)
-      ACC.ptr(v)[p] = k;
-    else if (STAGE < 3)
-      ACC.ptr(v)[p] = av + 2.0 * k;
-    else
-      kk = k + av;
-    double u = u0v + c * kk;
-    if (v == CHI) u = u < tinny ? tinny : u;
+    double accv, u;
+    rk4_values(v, k, u0v, av, accv, u);
+    if (STAGE < 3) ACC.ptr(v)[p] = accv;
     if (STAGE < 3 && v >= DXX && v <= AZZ)
       gA[v - DXX] = u;
     else

@@ -142,7 +142,7 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
   const long p = i + j * box.sy + k * box.sz;
   const int N = box.n[0] * box.n[1] * box.n[2];
   lds_double* t0 = (lds_double*)tiles;
-  LdsRk4<STAGE, kLdsTX, kLdsTY, TZ> X{
+  LdsRk4<STAGE, kLdsTX, kLdsTY, TZ, FieldsIn, FieldsOut, AuxOut, true> X{
       {in, in, C, t0, t0 + (kLdsBuffers / 2) * Tl::NPAD, p, (tx + 3) + Tl::LX * ((ty + 3) + Tl::LY * (tz + 3)),
        (int)((i0 - 3) + (j0 - 3) * box.sy + (k0 - 3) * box.sz), (int)box.sy, (int)box.sz, N, (int)threadIdx.x},
       {u0, acc, out, aux, p, R.c, R.tinny, {}}};

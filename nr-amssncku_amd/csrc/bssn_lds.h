@@ -186,6 +186,7 @@ struct LdsEnv {
       issue(group_field(g + 2), slot(g + 2));
       __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (!std::is_void_v<Self>) static_cast<Self*>(this)->sink_flush();
     ++g;
     if (v < 21) {
       const double x = cur()[lt];
@@ -261,10 +262,48 @@ struct LdsEnv {
 
 // The LDS environment and the RK4 epilogue as one object (passed to the point
 // algebra as both), so the environment can hand the epilogue its prefetches.
-template <int STAGE, int TX, int TY, int TZ, class FI = FieldsIn, class FO = FieldsOut, class AX = AuxOut>
-struct LdsRk4 : LdsEnv<TX, TY, TZ, FI, LdsRk4<STAGE, TX, TY, TZ, FI, FO, AX>>, Rk4SinkT<STAGE, FI, FO, AX> {
-  __device__ __forceinline__ void sink_rotate() { Rk4SinkT<STAGE, FI, FO, AX>::rotate(); }
-  __device__ __forceinline__ void prefetch(int v0, int v1) { Rk4SinkT<STAGE, FI, FO, AX>::prefetch(v0, v1); }
+// DEFER: the last put's stores wait for the next group boundary, where the
+// environment calls sink_flush after its barrier -- a store issued just
+// before a barrier would be waited for there (its s_waitcnt(0) waits for
+// every memory operation), one issued after it has a group's computation to
+// land.  Measured on one box against the immediate stores: the single-patch
+// kernel +0.9%, the tile-list kernel -1.4%, so only the former defers; and
+// sub-step 3, one store per field, gains nothing.
+template <int STAGE, int TX, int TY, int TZ, class FI = FieldsIn, class FO = FieldsOut, class AX = AuxOut,
+          bool DEFER = false>
+struct LdsRk4 : LdsEnv<TX, TY, TZ, FI, LdsRk4<STAGE, TX, TY, TZ, FI, FO, AX, DEFER>>, Rk4SinkT<STAGE, FI, FO, AX> {
+  using Sink = Rk4SinkT<STAGE, FI, FO, AX>;
+  static constexpr bool kDefer = DEFER && STAGE < 3;
+  int dv = -1;
+  double dacc, dout;
+  __device__ __forceinline__ void sink_rotate() { Sink::rotate(); }
+  __device__ __forceinline__ void prefetch(int v0, int v1) { Sink::prefetch(v0, v1); }
+  __device__ __forceinline__ void sink_flush() {
+    if (!kDefer || dv < 0) return;
+    if (STAGE < 3) Sink::ACC.ptr(dv)[Sink::p] = dacc;
+    if (!(STAGE < 3 && dv >= DXX && dv <= AZZ)) Sink::OUT.ptr(dv)[Sink::p] = dout;
+    dv = -1;
+  }
+  __device__ __forceinline__ void put(int v, double k) {
+    if (!kDefer) {
+      Sink::put(v, k);
+      return;
+    }
+    double u0v, av, accv, u;
+    Sink::u0_acc(v, u0v, av);
+    Sink::rk4_values(v, k, u0v, av, accv, u);
+    sink_flush();  // an earlier put since the last boundary
+    dv = v;
+    dacc = accv;
+    if (STAGE < 3 && v >= DXX && v <= AZZ)
+      Sink::gA[v - DXX] = u;
+    else
+      dout = u;
+  }
+  __device__ __forceinline__ void finish() {
+    sink_flush();
+    Sink::finish();
+  }
 };
 
 }  // namespace amss
