@@ -317,6 +317,13 @@ def main():
     stage_flops = perf.FLOPS_PER_PT_STAGE_RES if res_mode else (perf.FLOPS_PER_PT_RHS,) * 4
     flops = sum(f * pts * n for f, n in zip(stage_flops, nl))
     nbytes = sum(b * pts * n for b, n in zip(stage_bytes, nl))
+    # plain launches on frozen ghost zones run the ghost fill inside the
+    # fused kernel's launch in sub-steps 0, 1 and 3
+    fill_pts = (args.n + 6) ** 3 - args.n ** 3
+    fused_fill = variant == 1 and args.mode == "plain" and args.bc == "frozen"
+    fill_bytes = sum(perf.BYTES_PER_PT_GHOST_FILL * fill_pts * f * n
+                     for f, n in zip(perf.GHOST_FILL_STAGES, nl)) if fused_fill else 0
+    nbytes += fill_bytes
     ach_gbs = nbytes / ktime / 1e9
     ach_tf = flops / ktime / 1e12
     ai = sum(stage_flops) / sum(stage_bytes)
@@ -329,6 +336,7 @@ def main():
                 " (fused RHS + RK4 sub-step, stages 0-3)",
                 avg_launch_ms=round(ktime * 1e3 / launches, 4), launches=launches, timing=timing,
                 algorithmic_bytes_per_launch=int(nbytes / launches),
+                ghost_fill_bytes_per_launch=int(fill_bytes / launches),
                 algorithmic_flops_per_launch=int(flops / launches),
                 fp64=dict(achieved=round(ach_tf, 3), peak=perf.PEAK_FP64_TFLOPS, unit="TFLOP/s",
                           frac=round(ach_tf / perf.PEAK_FP64_TFLOPS, 4)),

@@ -458,6 +458,13 @@ int BssnLevel::Step(double dT, bool want_res) {
       Rk4Param R{it, variant_, cst[it], tinny};
       R.edge_k = edge_k_;
       R.edge_k_tiles = edge_k_tiles_;
+      // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap), whose
+      // frozen points already hold the same values unless the zone is deeper
+      // than the interior kernel's start (gw > 3); a 3-deep fill can run in
+      // the interior kernel's launch.
+      const bool frozen_fill = bc_ == AMSS_BC_FROZEN_GHOSTS && !(it == 2 && gw_ == 3);
+      if (frozen_fill && gw_ == 3) R.fill_w = 3;
+      const bool fused_fill = frozen_fill && rk4_fuses_fill(box, R);
       const bool edge = (bc_ != AMSS_BC_FROZEN_GHOSTS) || gw_ < 3;
       if (edge)
         if (int r = launch_edges(b, (int)pi, in, out, aux, R, want_res && it == 0)) return r;
@@ -469,11 +476,8 @@ int BssnLevel::Step(double dT, bool want_res) {
       if (edge)
         if (int r = join_edges()) return r;
       if (bc_ == AMSS_BC_FROZEN_GHOSTS) {
-        // Sub-step 2 writes the list sub-step 0 wrote (the pre/cor swap),
-        // whose frozen points already hold the same values unless the zone
-        // is deeper than the interior kernel's start (gw > 3).
         Faces w{{gw_, gw_, gw_}, {gw_, gw_, gw_}};
-        if (!(it == 2 && gw_ == 3))
+        if (frozen_fill && !fused_fill)
           launch_ghost_fill(list_in(b, StateList), list_out(b, out), box, w, it, tinny, dflag_, stream_);
       } else if (bc_ == AMSS_BC_COPY_LAYER) {
         // sommerfeld_rout(precor = 1) faces (sommerfeld_rout.f90:61-137)

@@ -269,4 +269,35 @@ AMSS_HD void edge_tile_point(const EdgeTile& E, int tid, int q[3], bool& live) {
     }
 }
 
+// Point t of the points of an nx x ny x nz array outside [3, n - 3) per axis
+// (z slabs whole, then y slabs over the inner z range, then x slabs over the
+// inner y and z ranges), in scalars only: t < ghost3_count(n).
+AMSS_HD long ghost3_count(int nx, int ny, int nz) {
+  return (long)nx * ny * nz - (long)(nx - 6) * (ny - 6) * (nz - 6);
+}
+AMSS_HD void ghost3_point(long t, int nx, int ny, int nz, int& i, int& j, int& k) {
+  const long zs = 3L * nx * ny, ys = 3L * nx * (nz - 6), xs = 3L * (ny - 6) * (nz - 6);
+  int x0 = 0, ex = nx, y0 = 0, ey = ny, z0 = 0;
+  if (t < 2 * zs) {  // z slabs
+    z0 = t < zs ? 0 : nz - 3;
+    t -= t < zs ? 0 : zs;
+  } else if ((t -= 2 * zs) < 2 * ys) {  // y slabs
+    y0 = t < ys ? 0 : ny - 3;
+    ey = 3;
+    z0 = 3;
+    t -= t < ys ? 0 : ys;
+  } else {  // x slabs
+    t -= 2 * ys;
+    x0 = t < xs ? 0 : nx - 3;
+    ex = 3;
+    y0 = 3;
+    ey = ny - 6;
+    z0 = 3;
+    t -= t < xs ? 0 : xs;
+  }
+  i = x0 + (int)(t % ex);
+  j = y0 + (int)((t / ex) % ey);
+  k = z0 + (int)(t / ((long)ex * ey));
+}
+
 }  // namespace amss

@@ -40,6 +40,11 @@ struct Rk4Param {
   double c;      // 0.5 dt, 0.5 dt, dt, dt/6
   double tinny;  // chitiny - 1 (bssn_class.C:179)
   int edge_order = -1;  // boundary launches: the one stencil-order shell to run (-1: all three)
+  // interior launches: with fill_w = 3, the frozen-ghost fill of the points
+  // outside [3, n - 3) per axis runs as extra workgroups of the LDS kernel's
+  // launch where rk4_fuses_fill holds (the caller then launches no
+  // launch_ghost_fill); 0: no fill
+  int fill_w = 0;
   // right-hand sides of the LDS boundary kernel: edge_k_tiles tiles x NFIELD
   // x 256 doubles (null: the direct boundary kernel takes those shells)
   double* edge_k = nullptr;
@@ -56,6 +61,17 @@ void launch_rhs_g2(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out
                    const Coef& C, int co, int* flag, bool edge, hipStream_t s);
 // RHS fused with RK4 sub-step R.stage (residuals written for stage 0 when want_res).
 // sg: Sommerfeld faces, applied by the boundary kernel (edge) on their layers.
+// Whether the interior launch of launch_rk4 takes the LDS kernel (variant 1,
+// an even row pitch, at least one whole tile) and, with R.fill_w = 3, runs
+// the frozen-ghost fill in its launch: the fill's points are exactly those
+// outside the interior box [3, n - 3), which no tile writes.
+inline bool rk4_fuses_fill(const Box& box, const Rk4Param& R) {
+  if (R.fill_w != 3 || R.variant != 1 || (box.sy % 2) != 0) return false;
+  for (int a = 0; a < 3; ++a)
+    if (box.lo[a] != 3 || box.hi[a] != box.n[a] - 3 || box.hi[a] - box.lo[a] < 8) return false;
+  return true;
+}
+
 void launch_rk4(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& acc, const FieldsOut& out,
                 const AuxOut& aux, const Box& box, const Coef& C, const Rk4Param& R, int gauge, bool want_res,
                 int* flag, bool edge, hipStream_t s, bool interior = true, const SommerGeom& sg = SommerGeom{});

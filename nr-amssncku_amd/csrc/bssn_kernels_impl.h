@@ -22,6 +22,26 @@ __device__ __forceinline__ bool bad_input(const FI& F, long p) {
   return b;
 }
 
+// The frozen-ghost fill of point p after an RK4 sub-step: the sub-step's
+// base state U0, the NaN test on it, enforce_ga the next RHS applies to its
+// input (sub-steps 0-2) and the lower bound on chi -- the sequence rk4 ->
+// copy -> lowerboundset -> enforce_ga the reference runs on such points
+// (bssn_class.C:3119-3156, 3415-3525).
+__device__ __forceinline__ void ghost_fill_point(const FieldsIn& u0, const FieldsOut& out, long p, int stage,
+                                                 double tinny, int* flag) {
+  double u[NFIELD];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) u[v] = u0.p[v][p];
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < 21; ++v) bad |= !(u[v] == u[v]);  // bssn_rhs.f90:110-113
+  if (bad) *flag = 1;
+  if (stage < 3) enforce_point(u + DXX, u + AXX);
+  u[CHI] = u[CHI] < tinny ? tinny : u[CHI];
+#pragma unroll
+  for (int v = 0; v < NFIELD; ++v) out.p[v][p] = u[v];
+}
+
 __device__ __forceinline__ bool interior_index(const Box& box, int& i, int& j, int& k) {
   i = box.lo[0] + blockIdx.x * kTileX + (threadIdx.x % kTileX);
   j = box.lo[1] + blockIdx.y * kTileY + (threadIdx.x / kTileX);
@@ -128,13 +148,27 @@ constexpr int kLdsTX = 8, kLdsTY = 8, kLdsTZ = 4;
 template <int GAUGE, int STAGE, bool CO, int TZ>
 __global__ void __launch_bounds__(kLdsTX * kLdsTY * TZ) __attribute__((amdgpu_waves_per_eu(AMSS_LDS_WAVES, AMSS_LDS_WAVES)))
 rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut aux, Box box,
-                                 Coef C, Rk4Param R, int* flag) {
+                                 Coef C, Rk4Param R, int* flag, long nfill) {
   using Tl = LdsTile<kLdsTX, kLdsTY, TZ>;
   __shared__ __attribute__((aligned(16))) double tiles[kLdsBuffers * Tl::NPAD];
+  const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
+  const int ntiles = gx * gy * ((box.hi[2] - box.lo[2]) / TZ);
+  if ((int)blockIdx.x >= ntiles) {
+    // the frozen-ghost fill of the nfill points outside [3, n - 3) of the
+    // array (nfill is 0 when the caller fills them), in the workgroups after
+    // the tiles (they start as the last tiles finish); its points lie
+    // outside every tile
+    const long t = (long)(blockIdx.x - ntiles) * blockDim.x + threadIdx.x;
+    if (t < nfill) {
+      int i, j, k;
+      ghost3_point(t, box.n[0], box.n[1], box.n[2], i, j, k);
+      ghost_fill_point(u0, out, i + j * box.sy + k * box.sz, STAGE, R.tinny, flag);
+    }
+    return;
+  }
   const int tx = threadIdx.x % kLdsTX, ty = tile_row((threadIdx.x / kLdsTX) % kLdsTY, kLdsTY),
             tz = threadIdx.x / (kLdsTX * kLdsTY);
-  const int gx = (box.hi[0] - box.lo[0]) / kLdsTX, gy = (box.hi[1] - box.lo[1]) / kLdsTY;
-  const int b = xcd_spread(blockIdx.x, gridDim.x);
+  const int b = xcd_spread(blockIdx.x, ntiles);
   const int i0 = box.lo[0] + (b % gx) * kLdsTX, j0 = box.lo[1] + ((b / gx) % gy) * kLdsTY;
   const int k0 = box.lo[2] + (b / (gx * gy)) * TZ;
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
@@ -272,18 +306,21 @@ void launch_rk4_gauge(const FieldsIn& in, const FieldsIn& u0, const FieldsOut& a
     const int T[3] = {kLdsTX, kLdsTY, tz};
     for (int a = 0; a < 3; ++a) core.hi[a] = box.lo[a] + (box.hi[a] - box.lo[a]) / T[a] * T[a];
     if (!box_empty(core)) {
-      const unsigned gr = (unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
-                          ((core.hi[2] - core.lo[2]) / tz);
+      const unsigned ntile = (unsigned)((core.hi[0] - core.lo[0]) / kLdsTX) * ((core.hi[1] - core.lo[1]) / kLdsTY) *
+                             ((core.hi[2] - core.lo[2]) / tz);
       const int nt = kLdsTX * kLdsTY * tz;
+      // the fused frozen-ghost fill (none unless rk4_fuses_fill)
+      const long nfill = rk4_fuses_fill(box, R) ? ghost3_count(box.n[0], box.n[1], box.n[2]) : 0;
+      const unsigned gr = ntile + (unsigned)((nfill + nt - 1) / nt);
 #define AMSS_LDS(TZ)                                                                                          \
   switch (R.stage) {                                                                                          \
     case 0:                                                                                                   \
-      if (want_res) rk4_lds_kernel<G, 0, true, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); \
-      else rk4_lds_kernel<G, 0, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag);         \
+      if (want_res) rk4_lds_kernel<G, 0, true, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag, nfill); \
+      else rk4_lds_kernel<G, 0, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag, nfill);         \
       break;                                                                                                  \
-    case 1: rk4_lds_kernel<G, 1, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break; \
-    case 2: rk4_lds_kernel<G, 2, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break; \
-    default: rk4_lds_kernel<G, 3, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag); break; \
+    case 1: rk4_lds_kernel<G, 1, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag, nfill); break; \
+    case 2: rk4_lds_kernel<G, 2, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag, nfill); break; \
+    default: rk4_lds_kernel<G, 3, false, TZ><<<gr, nt, 0, s>>>(in, u0, acc, out, aux, core, C, R, flag, nfill); break; \
   }
       if (tz == 8) {
         AMSS_LDS(8)

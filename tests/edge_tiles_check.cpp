@@ -12,7 +12,9 @@
 //    lies in the tile's window, and the window fits one LDS buffer;
 //  * the windows start on even x, hold whole 16-byte chunks and fit one
 //    LDS buffer, and the mirror image cells of a window below a symmetry
-//    plane take their values from cells inside it.
+//    plane take their values from cells inside it;
+//  * the frozen-ghost fill's enumeration (ghost3_point) gives every point
+//    outside [3, n - 3) once and no other.
 // Prints "ok <points>" or the first failure, and exits non-zero on failure.
 #include <cstdio>
 #include <cstdlib>
@@ -121,7 +123,32 @@ static int check_shape(const int n[3], const int mlo[3], long& points) {
   return 0;
 }
 
+static int check_ghost3(int nx, int ny, int nz) {
+  const int n[3] = {nx, ny, nz}, z[3] = {0, 0, 0};
+  std::vector<int> seen((long)nx * ny * nz, 0);
+  const long cnt = ghost3_count(nx, ny, nz);
+  for (long t = 0; t < cnt; ++t) {
+    int i, j, k;
+    ghost3_point(t, nx, ny, nz, i, j, k);
+    if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz) return fail("ghost fill point outside the array", n, z, i, j, k);
+    const bool inner = i >= 3 && i < nx - 3 && j >= 3 && j < ny - 3 && k >= 3 && k < nz - 3;
+    if (inner) return fail("ghost fill point inside the interior box", n, z, i, j, k);
+    if (seen[i + (long)nx * (j + (long)ny * k)]++) return fail("ghost fill point given twice", n, z, i, j, k);
+  }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const bool inner = i >= 3 && i < nx - 3 && j >= 3 && j < ny - 3 && k >= 3 && k < nz - 3;
+        if (!inner && !seen[i + (long)nx * (j + (long)ny * k)]) return fail("ghost point not filled", n, z, i, j, k);
+      }
+  return 0;
+}
+
 int main() {
+  for (int nx : {14, 15, 16, 22, 38, 134})
+    for (int ny : {14, 17, 30})
+      for (int nz : {14, 19, 22, 70})
+        if (check_ghost3(nx, ny, nz)) return 1;
   long points = 0;
   int shapes = 0;
   std::vector<int> sizes;
