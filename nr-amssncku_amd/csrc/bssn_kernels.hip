@@ -172,10 +172,22 @@ __global__ void __launch_bounds__(256) l2_parts_kernel(const L2Job* __restrict__
   const int nx = J.hi[0] - J.lo[0], ny = J.hi[1] - J.lo[1], nz = J.hi[2] - J.lo[2];
   const long N = (long)nx * ny * nz;
   double s = 0.0;
-  for (long t = blockIdx.x * 256L + threadIdx.x; t < N; t += (long)J.nparts * 256) {
-    const int i = J.lo[0] + (int)(t % nx), j = J.lo[1] + (int)((t / nx) % ny), k = J.lo[2] + (int)(t / ((long)nx * ny));
-    const double v = J.f[i + j * J.sy + k * J.sz];
-    s += v * v;
+  const long stride = (long)J.nparts * 256;
+  if (N + stride <= 0xFFFFFFFFL) {
+    // the same points in the same order, with 32-bit index arithmetic
+    const unsigned nxu = (unsigned)nx, nyu = (unsigned)ny, nxy = nxu * nyu;
+    for (unsigned t = blockIdx.x * 256u + threadIdx.x; t < (unsigned)N; t += (unsigned)stride) {
+      const unsigned r = t / nxu;
+      const int i = J.lo[0] + (int)(t - r * nxu), j = J.lo[1] + (int)(r % nyu), k = J.lo[2] + (int)(t / nxy);
+      const double v = J.f[i + j * J.sy + k * J.sz];
+      s += v * v;
+    }
+  } else {
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < N; t += stride) {
+      const int i = J.lo[0] + (int)(t % nx), j = J.lo[1] + (int)((t / nx) % ny), k = J.lo[2] + (int)(t / ((long)nx * ny));
+      const double v = J.f[i + j * J.sy + k * J.sz];
+      s += v * v;
+    }
   }
   red[threadIdx.x] = s;
   __syncthreads();
