@@ -318,7 +318,7 @@ def main():
     flops = sum(f * pts * n for f, n in zip(stage_flops, nl))
     nbytes = sum(b * pts * n for b, n in zip(stage_bytes, nl))
     # plain launches on frozen ghost zones run the ghost fill inside the
-    # fused kernel's launch in sub-steps 0, 1 and 3
+    # fused kernel's launch in sub-steps 0 and 1
     fill_pts = (args.n + 6) ** 3 - args.n ** 3
     fused_fill = variant == 1 and args.mode == "plain" and args.bc == "frozen"
     fill_bytes = sum(perf.BYTES_PER_PT_GHOST_FILL * fill_pts * f * n

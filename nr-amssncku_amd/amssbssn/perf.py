@@ -19,8 +19,9 @@ also writes the 7 residual fields: BYTES_PER_PT_STAGE_RES.
 BYTES_PER_PT_GHOST_FILL are the compulsory bytes of the frozen-ghost fill
 per ghost point (read the 24 base-state fields, write the 24 outputs).  On
 a level patch with 3 ghost planes and plain launches the fill runs in the
-same launch as sub-steps 0, 1 and 3 (sub-step 2 writes the buffer sub-step
-0 filled), over the (n + 6)^3 - n^3 ghost points: GHOST_FILL_STAGES.
+same launch as sub-steps 0 and 1 (sub-step 2 writes the buffer sub-step 0
+filled; sub-step 3's fill is a launch of its own), over the
+(n + 6)^3 - n^3 ghost points: GHOST_FILL_STAGES.
 """
 FLOPS_PER_PT_RHS = 7596
 FLOPS_PER_PT_RHS_CO0 = 8492
@@ -28,7 +29,7 @@ BYTES_PER_PT_STAGE = (72 * 8, 120 * 8, 120 * 8, 96 * 8)
 BYTES_PER_PT_STAGE_RES = (79 * 8, 120 * 8, 120 * 8, 96 * 8)
 FLOPS_PER_PT_STAGE_RES = (FLOPS_PER_PT_RHS_CO0, FLOPS_PER_PT_RHS, FLOPS_PER_PT_RHS, FLOPS_PER_PT_RHS)
 BYTES_PER_PT_GHOST_FILL = 48 * 8
-GHOST_FILL_STAGES = (1, 1, 0, 1)
+GHOST_FILL_STAGES = (1, 1, 0, 0)
 
 # MI355X published peaks
 PEAK_HBM_GBS = 8000.0        # HBM3E, GB/s

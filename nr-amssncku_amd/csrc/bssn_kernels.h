@@ -64,9 +64,11 @@ void launch_rhs_g2(const FieldsIn& in, const FieldsIn& mat, const FieldsOut& out
 // Whether the interior launch of launch_rk4 takes the LDS kernel (variant 1,
 // an even row pitch, at least one whole tile) and, with R.fill_w = 3, runs
 // the frozen-ghost fill in its launch: the fill's points are exactly those
-// outside the interior box [3, n - 3), which no tile writes.
+// outside the interior box [3, n - 3), which no tile writes.  Sub-steps 0
+// and 1 only: sub-step 3's kernel, which keeps fewer values in flight, ran
+// longer by more than the separate fill launch takes.
 inline bool rk4_fuses_fill(const Box& box, const Rk4Param& R) {
-  if (R.fill_w != 3 || R.variant != 1 || (box.sy % 2) != 0) return false;
+  if (R.fill_w != 3 || R.variant != 1 || R.stage > 1 || (box.sy % 2) != 0) return false;
   for (int a = 0; a < 3; ++a)
     if (box.lo[a] != 3 || box.hi[a] != box.n[a] - 3 || box.hi[a] - box.lo[a] < 8) return false;
   return true;

@@ -158,11 +158,13 @@ rk4_lds_kernel(FieldsIn in, FieldsIn u0, FieldsOut acc, FieldsOut out, AuxOut au
     // array (nfill is 0 when the caller fills them), in the workgroups after
     // the tiles (they start as the last tiles finish); its points lie
     // outside every tile
-    const long t = (long)(blockIdx.x - ntiles) * blockDim.x + threadIdx.x;
-    if (t < nfill) {
-      int i, j, k;
-      ghost3_point(t, box.n[0], box.n[1], box.n[2], i, j, k);
-      ghost_fill_point(u0, out, i + j * box.sy + k * box.sz, STAGE, R.tinny, flag);
+    if constexpr (STAGE < 2) {  // rk4_fuses_fill: sub-steps 0 and 1
+      const long t = (long)(blockIdx.x - ntiles) * blockDim.x + threadIdx.x;
+      if (t < nfill) {
+        int i, j, k;
+        ghost3_point(t, box.n[0], box.n[1], box.n[2], i, j, k);
+        ghost_fill_point(u0, out, i + j * box.sy + k * box.sz, STAGE, R.tinny, flag);
+      }
     }
     return;
   }
